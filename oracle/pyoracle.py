@@ -1,0 +1,217 @@
+"""ctypes binding of the CPU FP64 oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() as the
+checker, and by bench.py's cpu_baseline leg.  The product package never
+imports this module.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+
+class OraParams(C.Structure):
+    _fields_ = [
+        ("dim", C.c_int), ("n", C.c_int * 3), ("modes", C.c_int), ("precision", C.c_int),
+        ("loading", C.c_int), ("athermal_correction", C.c_int), ("softening", C.c_int),
+        ("seed", C.c_int),
+        ("E", C.c_double), ("nu", C.c_double), ("double_nu", C.c_int),
+        ("q0_mean", C.c_double), ("q0_std", C.c_double), ("gamma0", C.c_double),
+        ("temperature", C.c_double), ("trial_frequency", C.c_double),
+        ("lx", C.c_double), ("ly", C.c_double), ("lz", C.c_double),
+        ("strainrate", C.c_double), ("strainincrement", C.c_double),
+        ("perm_soft", C.c_double), ("temp_soft", C.c_double), ("perm_cap", C.c_double),
+        ("temp_soft_barrier", C.c_double), ("athermal_threshold", C.c_double),
+    ]
+
+
+class OraStepInfo(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int), ("site", C.c_int), ("mode", C.c_int), ("n_transformed", C.c_int),
+        ("n_negative", C.c_longlong), ("q_chosen", C.c_double), ("total_rate", C.c_double),
+        ("dt_kmc", C.c_double), ("totime", C.c_double), ("epsavg", C.c_double * 6),
+        ("stress", C.c_double * 6), ("printed", C.c_int),
+    ]
+
+
+class OraMT(C.Structure):
+    _fields_ = [("mt", C.c_uint32 * 624), ("mti", C.c_int)]
+
+
+F_EPS0, F_SIGMA, F_Q0, F_DEPS, F_DEPS_LAST, F_PERM, F_TEMP, F_TLAST, F_Q, F_SOFT, F_EPSO = range(11)
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        P = C.POINTER
+        L.ora_mt_seed.argtypes = [P(OraMT), C.c_ulong]
+        L.ora_mt_next.argtypes = [P(OraMT)]
+        L.ora_mt_next.restype = C.c_uint32
+        L.ora_mt_uniform.argtypes = [P(OraMT)]
+        L.ora_mt_uniform.restype = C.c_double
+        L.ora_gaussian.argtypes = [P(OraMT), C.c_double]
+        L.ora_gaussian.restype = C.c_double
+        L.ora_fft3d.argtypes = [P(C.c_double), C.c_int, C.c_int, C.c_int, C.c_int]
+        L.ora_create.argtypes = [P(OraParams)]
+        L.ora_create.restype = C.c_void_p
+        L.ora_create_empty.argtypes = [P(OraParams)]
+        L.ora_create_empty.restype = C.c_void_p
+        L.ora_destroy.argtypes = [C.c_void_p]
+        L.ora_field.argtypes = [C.c_void_p, C.c_int]
+        L.ora_field.restype = P(C.c_double)
+        L.ora_scalars.argtypes = [C.c_void_p, P(C.c_double), P(C.c_double), P(C.c_double)]
+        L.ora_set_scalars.argtypes = [C.c_void_p, P(C.c_double), P(C.c_double), C.c_double]
+        L.ora_consts.argtypes = [C.c_void_p, P(C.c_double)]
+        L.ora_rng.argtypes = [C.c_void_p]
+        L.ora_rng.restype = P(OraMT)
+        L.ora_counters.argtypes = [C.c_void_p, P(C.c_longlong)]
+        L.ora_stress_solve.argtypes = [C.c_void_p]
+        L.ora_softening.argtypes = [C.c_void_p]
+        L.ora_act_energy.argtypes = [C.c_void_p]
+        L.ora_act_energy.restype = C.c_longlong
+        L.ora_total_rate.argtypes = [C.c_void_p]
+        L.ora_total_rate.restype = C.c_double
+        L.ora_select.argtypes = [C.c_void_p, C.c_double, P(C.c_int), P(C.c_int)]
+        L.ora_transform.argtypes = [C.c_void_p, C.c_int, C.c_int]
+        L.ora_elastic_energy.argtypes = [C.c_void_p]
+        L.ora_elastic_energy.restype = C.c_double
+        L.ora_step.argtypes = [C.c_void_p, P(OraStepInfo)]
+        L.ora_set_print.argtypes = [C.c_void_p, C.c_int, C.c_double]
+        _lib = L
+    return _lib
+
+
+# default physics of the reference input (STZ/input, STZ/bmg.h, STZ/init.c:60-68)
+REFERENCE_2D = dict(
+    dim=2, n=(512, 512, 1), modes=20, precision=0, loading=0, athermal_correction=0,
+    softening=1, seed=123, E=69.46, nu=0.365, double_nu=1, q0_mean=2.0, q0_std=0.25,
+    gamma0=0.14, temperature=300.0, trial_frequency=1e13, lx=0.7, ly=0.7, lz=1.0,
+    strainrate=1e-4, strainincrement=1e-4, perm_soft=13.0, temp_soft=39.0,
+    perm_cap=float(-np.log(0.6)), temp_soft_barrier=0.37, athermal_threshold=0.33,
+)
+
+
+def make_params(**kw):
+    d = dict(REFERENCE_2D)
+    d.update(kw)
+    p = OraParams()
+    for k, v in d.items():
+        if k == "n":
+            for i in range(3):
+                p.n[i] = int(v[i]) if i < len(v) else 1
+        else:
+            setattr(p, k, v)
+    return p
+
+
+class Oracle:
+    """One oracle state (the reference's global state, one rank)."""
+
+    def __init__(self, empty=False, **kw):
+        self.params = make_params(**kw)
+        L = lib()
+        self._h = (L.ora_create_empty if empty else L.ora_create)(C.byref(self.params))
+        if not self._h:
+            raise RuntimeError("ora_create failed")
+        c = np.zeros(16)
+        L.ora_consts(self._h, c.ctypes.data_as(C.POINTER(C.c_double)))
+        self.consts = c
+        self.N = int(c[10]); self.C = int(c[11]); self.S = int(c[12]); self.M = int(c[13])
+        self.dims = tuple(int(x) for x in self.params.n)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib().ora_destroy(h)
+            self._h = None
+
+    def field(self, which):
+        n = {F_EPS0: self.C * self.N, F_SIGMA: self.C * self.N, F_EPSO: self.C * self.N,
+             F_Q0: self.M * self.N, F_Q: self.M * self.N, F_DEPS: self.M * self.S * self.N,
+             F_DEPS_LAST: self.S * self.N}.get(which, self.N)
+        ptr = lib().ora_field(self._h, which)
+        return np.ctypeslib.as_array(ptr, shape=(n,))
+
+    def scalars(self):
+        e = np.zeros(6); s = np.zeros(6); t = C.c_double()
+        dp = C.POINTER(C.c_double)
+        lib().ora_scalars(self._h, e.ctypes.data_as(dp), s.ctypes.data_as(dp), C.byref(t))
+        return e, s, t.value
+
+    def set_scalars(self, epsavg, stress, totime):
+        dp = C.POINTER(C.c_double)
+        e = np.ascontiguousarray(epsavg, dtype=np.float64)
+        s = np.ascontiguousarray(stress, dtype=np.float64)
+        lib().ora_set_scalars(self._h, e.ctypes.data_as(dp), s.ctypes.data_as(dp), float(totime))
+
+    def rng(self):
+        return lib().ora_rng(self._h)
+
+    def counters(self):
+        o = np.zeros(8, dtype=np.int64)
+        lib().ora_counters(self._h, o.ctypes.data_as(C.POINTER(C.c_longlong)))
+        return o
+
+    def solve(self):
+        lib().ora_stress_solve(self._h)
+
+    def softening(self):
+        lib().ora_softening(self._h)
+
+    def act_energy(self):
+        return lib().ora_act_energy(self._h)
+
+    def total_rate(self):
+        return lib().ora_total_rate(self._h)
+
+    def select(self, dice):
+        s = C.c_int(); m = C.c_int()
+        rc = lib().ora_select(self._h, float(dice), C.byref(s), C.byref(m))
+        return rc, s.value, m.value
+
+    def transform(self, mode, site):
+        lib().ora_transform(self._h, int(mode), int(site))
+
+    def elastic_energy(self):
+        return lib().ora_elastic_energy(self._h)
+
+    def set_print(self, printperiod, endstrain):
+        lib().ora_set_print(self._h, int(printperiod), float(endstrain))
+
+    def step(self):
+        info = OraStepInfo()
+        rc = lib().ora_step(self._h, C.byref(info))
+        return rc, info
+
+
+def mt_stream(seed, n):
+    r = OraMT()
+    L = lib()
+    L.ora_mt_seed(C.byref(r), seed)
+    return np.array([L.ora_mt_next(C.byref(r)) for _ in range(n)], dtype=np.uint64)
+
+
+def fft3d(a, sign):
+    """Unnormalised complex FFT of a C-ordered array of shape (n2, n1, n0)."""
+    a = np.ascontiguousarray(a, dtype=np.complex128)
+    shape = a.shape
+    n = list(shape[::-1]) + [1] * (3 - len(shape))
+    buf = a.view(np.float64).copy()
+    rc = lib().ora_fft3d(buf.ctypes.data_as(C.POINTER(C.c_double)), n[0], n[1], n[2], sign)
+    if rc:
+        raise RuntimeError("ora_fft3d failed")
+    return buf.view(np.complex128).reshape(shape)

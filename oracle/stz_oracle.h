@@ -1,0 +1,115 @@
+/*
+ * stz_oracle.h -- TEST INFRASTRUCTURE ONLY.
+ *
+ * CPU FP64 restatement of the mesoscale STZ kinetic Monte Carlo step of the
+ * reference simulator ("Mesoscale simulation STZ model/", abbreviated STZ/).
+ * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
+ * load this library, and only as the checker or as the timed CPU baseline.
+ * The product (metallic_glasses_amd/) never links or calls it.
+ *
+ * Parity pins for every function are listed in the header of stz_oracle.c.
+ */
+#ifndef STZ_ORACLE_H
+#define STZ_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Same field meaning as stz_params in include/stz.h. */
+typedef struct ora_params {
+    int dim;                 /* 2 or 3 */
+    int n[3];                /* n[0] fastest index (reference j / NY), n[1], n[2] slowest */
+    int modes;               /* MODES */
+    int precision;           /* ignored: the oracle is FP64 */
+    int loading;             /* 0 tension (x-x), 1 simple shear (x-y) */
+    int athermal_correction; /* 0: subtract C0^-1(0, <s_yy>, <s_xy>) as STZ/evolution.c:91-97; 1: loading-consistent */
+    int softening;           /* Softening Flag */
+    int seed;
+    double E, nu;            /* input moduli */
+    int double_nu;           /* LOW_T: nu *= 2 (STZ/init.c:26) */
+    double q0_mean, q0_std;  /* Q0 ~ N(q0_mean, q0_std) */
+    double gamma0;           /* GAMMA0 */
+    double temperature;      /* T__K */
+    double trial_frequency;  /* TRIAL_FREQUENCY */
+    double lx, ly, lz;       /* voxel edge lengths in nm */
+    double strainrate, strainincrement;
+    double perm_soft, temp_soft, perm_cap, temp_soft_barrier;
+    double athermal_threshold;
+} ora_params;
+
+typedef struct ora_state ora_state;
+
+/* step kinds */
+enum { ORA_ELASTIC = 0, ORA_THERMAL = 1, ORA_ATHERMAL = 2 };
+
+typedef struct ora_step_info {
+    int kind;
+    int site;          /* thermal: selected voxel */
+    int mode;          /* thermal: selected mode */
+    int n_transformed; /* athermal: voxels transformed */
+    long long n_negative;
+    double q_chosen;   /* thermal: Q of the selected (mode, site) before the transform */
+    double total_rate;
+    double dt_kmc;     /* 1/R */
+    double totime;     /* clock after the step */
+    double epsavg[6];
+    double stress[6];
+    int printed;       /* 1 if the print block ran (and re-solved the stress) */
+} ora_step_info;
+
+/* --- RNG (GSL mt19937 + polar Gaussian) ------------------------------- */
+typedef struct ora_mt { uint32_t mt[624]; int mti; } ora_mt;
+void     ora_mt_seed(ora_mt *r, unsigned long s);
+uint32_t ora_mt_next(ora_mt *r);
+double   ora_mt_uniform(ora_mt *r);
+double   ora_mt_uniform_pos(ora_mt *r);
+double   ora_gaussian(ora_mt *r, double sigma);
+
+/* --- FFT: unnormalised complex transform over a 1-, 2- or 3-D grid --- */
+/* data: interleaved (re, im), index (i2*n1 + i1)*n0 + i0; sign -1 forward, +1 backward */
+int  ora_fft3d(double *data, int n0, int n1, int n2, int sign);
+
+/* --- state ------------------------------------------------------------- */
+ora_state *ora_create(const ora_params *p);   /* Init + Alloc + Energy_Init + DStrain + initial solve */
+ora_state *ora_create_empty(const ora_params *p); /* same sizes, fields zero, no RNG draws */
+void       ora_destroy(ora_state *s);
+
+/* fields: index into the state; returns pointer to N doubles (or M*N for per-mode) */
+enum {
+    ORA_F_EPS0 = 0,       /* [C][N] */
+    ORA_F_SIGMA = 1,      /* [C][N] */
+    ORA_F_Q0 = 2,         /* [M][N] */
+    ORA_F_DEPS = 3,       /* [M][S][N] independent components */
+    ORA_F_DEPS_LAST = 4,  /* [S][N] */
+    ORA_F_PERM = 5, ORA_F_TEMP = 6, ORA_F_TLAST = 7,
+    ORA_F_Q = 8,          /* [M][N] last ActEnergy output */
+    ORA_F_SOFTENING = 9,  /* [N] */
+    ORA_F_EPSO = 10       /* [C][N] total strain of the last solve */
+};
+double *ora_field(ora_state *s, int which);
+void    ora_scalars(ora_state *s, double *epsavg, double *stress, double *totime);
+void    ora_set_scalars(ora_state *s, const double *epsavg, const double *stress, double totime);
+void    ora_consts(ora_state *s, double *out16);
+ora_mt *ora_rng(ora_state *s);
+void    ora_counters(ora_state *s, long long *out8);
+
+/* --- hot-path pieces (each follows one reference function) ------------ */
+void   ora_stress_solve(ora_state *s);               /* InhomogeneousStressSolver(uflag=0) */
+void   ora_softening(ora_state *s);                  /* SofteningFactor */
+long long ora_act_energy(ora_state *s);              /* ActEnergy + Q<0 count */
+double ora_total_rate(ora_state *s);                 /* KMC rate loop + total */
+int    ora_select(ora_state *s, double dice, int *site, int *mode); /* normalise + prefix + search */
+void   ora_transform(ora_state *s, int mode, int site);            /* VoxelShearTransform */
+double ora_elastic_energy(ora_state *s);             /* ElasticEnergy (sum) */
+
+/* one iteration of the main loop (STZ/bmg.c:138-411); returns 0, or <0 on error, 1 on termination */
+int ora_step(ora_state *s, ora_step_info *info);
+void ora_set_print(ora_state *s, int printperiod, double endstrain);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
