@@ -1,0 +1,238 @@
+#!/usr/bin/env python3
+"""kMC events/s of the STZ kinetic Monte Carlo step on one MI355X.
+
+Workload (default): 128x128x128 voxels, FP64 plain GPU path, mu = 35 GPa, nu = 0.35,
+20 shear-transformation modes, gamma0 = 0.1, Q0 ~ N(2.0, 0.2) eV from seed 1234,
+T = 300 K, simple shear at 1e-4/s.  A step is one iteration of the reference main loop
+(rates over all modes and voxels, then a thermal event with selection, eigenstrain
+update and a full Green's-function stress solve, or an athermal avalanche, or a pure
+elastic increment).  Warm-up runs the elastic loading until W thermal events have
+happened; the timed region is exactly K steps; `value` = thermal events / wall time.
+
+Also reported: the mixed-precision path (FP32 FFT/Green/rates, FP64 sums, clock,
+eigenstrain and mean stress) from the same state with its stress-field error against
+FP64; the roofline of the dominant kernel (k_rates, HBM-bound) from HIP events on the
+library's stream; the CPU FP64 oracle timed on one host core on a bounded sample that
+starts from the GPU's timed-region state.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--grid", default="128,128,128")
+    ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--seed", type=int, default=1234)
+    return ap.parse_args()
+
+
+def rate_bytes_per_voxel(C, M, S, b):
+    # sigma (C), Q0 (M), transformation strains (S*M), previous strain (S) in the storage
+    # precision, plus Perm, Temp, tlast in FP64
+    return (C + M + S * M + S) * b + 3 * 8
+
+
+def snapshot(g, mg):
+    st = {f: g.download(f) for f in (mg.F_EPS0, mg.F_SIGMA, mg.F_Q0, mg.F_DEPS, mg.F_DEPS_LAST,
+                                     mg.F_PERM, mg.F_TEMP, mg.F_TLAST)}
+    e, s, t = g.scalars()
+    return st, (e, s, t), g.rng_state(), g.print_state()
+
+
+def restore(g, snap, mg):
+    st, (e, s, t), rng, pr = snap
+    for f, a in st.items():
+        g.upload(f, a)
+    g.set_scalars(e, s, t)
+    g.set_rng_state(rng)
+    g.set_print(int(pr[1]), pr[2])
+    g.set_print_state(pr[0])
+
+
+def run_steps(g, k):
+    kinds = np.zeros(3, dtype=np.int64)
+    events = []
+    t0 = time.perf_counter()
+    for _ in range(k):
+        rc, r = g.step()
+        kinds[r.kind] += 1
+        if r.kind == 1:
+            events.append((r.site, r.mode))
+        if rc:
+            break
+    dt = time.perf_counter() - t0
+    return dt, kinds, events
+
+
+def main():
+    a = parse()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if a.gpus != 1 or world != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(every box has one GPU); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    import metallic_glasses_amd as mg
+
+    dims = tuple(int(x) for x in a.grid.split(","))
+    kw = mg.config_3d(n=dims, seed=a.seed, q0_std=0.2)
+    t_init = time.perf_counter()
+    g = mg.Stz(**kw)
+    g.set_print(5, 1.0)
+    init_s = time.perf_counter() - t_init
+    print(f"# init {dims} fp64: {init_s:.1f}s", file=sys.stderr, flush=True)
+
+    # warm-up: elastic loading until the first thermal events, then W events
+    nev = 0
+    nst = 0
+    while nev < a.warmup:
+        rc, r = g.step()
+        nst += 1
+        nev += r.kind == 1
+        if rc:
+            break
+    print(f"# warm-up: {nst} steps, {nev} thermal events", file=sys.stderr, flush=True)
+
+    snap = None
+    if not a.no_cpu or not a.no_mixed:
+        snap = snapshot(g, mg)
+
+    g.timing(True)
+    dt, kinds, events = run_steps(g, a.steps)
+    ms, n, sms, sn = g.timing_get()
+    N = g.N
+    C, M, S = g.C, g.M, g.S
+    ev_fp64 = kinds[1] / dt
+    k4_ms = ms[0] / max(n[0], 1)
+    k4_bytes = rate_bytes_per_voxel(C, M, S, 8) * N
+    achieved = k4_bytes / (k4_ms * 1e-3) / 1e9
+    solve_ms = ms[1] / max(n[1], 1)
+    print(f"# fp64: {a.steps} steps in {dt:.3f}s kinds(el,th,ath)={kinds.tolist()} "
+          f"k_rates {k4_ms:.3f} ms ({achieved:.0f} GB/s) solve {solve_ms:.3f} ms "
+          f"per-kernel {np.round(sms[:5] / np.maximum(sn[:5], 1), 4).tolist()}", file=sys.stderr, flush=True)
+
+    out = {
+        "metric": "kMC events/s on 1 GPU, mixed vs FP64 at fixed stress err; FFT HBM roofline %",
+        "value": round(float(ev_fp64), 3),
+        "unit": "events/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(dt * 1e3 / a.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded MT19937 initial state, as the reference)",
+        "config": {"workload": f"STZ kMC {dims[0]}x{dims[1]}x{dims[2]} FP64, M=20, simple shear, seed {a.seed}",
+                   "grid": list(dims), "modes": M, "precision": "fp64", "parallelism": "single GPU"},
+        "steps_by_kind": {"elastic": int(kinds[0]), "thermal": int(kinds[1]), "athermal": int(kinds[2])},
+        "roofline": {"bound": "hbm", "kernel": "k_rates", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "bytes_per_launch": int(k4_bytes), "avg_ms": round(k4_ms, 5)},
+        "solve_ms": round(solve_ms, 5),
+    }
+    # FFT roofline: algorithmic bytes of the five solve kernels
+    b = 8
+    h0 = dims[0] // 2 + 1
+    spec = C * dims[1] * dims[2] * h0 * 2 * b
+    real = C * N * b
+    fft_bytes = [real + spec, 2 * spec, 2 * spec, 2 * spec, spec + real + C * N * 8]
+    fft = []
+    names = ["k_fwd_x", "k_ypass_fwd", "k_fused", "k_ypass_inv", "k_inv_x"]
+    for i in range(5):
+        t = sms[i] / max(sn[i], 1)
+        gbs = fft_bytes[i] / (t * 1e-3) / 1e9 if t > 0 else 0.0
+        fft.append({"kernel": names[i], "avg_ms": round(t, 5), "GB/s": round(gbs, 1),
+                    "frac": round(gbs / HBM_PEAK_GBS, 4)})
+    out["fft_roofline"] = fft
+
+    if not a.no_mixed and snap is not None:
+        kw_m = dict(kw, precision=mg.MIXED)
+        ref = mg.Stz(init_state=False, **kw)
+        restore(ref, snap, mg)
+        ref.stress_solve()
+        sref = ref.download(mg.F_SIGMA)
+        ref.close()
+        gm = mg.Stz(init_state=False, **kw_m)
+        restore(gm, snap, mg)
+        gm.stress_solve()
+        smix = gm.download(mg.F_SIGMA)
+        err = float(np.linalg.norm(smix - sref) / np.linalg.norm(sref))
+        restore(gm, snap, mg)
+        gm.timing(True)
+        dtm, kinds_m, events_m = run_steps(gm, a.steps)
+        msm, nm, _, _ = gm.timing_get()
+        k4m = msm[0] / max(nm[0], 1)
+        kb = rate_bytes_per_voxel(C, M, S, 4) * N
+        same = 0
+        for x, y in zip(events, events_m):
+            if x != y:
+                break
+            same += 1
+        out["mixed"] = {"value": round(float(kinds_m[1] / dtm), 3), "unit": "events/s",
+                        "stress_rel_err": err, "ms_per_step": round(dtm * 1e3 / a.steps, 4),
+                        "speedup_vs_fp64": round(float(kinds_m[1] / dtm) / ev_fp64, 3),
+                        "k_rates_GBs": round(kb / (k4m * 1e-3) / 1e9, 1),
+                        "events_identical_to_fp64": same}
+        gm.close()
+        print(f"# mixed: {out['mixed']}", file=sys.stderr, flush=True)
+
+    if not a.no_cpu and snap is not None:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle as po
+        cores = min(os.cpu_count() or 1, int(os.environ.get("STZ_CPU_THREADS", "16")))
+        cores = po.set_threads(cores)
+        o = po.Oracle(empty=True, **kw)
+        st, (e, s, t), rng, pr = snap
+        for f_g, f_o in [(mg.F_EPS0, po.F_EPS0), (mg.F_SIGMA, po.F_SIGMA), (mg.F_Q0, po.F_Q0),
+                         (mg.F_DEPS, po.F_DEPS), (mg.F_DEPS_LAST, po.F_DEPS_LAST), (mg.F_PERM, po.F_PERM),
+                         (mg.F_TEMP, po.F_TEMP), (mg.F_TLAST, po.F_TLAST)]:
+            o.field(f_o)[:] = st[f_g]
+        o.set_scalars(e, s, t)
+        r = o.rng()
+        for i in range(624):
+            r.contents.mt[i] = int(rng[i])
+        r.contents.mti = int(rng[624])
+        o.set_print(int(pr[1]), pr[2])
+        po.lib().ora_set_strainhist(o._h, float(pr[0]))
+        cev = 0
+        csteps = 0
+        cmatch = 0
+        t0 = time.perf_counter()
+        while time.perf_counter() - t0 < a.cpu_seconds or cev == 0:
+            rc, info = o.step()
+            csteps += 1
+            if info.kind == 1:
+                if cev < len(events) and (info.site, info.mode) == events[cev] and cmatch == cev:
+                    cmatch += 1
+                cev += 1
+            if rc:
+                break
+        cdt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": round(cev / cdt, 5), "unit": "events/s", "cores": cores, "kind": "port",
+                               "sample": f"{csteps} steps ({cev} thermal events) of the same {dims[0]}^3 FP64 run "
+                                         f"from the GPU's timed-region start state, {cdt:.1f}s on {cores} threads",
+                               "events_matching_gpu": cmatch}
+        print(f"# cpu: {out['cpu_baseline']}", file=sys.stderr, flush=True)
+
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

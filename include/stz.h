@@ -1,0 +1,201 @@
+/*
+ * stz.h -- C-ABI of the MI355X-native STZ kinetic Monte Carlo step.
+ *
+ * The reference simulator ("Mesoscale simulation STZ model/", abbreviated STZ/)
+ * has no plugin API: its hot path sits behind plain C prototypes acting on
+ * module globals (STZ/bmg.h:149-152, 295-311) and aborts the MPI job on error.
+ * This header replaces those prototypes with an explicit context, plain
+ * pointers and sizes, and negative return codes instead of MPI_Abort.
+ *
+ * Conventions
+ *   - Every function returns STZ_OK (0) or a negative code.
+ *   - Host buffers are caller-owned and read/written only during the call.
+ *   - One context per host thread; a context is not internally thread-safe.
+ *   - Doubles cross the ABI in both precisions.
+ *   - Field layouts (N = n[0]*n[1]*n[2] voxels, voxel index
+ *     v = (i2*n[1] + i1)*n[0] + i0, i0 fastest; C = 3 (2-D) or 6 (3-D)
+ *     tensor components xx, yy, xy  or  xx, yy, zz, yz, xz, xy;
+ *     S = 2 or 5 stored transformation-strain components
+ *     xx, xy  or  xx, yy, yz, xz, xy; M = modes):
+ *       EPS0 [C][N], SIGMA [C][N], Q0 [M][N], DEPS [M][S][N],
+ *       DEPS_LAST [S][N], PERM/TEMP/TLAST [N].
+ */
+#ifndef STZ_H
+#define STZ_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* return codes; -1 and -4 mirror the reference's MPI_Abort codes */
+#define STZ_OK 0
+#define STZ_E_SELECT (-1)      /* KMC failed to locate an activation mode (STZ/evolution.c:186-234) */
+#define STZ_E_HIP (-2)         /* HIP runtime error */
+#define STZ_E_ARG (-3)         /* bad argument / unsupported configuration */
+#define STZ_E_NAN (-4)         /* NaN in the stress solve (STZ/elastic.c:212-215) */
+#define STZ_E_UNSUPPORTED (-5) /* crystal-like MRO voxel drawn (out of scope) */
+
+#define STZ_FP64 0
+#define STZ_MIXED 1
+#define STZ_TENSION 0
+#define STZ_SHEAR 1
+
+/* fields for stz_upload / stz_download */
+#define STZ_F_EPS0 0
+#define STZ_F_SIGMA 1      /* download applies the pending uniform stress increment */
+#define STZ_F_Q0 2
+#define STZ_F_DEPS 3
+#define STZ_F_DEPS_LAST 4
+#define STZ_F_PERM 5
+#define STZ_F_TEMP 6
+#define STZ_F_TLAST 7
+#define STZ_F_MINQ 8       /* download only: per-voxel min over modes of Q from the last stz_rates */
+
+/* step kinds */
+#define STZ_STEP_ELASTIC 0
+#define STZ_STEP_THERMAL 1
+#define STZ_STEP_ATHERMAL 2
+
+/*
+ * Run parameters.  The reference fixes most of these at compile time
+ * (STZ/bmg.h:22-98, STZ/bmgFlags.h) or in the 17-field input file
+ * (STZ/io.c:4-101); here they are runtime values.
+ */
+typedef struct stz_params {
+    int dim;                 /* 2 (the reference) or 3 */
+    int n[3];                /* grid; n[0] fastest (reference NY), n[1] (reference NX), n[2] (1 in 2-D) */
+    int modes;               /* MODES, STZ/bmg.h:29 */
+    int precision;           /* STZ_FP64 | STZ_MIXED */
+    int loading;             /* STZ_TENSION (bmgFlags TENSION) | STZ_SHEAR (bmgFlags SHEAR) */
+    int athermal_correction; /* 0: subtract C0^-1(0, <s_yy>, <s_xy>) as STZ/evolution.c:91-97;
+                                1: subtract every mean-stress component except the loaded one */
+    int softening;           /* "Softening Flag", STZ/io.c:53-56 */
+    int seed;                /* "Random seed", STZ/io.c:33-36 */
+    double E, nu;            /* "Moduli(E, nu)", STZ/io.c:7-12 */
+    int double_nu;           /* 1: LOW_T doubling of nu, STZ/init.c:26 */
+    double q0_mean;          /* "Stress-free energy barrier", STZ/io.c:14-18 */
+    double q0_std;           /* 0.25 hard-coded at STZ/init.c:185 */
+    double gamma0;           /* GAMMA0, STZ/bmg.h:94 */
+    double temperature;      /* T__K, STZ/bmg.h:36 */
+    double trial_frequency;  /* TRIAL_FREQUENCY, STZ/bmg.h:42 */
+    double lx, ly, lz;       /* voxel edges (nm), STZ/bmg.h:84-90 */
+    double strainrate;       /* "Applied strain rate" */
+    double strainincrement;  /* "Strain increment" */
+    double perm_soft, temp_soft, perm_cap; /* STZ/init.c:60-68 */
+    double temp_soft_barrier;              /* 0.37 eV in tTemp, STZ/init.c:49 */
+    double athermal_threshold;             /* 0.33 eV, STZ/evolution.c:65 */
+} stz_params;
+
+typedef struct stz_step_result {
+    int kind;                /* STZ_STEP_* */
+    int site, mode;          /* thermal event */
+    int n_transformed;       /* athermal: voxels transformed */
+    long long n_negative;    /* #(mode, voxel) with Q < 0 (STZ/bmg.c:147-157) */
+    double q_chosen;         /* Q of the selected (mode, site), written to actQ.out */
+    double total_rate;       /* R */
+    double dt_kmc;           /* 1/R */
+    double totime;           /* clock after the step */
+    double epsavg[6];        /* EpsAvg after the step */
+    double stress[6];        /* Stress (<sigma>) after the step */
+    int printed;             /* the print block ran and re-solved the stress */
+    double soc_elast;        /* athermal: sum |DWork| of the transformed voxels (STZ/evolution.c:66-68) */
+} stz_step_result;
+
+typedef struct stz_ctx stz_ctx;
+
+/* Allocates device state for p (fields zero, tlast = -100*tTemp) on the current HIP device.
+   Replaces Init + FFT_MPI_Init + Grad + Alloc (STZ/init.c:6-140, 225-479). */
+int stz_create(const stz_params *p, stz_ctx **out);
+void stz_destroy(stz_ctx *ctx);
+const char *stz_last_error(const stz_ctx *ctx);
+
+/* Draws the initial state from the context's MT19937 stream in the reference order
+   (MRO map, Energy_Init, DStrain: STZ/init.c:355-396, 181-198; STZ/elastic.c:124-129)
+   and runs the initial stress solve (STZ/bmg.c:117-127). */
+int stz_init_state(stz_ctx *ctx);
+
+int stz_upload(stz_ctx *ctx, int field, const double *host);
+int stz_download(stz_ctx *ctx, int field, double *host);
+int stz_set_scalars(stz_ctx *ctx, const double epsavg[6], const double stress[6], double totime);
+int stz_get_scalars(stz_ctx *ctx, double epsavg[6], double stress[6], double *totime);
+
+/* InhomogeneousStressSolver(..., uflag=0) with uniform moduli (STZ/elastic.c:977-1066,
+   779-828): sigma field and <sigma> from EPS0 and EpsAvg.  Discards a pending uniform
+   stress increment (the reference's solve overwrites Sigma). */
+int stz_stress_solve(stz_ctx *ctx);
+
+/* SofteningFactor + ActEnergy + Q<0 count + KMC rate total
+   (STZ/init.c:711-739, 561-690; STZ/bmg.c:147-157; STZ/evolution.c:120-134). */
+int stz_rates(stz_ctx *ctx, long long *n_negative, double *total_rate);
+
+/* KMC normalise + prefix + search (STZ/evolution.c:136-224) on the rates of the last
+   stz_rates call: first event index i = mode*N + site whose cumulative rate reaches
+   dice*R.  q receives Q(mode, site). */
+int stz_select(stz_ctx *ctx, double dice, int *site, int *mode, double *q);
+
+/* VoxelShearTransform glass branch (STZ/elastic.c:361-520) at (site, mode) with the new
+   transformation strains of all M modes of the voxel (deps_new[m*S + s], already scaled
+   by 0.5*gamma0).  Uses the context clock for tlast. */
+int stz_apply_event(stz_ctx *ctx, int site, int mode, const double *deps_new);
+
+/* athermal_plasticity scan (STZ/evolution.c:52-77): number of voxels whose minimum Q
+   over modes is < 0 or within the threshold, in row-major order. */
+int stz_athermal_scan(stz_ctx *ctx, int *n);
+int stz_athermal_list(stz_ctx *ctx, int *sites, int *modes, int cap);
+/* applies the scanned transforms; deps_new holds n*M*S values, voxel by voxel in list order.
+   soc_elast receives sum |DWork| of the transforms (STZ/evolution.c:66-68). */
+int stz_athermal_apply(stz_ctx *ctx, const double *deps_new, double *soc_elast);
+
+/* Sigma[comp][*] += delta (STZ/evolution.c:293-296, 311-314, 344-356), applied lazily
+   before the next rate evaluation. */
+int stz_add_uniform_stress(stz_ctx *ctx, int comp, double delta);
+
+/* ElasticEnergy sum over voxels (STZ/elastic.c:533-547). */
+int stz_elastic_energy(stz_ctx *ctx, double *e);
+
+/* One iteration of the reference main loop (STZ/bmg.c:138-411) driven by the context's
+   own MT19937 stream: softening, ActEnergy, athermal or thermal/elastic KMC, loading,
+   print-period re-solve.  Returns 1 when the run terminates (end strain / instability). */
+int stz_set_print(stz_ctx *ctx, int printperiod, double endstrain);
+int stz_step(stz_ctx *ctx, stz_step_result *res);
+
+/* stz_step in two halves, for callers that write the print-block outputs from the state
+   before the print-step re-solve as the reference does (STZ/bmg.c:181-287):
+   stz_step_kmc runs the iteration up to the print block and sets res->printed = 1 when
+   the print block is due (it then returns 0 and the caller must call stz_step_print);
+   stz_step_print re-solves the stress (STZ/bmg.c:289-295) and checks termination. */
+int stz_step_kmc(stz_ctx *ctx, stz_step_result *res);
+int stz_step_print(stz_ctx *ctx, stz_step_result *res);
+
+/* Host RNG of the context (GSL mt19937 + gsl_ran_gaussian polar method). */
+int stz_rng_seed(stz_ctx *ctx, unsigned long seed);
+int stz_rng_uniform(stz_ctx *ctx, double *u);
+int stz_rng_gaussian(stz_ctx *ctx, double sigma, double *g);
+
+/* Counters: step, astep, kstep, estep, transforms, solves. */
+int stz_counters(stz_ctx *ctx, long long out[6]);
+
+/* MT19937 state (624 words + position) of the context stream, for checkpoints and for
+   handing a run over to another implementation mid-trajectory.  No counterpart in the
+   reference, whose RNG state is never saved. */
+int stz_rng_get_state(stz_ctx *ctx, unsigned int words[625]);
+int stz_rng_set_state(stz_ctx *ctx, const unsigned int words[625]);
+
+/* Run-control scalars: out[0] = last printed strain (strainhist[N_print-1], STZ/bmg.c:182),
+   out[1] = print period, out[2] = end strain. */
+int stz_get_print_state(stz_ctx *ctx, double out[3]);
+int stz_set_print_state(stz_ctx *ctx, double strainhist_last);
+
+/* Kernel timing with HIP events on the context stream (measurement hook; no counterpart
+   in the reference).  Groups: 0 rates (k_rates), 1 stress solve (all FFT/Green kernels),
+   2 selection, 3 transform apply, 4 athermal scan.  ms[g] = summed milliseconds,
+   n[g] = launches timed. */
+int stz_timing_enable(stz_ctx *ctx, int on);
+int stz_timing_get(stz_ctx *ctx, double ms[8], long long n[8]);
+/* per-kernel timing inside the stress solve: k_fwd_x, k_ypass(fwd), k_fused, k_ypass(inv), k_inv_x */
+int stz_timing_solve_get(stz_ctx *ctx, double ms[8], long long n[8]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

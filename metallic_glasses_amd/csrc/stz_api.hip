@@ -1,0 +1,867 @@
+// stz_api.hip -- the C-ABI (include/stz.h): context, state I/O, and the host
+// orchestration of one main-loop iteration of the reference (STZ/bmg.c:138-411).
+#include "stz_internal.h"
+
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <vector>
+
+namespace {
+
+// Init (STZ/init.c:6-75) with the compile-time constants as parameters
+void derive(stz_ctx *c)
+{
+    const stz_params &p = c->p;
+    StzConsts &k = c->k;
+    k.E = p.E;
+    k.nu = p.nu;
+    if (p.double_nu) k.nu *= 2;                                              // :26
+    k.mu = k.E / (2 + 2 * k.nu);                                             // :27
+    if (p.dim == 2) k.lambda = k.E * k.nu / (1 - k.nu * k.nu);               // :44
+    else k.lambda = k.E * k.nu / ((1 + k.nu) * (1 - 2 * k.nu));
+    k.alpha = (k.lambda + k.mu) / (k.lambda + 2.0 * k.mu);                   // elastic.c:786
+    k.kT = p.temperature * STZ_BOLZ * STZ_J_IN_EV;                           // bmg.h:73
+    k.dt = p.strainincrement / p.strainrate;                                 // :48
+    k.tTemp = 1 / (p.trial_frequency * std::exp(-p.temp_soft_barrier / k.kT)); // :49
+    k.voxel = p.lx * p.ly * p.lz;                                            // bmg.h:90
+    k.ratepre = k.voxel * p.trial_frequency;                                 // evolution.c:125
+    k.permsoft = p.perm_soft;
+    k.tempsoft = p.temp_soft;
+    k.permcap = p.perm_cap;
+    k.thr = p.athermal_threshold;
+    k.half_gamma0 = 0.5 * p.gamma0;                                          // elastic.c:127
+}
+
+// isotropicinv (STZ/elastic.c:522-531), the form used by the loading
+void isotropicinv(const stz_ctx *c, const double *st, double *e)
+{
+    const double lam = c->k.lambda, mu = c->k.mu;
+    if (c->C == 3) {
+        double tra = -lam / 2 / mu / (2 * lam + 2 * mu) * (st[0] + st[1]);
+        e[0] = tra + st[0] / 2 / mu;
+        e[1] = tra + st[1] / 2 / mu;
+        e[2] = st[2] / 2 / mu;
+    } else {
+        double tra = -lam / 2 / mu / (3 * lam + 2 * mu) * ((st[0] + st[1]) + st[2]);
+        for (int i = 0; i < 3; ++i) e[i] = tra + st[i] / 2 / mu;
+        for (int i = 3; i < 6; ++i) e[i] = st[i] / 2 / mu;
+    }
+}
+
+template <typename R>
+__global__ void k_fill(R *p, long long n, R v)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+template <typename R>
+__global__ void k_add(R *p, long long n, double d)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = p[i] + (R)d;
+}
+
+float elapsed(stz_ctx *ctx, int a, int b)
+{
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, ctx->tev[a], ctx->tev[b]) != hipSuccess) return 0.f;
+    return ms;
+}
+
+int dalloc(stz_ctx *ctx, void **p, size_t bytes)
+{
+    if (bytes == 0) bytes = 16;
+    STZ_HIP(hipMalloc(p, bytes));
+    STZ_HIP(hipMemsetAsync(*p, 0, bytes, ctx->stream));
+    return STZ_OK;
+}
+
+size_t field_count(const stz_ctx *c, int f)
+{
+    const size_t N = (size_t)c->N;
+    switch (f) {
+    case STZ_F_EPS0: case STZ_F_SIGMA: return (size_t)c->C * N;
+    case STZ_F_Q0: return (size_t)c->M * N;
+    case STZ_F_DEPS: return (size_t)c->M * c->S * N;
+    case STZ_F_DEPS_LAST: return (size_t)c->S * N;
+    case STZ_F_PERM: case STZ_F_TEMP: case STZ_F_TLAST: case STZ_F_MINQ: return N;
+    default: return 0;
+    }
+}
+
+// copy doubles into a device array of the storage precision
+int put_real(stz_ctx *ctx, void *dst, const double *src, size_t n, size_t offset = 0)
+{
+    if (ctx->rsz == 8) {
+        STZ_HIP(hipMemcpyAsync((double *)dst + offset, src, n * 8, hipMemcpyHostToDevice, ctx->stream));
+    } else {
+        std::vector<float> tmp(n);
+        for (size_t i = 0; i < n; ++i) tmp[i] = (float)src[i];
+        STZ_HIP(hipMemcpyAsync((float *)dst + offset, tmp.data(), n * 4, hipMemcpyHostToDevice, ctx->stream));
+        STZ_HIP(hipStreamSynchronize(ctx->stream));
+    }
+    return STZ_OK;
+}
+
+int get_real(stz_ctx *ctx, const void *src, double *dst, size_t n)
+{
+    if (ctx->rsz == 8) {
+        STZ_HIP(hipMemcpyAsync(dst, src, n * 8, hipMemcpyDeviceToHost, ctx->stream));
+        STZ_HIP(hipStreamSynchronize(ctx->stream));
+    } else {
+        std::vector<float> tmp(n);
+        STZ_HIP(hipMemcpyAsync(tmp.data(), src, n * 4, hipMemcpyDeviceToHost, ctx->stream));
+        STZ_HIP(hipStreamSynchronize(ctx->stream));
+        for (size_t i = 0; i < n; ++i) dst[i] = (double)tmp[i];
+    }
+    return STZ_OK;
+}
+
+// applies a pending FP64 uniform stress increment to the field now
+int flush_pending(stz_ctx *ctx)
+{
+    if (ctx->pend_comp < 0) return STZ_OK;
+    const long long n = ctx->N;
+    double *p = (double *)ctx->sigma + (long long)ctx->pend_comp * n;
+    hipLaunchKernelGGL((k_add<double>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, p, n, ctx->pend_delta);
+    STZ_CHECK_LAUNCH();
+    ctx->pend_comp = -1;
+    ctx->pend_delta = 0.0;
+    return STZ_OK;
+}
+
+int ensure_dnew(stz_ctx *ctx, size_t count)
+{
+    if (count <= ctx->dnew_cap) return STZ_OK;
+    if (ctx->dnew) STZ_HIP(hipFree(ctx->dnew));
+    size_t cap = count < 4096 ? 4096 : count;
+    STZ_HIP(hipMalloc(&ctx->dnew, cap * sizeof(double)));
+    ctx->dnew_cap = cap;
+    return STZ_OK;
+}
+
+int upload_soff(stz_ctx *ctx)
+{
+    STZ_HIP(hipMemcpyAsync(ctx->res + 16, ctx->soff, 6 * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    return STZ_OK;
+}
+
+void draw_voxel_deps(stz_ctx *ctx, double *out)
+{
+    // VoxelShearTransform redraw, STZ/elastic.c:508-512: mode by mode, components in
+    // stored order, each 0.5*GAMMA0 * gsl_ran_gaussian(r, 1.0)
+    const int MS = ctx->M * ctx->S;
+    for (int i = 0; i < MS; ++i) out[i] = ctx->k.half_gamma0 * ctx->rng.gaussian(1.0);
+}
+
+int solve(stz_ctx *ctx)
+{
+    ctx->pend_comp = -1;
+    ctx->pend_delta = 0.0;
+    for (int c = 0; c < 6; ++c) ctx->soff[c] = 0.0;
+    STZ_HIP(hipMemsetAsync(ctx->res + 15, 0, sizeof(double), ctx->stream));
+    int rc = stz_solve_launch(ctx);
+    if (rc) return rc;
+    STZ_HIP(hipMemcpyAsync(ctx->hres, ctx->res, 16 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    STZ_HIP(hipStreamSynchronize(ctx->stream));
+    int nan;
+    std::memcpy(&nan, ctx->hres + 15, sizeof(int));
+    if (nan) {
+        ctx->err = "NaN in the stress solve";
+        return STZ_E_NAN;
+    }
+    for (int c = 0; c < ctx->C; ++c) ctx->stress[c] = ctx->hres[c];
+    ctx->nsolve++;
+    if (ctx->timing) {
+        for (int i = 0; i < 6; ++i) {
+            ctx->sms[i] += elapsed(ctx, i, i + 1);
+            ctx->sn[i]++;
+        }
+        ctx->tms[1] += elapsed(ctx, 0, 6);
+        ctx->tn[1]++;
+    }
+    return STZ_OK;
+}
+
+int add_uniform(stz_ctx *ctx, int comp, double delta)
+{
+    if (ctx->prec == STZ_FP64) {
+        if (ctx->pend_comp >= 0) {
+            int rc = flush_pending(ctx);
+            if (rc) return rc;
+        }
+        ctx->pend_comp = comp;
+        ctx->pend_delta = delta;
+    } else {
+        ctx->soff[comp] += delta;
+    }
+    return STZ_OK;
+}
+
+int rates(stz_ctx *ctx, long long *nneg, double *R)
+{
+    int rc = stz_rates_launch(ctx);
+    if (rc) return rc;
+    ctx->pend_comp = -1;     // consumed by the rate pass
+    ctx->pend_delta = 0.0;
+    STZ_HIP(hipMemcpyAsync(ctx->hres + 8, ctx->res + 8, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    STZ_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->rate_total = ctx->hres[8];
+    if (ctx->timing) {
+        ctx->tms[0] += elapsed(ctx, 8, 9);
+        ctx->tn[0]++;
+        ctx->tms[5] += elapsed(ctx, 9, 10);
+        ctx->tn[5]++;
+    }
+    *R = ctx->hres[8];
+    *nneg = (long long)ctx->hres[9];
+    return STZ_OK;
+}
+
+int select_event(stz_ctx *ctx, double dice, int *site, int *mode, double *q)
+{
+    if (!(dice > 0.0)) {
+        ctx->err = "KMC failed to locate an activation mode";
+        return STZ_E_SELECT;
+    }
+    int rc = stz_select_launch(ctx, dice * ctx->rate_total, dice);
+    if (rc) return rc;
+    STZ_HIP(hipMemcpyAsync(ctx->hres + 10, ctx->res + 10, 3 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    STZ_HIP(hipStreamSynchronize(ctx->stream));
+    if (ctx->timing) {
+        ctx->tms[2] += elapsed(ctx, 11, 12);
+        ctx->tn[2]++;
+    }
+    if (ctx->hres[10] < 0) {
+        ctx->err = "KMC failed to locate an activation mode";
+        return STZ_E_SELECT;
+    }
+    *site = (int)ctx->hres[10];
+    *mode = (int)ctx->hres[11];
+    *q = ctx->hres[12];
+    return STZ_OK;
+}
+
+int apply_event(stz_ctx *ctx, int site, int mode, const double *dnew)
+{
+    const size_t ms = (size_t)ctx->M * ctx->S;
+    int rc = ensure_dnew(ctx, ms);
+    if (rc) return rc;
+    STZ_HIP(hipMemcpyAsync(ctx->dnew, dnew, ms * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    rc = stz_apply_launch(ctx, site, mode);
+    if (rc) return rc;
+    ctx->ntrans++;
+    return STZ_OK;
+}
+
+int athermal_scan(stz_ctx *ctx, int *n)
+{
+    int rc = stz_athermal_launch(ctx);
+    if (rc) return rc;
+    STZ_HIP(hipMemcpyAsync(ctx->hres + 13, ctx->res + 13, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    STZ_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->n_athermal = (int)ctx->hres[13];
+    *n = ctx->n_athermal;
+    return STZ_OK;
+}
+
+int athermal_apply(stz_ctx *ctx, const double *dnew, double *soc)
+{
+    const int n = ctx->n_athermal;
+    const size_t cnt = (size_t)n * ctx->M * ctx->S;
+    int rc = ensure_dnew(ctx, cnt);
+    if (rc) return rc;
+    if (n > 0) STZ_HIP(hipMemcpyAsync(ctx->dnew, dnew, cnt * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    rc = upload_soff(ctx);
+    if (rc) return rc;
+    rc = stz_athermal_apply_launch(ctx, n);
+    if (rc) return rc;
+    const int nb = (n + 255) / 256;
+    std::vector<double> part(nb > 0 ? nb : 1);
+    if (nb > 0) STZ_HIP(hipMemcpyAsync(part.data(), ctx->socp, nb * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    STZ_HIP(hipStreamSynchronize(ctx->stream));
+    double s = 0.0;
+    for (int i = 0; i < nb; ++i) s += part[i];
+    if (soc) *soc = s;
+    ctx->ntrans += n;
+    return STZ_OK;
+}
+
+// athermal error subtraction, STZ/evolution.c:91-97 (reference) or loading-consistent
+void athermal_correction(stz_ctx *ctx)
+{
+    double ts[6] = {0, 0, 0, 0, 0, 0}, te[6];
+    if (ctx->p.athermal_correction == 0) {
+        for (int c = 1; c < ctx->C; ++c) ts[c] = ctx->stress[c];
+    } else {
+        for (int c = 0; c < ctx->C; ++c) ts[c] = (c == ctx->lc) ? 0.0 : ctx->stress[c];
+    }
+    isotropicinv(ctx, ts, te);
+    for (int c = 0; c < ctx->C; ++c) ctx->epsavg[c] -= te[c];
+}
+
+// KMC loading after a thermal event (STZ/evolution.c:281-315)
+int load_thermal(stz_ctx *ctx, double t)
+{
+    const double sr = ctx->p.strainrate, mu = ctx->k.mu;
+    const int C = ctx->C, lc = ctx->lc;
+    double ts[6] = {0, 0, 0, 0, 0, 0}, te[6];
+    if (ctx->p.loading == STZ_SHEAR) {
+        for (int c = 0; c < C; ++c) ts[c] = (c == lc) ? 0 : ctx->stress[c];
+        isotropicinv(ctx, ts, te);
+        for (int c = 0; c < C; ++c)
+            if (c != lc) ctx->epsavg[c] += -te[c];
+        ctx->epsavg[lc] += sr * t;
+        ctx->stress[lc] += 2 * mu * sr * t;
+        return add_uniform(ctx, lc, 2 * mu * sr * t);
+    }
+    const double nu = ctx->k.nu, E = ctx->k.E;
+    for (int c = 1; c < C; ++c) ts[c] = ctx->stress[c];
+    isotropicinv(ctx, ts, te);
+    ctx->epsavg[0] += (sr * t * 1 - te[0]);
+    ctx->epsavg[1] += (sr * t * (-nu) - te[1]);
+    if (C == 3) {
+        ctx->epsavg[2] += (-te[2]);
+    } else {
+        ctx->epsavg[2] += (sr * t * (-nu) - te[2]);
+        for (int c = 3; c < 6; ++c) ctx->epsavg[c] += (-te[c]);
+    }
+    ctx->stress[0] += E * sr * t;
+    return add_uniform(ctx, 0, E * sr * t);
+}
+
+// pure elastic increment (STZ/evolution.c:321-358)
+int load_elastic(stz_ctx *ctx)
+{
+    const double sr = ctx->p.strainrate, inc = ctx->p.strainincrement, mu = ctx->k.mu;
+    const int C = ctx->C, lc = ctx->lc;
+    if (ctx->p.loading == STZ_SHEAR) {
+        ctx->epsavg[lc] += sr * 1;
+        ctx->stress[lc] += 2 * mu * inc;
+        return add_uniform(ctx, lc, 2 * mu * inc);
+    }
+    const double nu = ctx->k.nu, E = ctx->k.E;
+    ctx->epsavg[0] += (inc * 1);
+    ctx->epsavg[1] += (inc * (-nu));
+    if (C == 3) {
+        ctx->epsavg[2] += (inc * 0.0);
+    } else {
+        ctx->epsavg[2] += (inc * (-nu));
+        for (int c = 3; c < 6; ++c) ctx->epsavg[c] += (inc * 0.0);
+    }
+    ctx->stress[0] += E * inc;
+    return add_uniform(ctx, 0, E * inc);
+}
+
+bool terminated(const stz_ctx *ctx)
+{
+    if (std::fabs(ctx->epsavg[ctx->lc]) > ctx->endstrain) return true;                 // bmg.c:383,397
+    if (ctx->p.loading == STZ_TENSION && ctx->stress[0] < -0.5) return true;            // bmg.c:388
+    if (ctx->p.loading == STZ_SHEAR && std::fabs(ctx->stress[ctx->lc]) < STZ_EPSILON) return true; // bmg.c:402
+    return false;
+}
+
+}  // namespace
+
+// ==================================================================================
+extern "C" {
+
+int stz_create(const stz_params *p, stz_ctx **out)
+{
+    if (!p || !out) return STZ_E_ARG;
+    *out = nullptr;
+    if (p->dim != 2 && p->dim != 3) return STZ_E_ARG;
+    if (p->modes < 1 || p->modes > STZ_MAXM) return STZ_E_ARG;
+    if (p->precision != STZ_FP64 && p->precision != STZ_MIXED) return STZ_E_ARG;
+    if (p->loading != STZ_TENSION && p->loading != STZ_SHEAR) return STZ_E_ARG;
+    int n[3] = {p->n[0], p->n[1], p->dim == 2 ? 1 : p->n[2]};
+    if (!stz_solve_supported(p->dim, n, p->precision)) return STZ_E_ARG;
+    stz_ctx *ctx = new (std::nothrow) stz_ctx();
+    if (!ctx) return STZ_E_ARG;
+    ctx->p = *p;
+    ctx->p.n[2] = n[2];
+    ctx->dim = p->dim;
+    for (int a = 0; a < 3; ++a) ctx->n[a] = n[a];
+    ctx->C = p->dim == 2 ? 3 : 6;
+    ctx->S = p->dim == 2 ? 2 : 5;
+    ctx->M = p->modes;
+    ctx->N = (long long)n[0] * n[1] * n[2];
+    ctx->lc = p->loading == STZ_TENSION ? 0 : (ctx->C == 3 ? 2 : 5);
+    ctx->prec = p->precision;
+    ctx->rsz = p->precision == STZ_FP64 ? 8 : 4;
+    ctx->h0 = n[0] / 2 + 1;
+    const int per_line = (int)(128 / (2 * ctx->rsz));   // complex values per 128-byte line
+    ctx->hp = (ctx->h0 + per_line - 1) / per_line * per_line;
+    ctx->nblk = (ctx->N + STZ_BLK - 1) / STZ_BLK;
+    ctx->ngrp = ((long long)ctx->M * ctx->nblk + STZ_GRP - 1) / STZ_GRP;
+    derive(ctx);
+    ctx->rng.seed((unsigned long)p->seed);     // STZ/init.c:143-147 at rank 0
+
+    auto fail = [&](int rc) {
+        stz_destroy(ctx);
+        return rc;
+    };
+    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) return fail(STZ_E_HIP);
+    const size_t N = (size_t)ctx->N, R = ctx->rsz;
+    const size_t rows = (size_t)n[1] * n[2];
+    int rc = STZ_OK;
+    rc = rc ? rc : dalloc(ctx, (void **)&ctx->eps0, ctx->C * N * 8);
+    rc = rc ? rc : dalloc(ctx, &ctx->sigma, ctx->C * N * R);
+    rc = rc ? rc : dalloc(ctx, &ctx->q0, ctx->M * N * R);
+    rc = rc ? rc : dalloc(ctx, &ctx->deps, (size_t)ctx->M * ctx->S * N * R);
+    rc = rc ? rc : dalloc(ctx, &ctx->deps_last, ctx->S * N * R);
+    rc = rc ? rc : dalloc(ctx, (void **)&ctx->perm, N * 8);
+    rc = rc ? rc : dalloc(ctx, (void **)&ctx->temp, N * 8);
+    rc = rc ? rc : dalloc(ctx, (void **)&ctx->tlast, N * 8);
+    rc = rc ? rc : dalloc(ctx, &ctx->spec, (size_t)ctx->C * rows * ctx->hp * 2 * R);
+    rc = rc ? rc : dalloc(ctx, &ctx->minq, N * R);
+    rc = rc ? rc : dalloc(ctx, (void **)&ctx->argm, N);
+    rc = rc ? rc : dalloc(ctx, (void **)&ctx->bsum, (size_t)ctx->ngrp * STZ_GRP * 8);
+    rc = rc ? rc : dalloc(ctx, (void **)&ctx->bneg, ctx->nblk * sizeof(int));
+    rc = rc ? rc : dalloc(ctx, (void **)&ctx->gsum, ctx->ngrp * 8);
+    rc = rc ? rc : dalloc(ctx, (void **)&ctx->res, 64 * 8);
+    rc = rc ? rc : dalloc(ctx, (void **)&ctx->spart, rows * ctx->C * 8);
+    rc = rc ? rc : dalloc(ctx, (void **)&ctx->acnt, ctx->nblk * sizeof(int));
+    rc = rc ? rc : dalloc(ctx, (void **)&ctx->aoff, ctx->nblk * sizeof(int));
+    rc = rc ? rc : dalloc(ctx, (void **)&ctx->alist, N * sizeof(int));
+    rc = rc ? rc : dalloc(ctx, (void **)&ctx->socp, ctx->nblk * 8);
+    if (rc) return fail(rc);
+    if (hipHostMalloc((void **)&ctx->hres, 64 * sizeof(double), hipHostMallocDefault) != hipSuccess) return fail(STZ_E_HIP);
+    std::memset(ctx->hres, 0, 64 * sizeof(double));
+    rc = ensure_dnew(ctx, (size_t)ctx->M * ctx->S);
+    if (rc) return fail(rc);
+    rc = stz_fft_setup(ctx);
+    if (rc) return fail(rc);
+    // tlast = -100*tTemp (STZ/init.c:312)
+    hipLaunchKernelGGL((k_fill<double>), dim3((unsigned)((N + 255) / 256)), dim3(256), 0, ctx->stream, ctx->tlast,
+                       (long long)N, -100 * ctx->k.tTemp);
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(ctx->stream) != hipSuccess) return fail(STZ_E_HIP);
+    *out = ctx;
+    return STZ_OK;
+}
+
+void stz_destroy(stz_ctx *ctx)
+{
+    if (!ctx) return;
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    void *ptrs[] = {ctx->eps0, ctx->sigma, ctx->q0, ctx->deps, ctx->deps_last, ctx->perm, ctx->temp, ctx->tlast,
+                    ctx->spec, ctx->minq, ctx->argm, ctx->bsum, ctx->bneg, ctx->gsum, ctx->res, ctx->spart,
+                    ctx->acnt, ctx->aoff, ctx->alist, ctx->dnew, ctx->socp, ctx->gneg};
+    for (void *q : ptrs)
+        if (q) (void)hipFree(q);
+    stz_fft_free(ctx);
+    for (int i = 0; i < 24; ++i)
+        if (ctx->tev[i]) (void)hipEventDestroy(ctx->tev[i]);
+    if (ctx->hres) (void)hipHostFree(ctx->hres);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+const char *stz_last_error(const stz_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int stz_init_state(stz_ctx *ctx)
+{
+    if (!ctx) return STZ_E_ARG;
+    const long long N = ctx->N;
+    // MRO map: one uniform per voxel, glass unless u <= xMROFrac = 0 (STZ/init.c:355-396)
+    for (long long v = 0; v < N; ++v) {
+        const double u = ctx->rng.uniform();
+        if (u <= 0.0) {
+            ctx->err = "crystal-like MRO voxel drawn; only the glass branch is supported";
+            return STZ_E_UNSUPPORTED;
+        }
+    }
+    std::vector<double> buf((size_t)N * ctx->S);
+    // Energy_Init, STZ/init.c:181-198: mode-major, Q0 = N(0, std) + mean
+    for (int m = 0; m < ctx->M; ++m) {
+        for (long long v = 0; v < N; ++v) buf[v] = ctx->rng.gaussian(ctx->p.q0_std) + ctx->p.q0_mean;
+        int rc = put_real(ctx, ctx->q0, buf.data(), (size_t)N, (size_t)m * N);
+        if (rc) return rc;
+        STZ_HIP(hipStreamSynchronize(ctx->stream));
+    }
+    // DStrain glass branch, STZ/elastic.c:124-129: mode-major, voxel, then components
+    for (int m = 0; m < ctx->M; ++m) {
+        for (long long v = 0; v < N; ++v)
+            for (int k = 0; k < ctx->S; ++k) buf[(size_t)k * N + v] = ctx->k.half_gamma0 * ctx->rng.gaussian(1.0);
+        int rc = put_real(ctx, ctx->deps, buf.data(), (size_t)N * ctx->S, (size_t)m * ctx->S * N);
+        if (rc) return rc;
+        STZ_HIP(hipStreamSynchronize(ctx->stream));
+    }
+    for (int c = 0; c < 6; ++c) ctx->epsavg[c] = 0.0;
+    return solve(ctx);   // STZ/bmg.c:117-127
+}
+
+int stz_upload(stz_ctx *ctx, int field, const double *host)
+{
+    if (!ctx || !host) return STZ_E_ARG;
+    const size_t n = field_count(ctx, field);
+    if (n == 0 || field == STZ_F_MINQ) return STZ_E_ARG;
+    int rc = STZ_OK;
+    switch (field) {
+    case STZ_F_EPS0: STZ_HIP(hipMemcpyAsync(ctx->eps0, host, n * 8, hipMemcpyHostToDevice, ctx->stream)); break;
+    case STZ_F_SIGMA:
+        ctx->pend_comp = -1;
+        ctx->pend_delta = 0.0;
+        for (int c = 0; c < 6; ++c) ctx->soff[c] = 0.0;
+        rc = put_real(ctx, ctx->sigma, host, n);
+        break;
+    case STZ_F_Q0: rc = put_real(ctx, ctx->q0, host, n); break;
+    case STZ_F_DEPS: rc = put_real(ctx, ctx->deps, host, n); break;
+    case STZ_F_DEPS_LAST: rc = put_real(ctx, ctx->deps_last, host, n); break;
+    case STZ_F_PERM: STZ_HIP(hipMemcpyAsync(ctx->perm, host, n * 8, hipMemcpyHostToDevice, ctx->stream)); break;
+    case STZ_F_TEMP: STZ_HIP(hipMemcpyAsync(ctx->temp, host, n * 8, hipMemcpyHostToDevice, ctx->stream)); break;
+    case STZ_F_TLAST: STZ_HIP(hipMemcpyAsync(ctx->tlast, host, n * 8, hipMemcpyHostToDevice, ctx->stream)); break;
+    default: return STZ_E_ARG;
+    }
+    if (rc) return rc;
+    STZ_HIP(hipStreamSynchronize(ctx->stream));
+    return STZ_OK;
+}
+
+int stz_download(stz_ctx *ctx, int field, double *host)
+{
+    if (!ctx || !host) return STZ_E_ARG;
+    const size_t n = field_count(ctx, field);
+    if (n == 0) return STZ_E_ARG;
+    switch (field) {
+    case STZ_F_EPS0: STZ_HIP(hipMemcpyAsync(host, ctx->eps0, n * 8, hipMemcpyDeviceToHost, ctx->stream)); break;
+    case STZ_F_SIGMA: {
+        int rc = get_real(ctx, ctx->sigma, host, n);
+        if (rc) return rc;
+        const size_t N = (size_t)ctx->N;
+        if (ctx->prec == STZ_FP64) {
+            if (ctx->pend_comp >= 0)
+                for (size_t v = 0; v < N; ++v) host[(size_t)ctx->pend_comp * N + v] += ctx->pend_delta;
+        } else {
+            for (int c = 0; c < ctx->C; ++c)
+                for (size_t v = 0; v < N; ++v) host[(size_t)c * N + v] += ctx->soff[c];
+        }
+        return STZ_OK;
+    }
+    case STZ_F_Q0: return get_real(ctx, ctx->q0, host, n);
+    case STZ_F_DEPS: return get_real(ctx, ctx->deps, host, n);
+    case STZ_F_DEPS_LAST: return get_real(ctx, ctx->deps_last, host, n);
+    case STZ_F_MINQ: return get_real(ctx, ctx->minq, host, n);
+    case STZ_F_PERM: STZ_HIP(hipMemcpyAsync(host, ctx->perm, n * 8, hipMemcpyDeviceToHost, ctx->stream)); break;
+    case STZ_F_TEMP: STZ_HIP(hipMemcpyAsync(host, ctx->temp, n * 8, hipMemcpyDeviceToHost, ctx->stream)); break;
+    case STZ_F_TLAST: STZ_HIP(hipMemcpyAsync(host, ctx->tlast, n * 8, hipMemcpyDeviceToHost, ctx->stream)); break;
+    default: return STZ_E_ARG;
+    }
+    STZ_HIP(hipStreamSynchronize(ctx->stream));
+    return STZ_OK;
+}
+
+int stz_set_scalars(stz_ctx *ctx, const double epsavg[6], const double stress[6], double totime)
+{
+    if (!ctx) return STZ_E_ARG;
+    for (int c = 0; c < 6; ++c) {
+        if (epsavg) ctx->epsavg[c] = epsavg[c];
+        if (stress) ctx->stress[c] = stress[c];
+    }
+    ctx->totime = totime;
+    return STZ_OK;
+}
+
+int stz_get_scalars(stz_ctx *ctx, double epsavg[6], double stress[6], double *totime)
+{
+    if (!ctx) return STZ_E_ARG;
+    for (int c = 0; c < 6; ++c) {
+        if (epsavg) epsavg[c] = ctx->epsavg[c];
+        if (stress) stress[c] = ctx->stress[c];
+    }
+    if (totime) *totime = ctx->totime;
+    return STZ_OK;
+}
+
+int stz_stress_solve(stz_ctx *ctx) { return ctx ? solve(ctx) : STZ_E_ARG; }
+
+int stz_rates(stz_ctx *ctx, long long *n_negative, double *total_rate)
+{
+    if (!ctx) return STZ_E_ARG;
+    long long nn;
+    double R;
+    int rc = rates(ctx, &nn, &R);
+    if (rc) return rc;
+    if (n_negative) *n_negative = nn;
+    if (total_rate) *total_rate = R;
+    return STZ_OK;
+}
+
+int stz_select(stz_ctx *ctx, double dice, int *site, int *mode, double *q)
+{
+    if (!ctx || !site || !mode) return STZ_E_ARG;
+    double qq;
+    int rc = select_event(ctx, dice, site, mode, &qq);
+    if (rc) return rc;
+    if (q) *q = qq;
+    return STZ_OK;
+}
+
+int stz_apply_event(stz_ctx *ctx, int site, int mode, const double *deps_new)
+{
+    if (!ctx || !deps_new || site < 0 || site >= ctx->N || mode < 0 || mode >= ctx->M) return STZ_E_ARG;
+    int rc = apply_event(ctx, site, mode, deps_new);
+    if (rc) return rc;
+    STZ_HIP(hipStreamSynchronize(ctx->stream));
+    return STZ_OK;
+}
+
+int stz_athermal_scan(stz_ctx *ctx, int *n)
+{
+    if (!ctx || !n) return STZ_E_ARG;
+    return athermal_scan(ctx, n);
+}
+
+int stz_athermal_list(stz_ctx *ctx, int *sites, int *modes, int cap)
+{
+    if (!ctx || !sites) return STZ_E_ARG;
+    const int n = ctx->n_athermal < cap ? ctx->n_athermal : cap;
+    if (n <= 0) return STZ_OK;
+    STZ_HIP(hipMemcpyAsync(sites, ctx->alist, n * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    STZ_HIP(hipStreamSynchronize(ctx->stream));
+    if (modes) {
+        std::vector<uint8_t> am((size_t)ctx->N);
+        STZ_HIP(hipMemcpyAsync(am.data(), ctx->argm, (size_t)ctx->N, hipMemcpyDeviceToHost, ctx->stream));
+        STZ_HIP(hipStreamSynchronize(ctx->stream));
+        for (int i = 0; i < n; ++i) modes[i] = am[sites[i]];
+    }
+    return STZ_OK;
+}
+
+int stz_athermal_apply(stz_ctx *ctx, const double *deps_new, double *soc_elast)
+{
+    if (!ctx || (!deps_new && ctx->n_athermal > 0)) return STZ_E_ARG;
+    return athermal_apply(ctx, deps_new, soc_elast);
+}
+
+int stz_add_uniform_stress(stz_ctx *ctx, int comp, double delta)
+{
+    if (!ctx || comp < 0 || comp >= ctx->C) return STZ_E_ARG;
+    return add_uniform(ctx, comp, delta);
+}
+
+int stz_elastic_energy(stz_ctx *ctx, double *e)
+{
+    if (!ctx || !e) return STZ_E_ARG;
+    int rc = upload_soff(ctx);
+    if (rc) return rc;
+    rc = stz_energy_launch(ctx);
+    if (rc) return rc;
+    std::vector<double> part((size_t)ctx->nblk);
+    STZ_HIP(hipMemcpyAsync(part.data(), ctx->socp, ctx->nblk * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    STZ_HIP(hipStreamSynchronize(ctx->stream));
+    double s = 0.0;
+    for (long long i = 0; i < ctx->nblk; ++i) s += part[(size_t)i];
+    *e = s;
+    return STZ_OK;
+}
+
+int stz_set_print(stz_ctx *ctx, int printperiod, double endstrain)
+{
+    if (!ctx) return STZ_E_ARG;
+    ctx->printperiod = printperiod;
+    ctx->endstrain = endstrain;
+    return STZ_OK;
+}
+
+int stz_step_kmc(stz_ctx *ctx, stz_step_result *res)
+{
+    if (!ctx || !res) return STZ_E_ARG;
+    std::memset(res, 0, sizeof(*res));
+    res->site = res->mode = -1;
+    ctx->step++;
+    long long nneg;
+    double R;
+    int rc = rates(ctx, &nneg, &R);               // SofteningFactor + ActEnergy + count (bmg.c:141-157)
+    if (rc) return rc;
+    res->n_negative = nneg;
+    res->total_rate = R;
+    const size_t ms = (size_t)ctx->M * ctx->S;
+    if (nneg > 0) {
+        // athermal_plasticity (STZ/evolution.c:47-100)
+        ctx->astep++;
+        int n;
+        rc = athermal_scan(ctx, &n);
+        if (rc) return rc;
+        ctx->hbuf.resize((size_t)n * ms + 1);
+        for (int i = 0; i < n; ++i) draw_voxel_deps(ctx, ctx->hbuf.data() + (size_t)i * ms);
+        double soc = 0.0;
+        rc = athermal_apply(ctx, ctx->hbuf.data(), &soc);
+        if (rc) return rc;
+        ctx->avlan += n;
+        ctx->soc_elast += soc;
+        rc = solve(ctx);
+        if (rc) return rc;
+        athermal_correction(ctx);
+        res->kind = STZ_STEP_ATHERMAL;
+        res->n_transformed = n;
+        res->soc_elast = soc;
+    } else {
+        ctx->avlan = 0;                           // UpdateSOC (STZ/evolution.c:5-45)
+        ctx->soc_elast = 0.0;
+        const double t = 1 / R;                   // STZ/evolution.c:134
+        res->dt_kmc = t;
+        if (t < ctx->k.dt) {
+            ctx->kstep++;
+            const double dice = ctx->rng.uniform();   // STZ/evolution.c:142
+            int site, mode;
+            double q;
+            rc = select_event(ctx, dice, &site, &mode, &q);
+            if (rc) return rc;
+            ctx->hbuf.resize(ms);
+            draw_voxel_deps(ctx, ctx->hbuf.data());
+            rc = apply_event(ctx, site, mode, ctx->hbuf.data());
+            if (rc) return rc;
+            rc = solve(ctx);
+            if (rc) return rc;
+            rc = load_thermal(ctx, t);
+            if (rc) return rc;
+            ctx->totime += t;                     // STZ/evolution.c:317
+            res->kind = STZ_STEP_THERMAL;
+            res->site = site;
+            res->mode = mode;
+            res->q_chosen = q;
+        } else {
+            ctx->estep++;
+            ctx->totime += ctx->k.dt;             // STZ/evolution.c:324
+            rc = load_elastic(ctx);
+            if (rc) return rc;
+            res->kind = STZ_STEP_ELASTIC;
+        }
+    }
+    int term = 0;
+    if (ctx->printperiod > 0) {
+        const double cur = ctx->epsavg[ctx->lc];
+        if (std::fabs(cur - ctx->strainhist_last) >= (ctx->printperiod * ctx->p.strainincrement * 0.99)) {
+            ctx->strainhist_last = cur;           // strainhist[N_print-1], STZ/bmg.c:193-201
+            res->printed = 1;
+        } else {
+            term = terminated(ctx) ? 1 : 0;
+        }
+    }
+    res->totime = ctx->totime;
+    for (int c = 0; c < 6; ++c) {
+        res->epsavg[c] = ctx->epsavg[c];
+        res->stress[c] = ctx->stress[c];
+    }
+    return term;
+}
+
+int stz_step_print(stz_ctx *ctx, stz_step_result *res)
+{
+    if (!ctx || !res) return STZ_E_ARG;
+    int rc = solve(ctx);                          // print-step re-solve, STZ/bmg.c:289-295
+    if (rc) return rc;
+    for (int c = 0; c < 6; ++c) res->stress[c] = ctx->stress[c];
+    return terminated(ctx) ? 1 : 0;               // STZ/bmg.c:382-410
+}
+
+int stz_step(stz_ctx *ctx, stz_step_result *res)
+{
+    int rc = stz_step_kmc(ctx, res);
+    if (rc < 0 || !res->printed) return rc;
+    return stz_step_print(ctx, res);
+}
+
+int stz_rng_seed(stz_ctx *ctx, unsigned long seed)
+{
+    if (!ctx) return STZ_E_ARG;
+    ctx->rng.seed(seed);
+    return STZ_OK;
+}
+
+int stz_rng_uniform(stz_ctx *ctx, double *u)
+{
+    if (!ctx || !u) return STZ_E_ARG;
+    *u = ctx->rng.uniform();
+    return STZ_OK;
+}
+
+int stz_rng_gaussian(stz_ctx *ctx, double sigma, double *g)
+{
+    if (!ctx || !g) return STZ_E_ARG;
+    *g = ctx->rng.gaussian(sigma);
+    return STZ_OK;
+}
+
+int stz_rng_get_state(stz_ctx *ctx, unsigned int words[625])
+{
+    if (!ctx || !words) return STZ_E_ARG;
+    for (int i = 0; i < 624; ++i) words[i] = ctx->rng.state[i];
+    words[624] = (unsigned int)ctx->rng.pos;
+    return STZ_OK;
+}
+
+int stz_rng_set_state(stz_ctx *ctx, const unsigned int words[625])
+{
+    if (!ctx || !words || words[624] > 624) return STZ_E_ARG;
+    for (int i = 0; i < 624; ++i) ctx->rng.state[i] = words[i];
+    ctx->rng.pos = (int)words[624];
+    return STZ_OK;
+}
+
+int stz_get_print_state(stz_ctx *ctx, double out[3])
+{
+    if (!ctx || !out) return STZ_E_ARG;
+    out[0] = ctx->strainhist_last;
+    out[1] = ctx->printperiod;
+    out[2] = ctx->endstrain;
+    return STZ_OK;
+}
+
+int stz_set_print_state(stz_ctx *ctx, double strainhist_last)
+{
+    if (!ctx) return STZ_E_ARG;
+    ctx->strainhist_last = strainhist_last;
+    return STZ_OK;
+}
+
+int stz_timing_enable(stz_ctx *ctx, int on)
+{
+    if (!ctx) return STZ_E_ARG;
+    if (on && !ctx->tev[0]) {
+        for (int i = 0; i < 24; ++i) STZ_HIP(hipEventCreate(&ctx->tev[i]));
+    }
+    ctx->timing = on ? 1 : 0;
+    for (int i = 0; i < 8; ++i) {
+        ctx->tms[i] = ctx->sms[i] = 0.0;
+        ctx->tn[i] = ctx->sn[i] = 0;
+    }
+    return STZ_OK;
+}
+
+int stz_timing_get(stz_ctx *ctx, double ms[8], long long n[8])
+{
+    if (!ctx || !ms || !n) return STZ_E_ARG;
+    for (int i = 0; i < 8; ++i) {
+        ms[i] = ctx->tms[i];
+        n[i] = ctx->tn[i];
+    }
+    return STZ_OK;
+}
+
+int stz_timing_solve_get(stz_ctx *ctx, double ms[8], long long n[8])
+{
+    if (!ctx || !ms || !n) return STZ_E_ARG;
+    for (int i = 0; i < 8; ++i) {
+        ms[i] = ctx->sms[i];
+        n[i] = ctx->sn[i];
+    }
+    return STZ_OK;
+}
+
+int stz_counters(stz_ctx *ctx, long long out[6])
+{
+    if (!ctx || !out) return STZ_E_ARG;
+    out[0] = ctx->step;
+    out[1] = ctx->astep;
+    out[2] = ctx->kstep;
+    out[3] = ctx->estep;
+    out[4] = ctx->ntrans;
+    out[5] = ctx->nsolve;
+    return STZ_OK;
+}
+
+}  // extern "C"

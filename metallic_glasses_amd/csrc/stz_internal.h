@@ -1,0 +1,317 @@
+// stz_internal.h -- context layout and device helpers shared by the HIP sources.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "stz.h"
+#include "stz_rng.h"
+
+#define STZ_MAXC 6
+#define STZ_MAXS 5
+#define STZ_MAXM 64
+#define STZ_BLK 256          // voxels per rate block (one per thread)
+#define STZ_GRP 256          // rate blocks per selection group
+
+// physical constants of the reference, STZ/bmg.h:68-74
+#define STZ_EV_IN_J 1.60217733e-19
+#define STZ_J_IN_EV (1 / STZ_EV_IN_J)
+#define STZ_EPSILON 1e-17
+#define STZ_BOLZ 1.380658e-23
+#define STZ_PI_REF 3.14159265359
+
+template <typename R>
+struct alignas(2 * sizeof(R)) cpx {
+    R x, y;
+};
+
+// Constants derived from stz_params exactly as STZ/init.c:6-75 does.
+struct StzConsts {
+    double E, nu, mu, lambda, alpha, kT, tTemp, dt, voxel, ratepre, wfac_v;
+    double permsoft, tempsoft, permcap, thr, half_gamma0;
+};
+
+struct FftTwiddles {
+    void *tw[3] = {nullptr, nullptr, nullptr};   // per axis, L complex e^{-2 pi i t / L}
+};
+
+struct stz_ctx {
+    stz_params p;
+    StzConsts k;
+    int C, S, M, dim;
+    int n[3];
+    long long N;
+    int lc;                 // loaded component
+    int prec;               // STZ_FP64 | STZ_MIXED
+    size_t rsz;             // sizeof real storage type (8 or 4)
+    int h0, hp;             // half spectrum length, padded row stride (complex elements)
+    long long nblk;         // rate blocks per mode
+    long long ngrp;         // selection groups
+
+    hipStream_t stream = nullptr;
+
+    // device fields
+    double *eps0 = nullptr;       // [C][N]
+    void *sigma = nullptr;        // [C][N] R
+    void *q0 = nullptr;           // [M][N] R
+    void *deps = nullptr;         // [M][S][N] R
+    void *deps_last = nullptr;    // [S][N] R
+    double *perm = nullptr, *temp = nullptr, *tlast = nullptr;
+    void *spec = nullptr;         // [C][n1*n2][hp] cpx<R>
+    void *minq = nullptr;         // [N] R
+    uint8_t *argm = nullptr;      // [N]
+    double *bsum = nullptr;       // [M*nblk] block rate sums
+    int *bneg = nullptr;          // [nblk] Q<0 counts per voxel block
+    double *gsum = nullptr;       // [ngrp]
+    long long *gneg = nullptr;    // [ngrp]
+    double *res = nullptr;        // small result area (device)
+    double *hres = nullptr;       // pinned host mirror
+    double *spart = nullptr;      // stress-mean partials [nrowblk][C]
+    int *acnt = nullptr;          // athermal per-block counts [nblk]
+    int *aoff = nullptr;          // athermal per-block offsets [nblk]
+    int *alist = nullptr;         // athermal voxel list [N]
+    double *dnew = nullptr;       // new transformation strains for apply [cap]
+    size_t dnew_cap = 0;
+    double *socp = nullptr;       // athermal DWork partials
+    FftTwiddles tw;
+    int n_athermal = 0;
+
+    // host-side scalars (the reference's globals)
+    double epsavg[6] = {0, 0, 0, 0, 0, 0};
+    double stress[6] = {0, 0, 0, 0, 0, 0};
+    double totime = 0.0;
+    double rate_total = 0.0;
+    int pend_comp = -1;
+    double pend_delta = 0.0;       // FP64: applied and written back by the next rate pass
+    double soff[6] = {0, 0, 0, 0, 0, 0}; // MIXED: uniform stress offset added on read
+
+    // run control
+    int printperiod = 0;
+    double endstrain = 1e300;
+    double strainhist_last = 0.0;
+    long long step = 0, astep = 0, kstep = 0, estep = 0, ntrans = 0, nsolve = 0;
+    long long avlan = 0;
+    double soc_elast = 0.0;
+
+    StzRng rng;
+    std::vector<double> hbuf;      // host staging for redraws
+    std::string err;
+
+    // kernel timing with HIP events on `stream`
+    int timing = 0;
+    hipEvent_t tev[24] = {};
+    double tms[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    long long tn[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    double sms[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    long long sn[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+};
+
+// event slots: solve 0..6, rates 8..10, select 11..12, apply 13..14, athermal 15..16
+inline void tmark(stz_ctx *c, int i)
+{
+    if (c->timing) (void)hipEventRecord(c->tev[i], c->stream);
+}
+
+// ---------------------------------------------------------------------------
+// error helpers
+// ---------------------------------------------------------------------------
+#define STZ_HIP(call)                                                              \
+    do {                                                                           \
+        hipError_t e_ = (call);                                                    \
+        if (e_ != hipSuccess) {                                                    \
+            ctx->err = std::string(#call) + ": " + hipGetErrorString(e_);          \
+            return STZ_E_HIP;                                                      \
+        }                                                                          \
+    } while (0)
+
+#define STZ_CHECK_LAUNCH()                                                         \
+    do {                                                                           \
+        hipError_t e_ = hipGetLastError();                                         \
+        if (e_ != hipSuccess) {                                                    \
+            ctx->err = std::string("launch: ") + hipGetErrorString(e_);            \
+            return STZ_E_HIP;                                                      \
+        }                                                                          \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// device helpers
+// ---------------------------------------------------------------------------
+template <typename R>
+__device__ __forceinline__ cpx<R> cmul(cpx<R> a, cpx<R> b)
+{
+    return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+template <typename R>
+__device__ __forceinline__ cpx<R> cadd(cpx<R> a, cpx<R> b) { return {a.x + b.x, a.y + b.y}; }
+template <typename R>
+__device__ __forceinline__ cpx<R> csub(cpx<R> a, cpx<R> b) { return {a.x - b.x, a.y - b.y}; }
+// multiply by s*i (s = +1 or -1)
+template <typename R>
+__device__ __forceinline__ cpx<R> mul_si(cpx<R> a, R s) { return {-s * a.y, s * a.x}; }
+
+// In-register DFT of size Q, sign s (-1 forward, +1 backward).
+template <typename R, int Q>
+struct Dft;
+
+template <typename R>
+struct Dft<R, 2> {
+    static __device__ __forceinline__ void run(cpx<R> *v, R)
+    {
+        cpx<R> a = v[0], b = v[1];
+        v[0] = cadd(a, b);
+        v[1] = csub(a, b);
+    }
+};
+
+template <typename R>
+struct Dft<R, 4> {
+    static __device__ __forceinline__ void run(cpx<R> *v, R s)
+    {
+        cpx<R> ac = cadd(v[0], v[2]), am = csub(v[0], v[2]);
+        cpx<R> bd = cadd(v[1], v[3]), bm = mul_si(csub(v[1], v[3]), s);
+        v[0] = cadd(ac, bd);
+        v[2] = csub(ac, bd);
+        v[1] = cadd(am, bm);
+        v[3] = csub(am, bm);
+    }
+};
+
+template <typename R>
+struct Dft<R, 8> {
+    static __device__ __forceinline__ void run(cpx<R> *v, R s)
+    {
+        const R h = (R)0.70710678118654752440084436210484903928;
+        cpx<R> e[4] = {v[0], v[2], v[4], v[6]};
+        cpx<R> o[4] = {v[1], v[3], v[5], v[7]};
+        Dft<R, 4>::run(e, s);
+        Dft<R, 4>::run(o, s);
+        // o[k] *= w8^k, w8 = e^{s*2*pi*i/8}
+        cpx<R> o1 = {h * (o[1].x - s * o[1].y), h * (o[1].y + s * o[1].x)};
+        cpx<R> o2 = mul_si(o[2], s);
+        cpx<R> o3 = {h * (-o[3].x - s * o[3].y), h * (-o[3].y + s * o[3].x)};
+        v[0] = cadd(e[0], o[0]);
+        v[4] = csub(e[0], o[0]);
+        v[1] = cadd(e[1], o1);
+        v[5] = csub(e[1], o1);
+        v[2] = cadd(e[2], o2);
+        v[6] = csub(e[2], o2);
+        v[3] = cadd(e[3], o3);
+        v[7] = csub(e[3], o3);
+    }
+};
+
+template <typename R>
+struct Dft<R, 3> {
+    static __device__ __forceinline__ void run(cpx<R> *v, R s)
+    {
+        const R c3 = (R)0.86602540378443864676372317075293618347;  // sqrt(3)/2
+        cpx<R> a = v[0], b = v[1], c = v[2];
+        cpx<R> bc = cadd(b, c);
+        cpx<R> t1 = {a.x - (R)0.5 * bc.x, a.y - (R)0.5 * bc.y};
+        cpx<R> d = csub(b, c);
+        cpx<R> t2 = {-s * c3 * d.y, s * c3 * d.x};
+        v[0] = cadd(a, bc);
+        v[1] = cadd(t1, t2);
+        v[2] = csub(t1, t2);
+    }
+};
+
+// constexpr radix plan: prefer 8, then 4, 2, 3
+__host__ __device__ constexpr int radix_count(int L)
+{
+    int c = 0;
+    while (L % 8 == 0 && L > 4) { L /= 8; ++c; }
+    while (L % 4 == 0) { L /= 4; ++c; }
+    while (L % 2 == 0) { L /= 2; ++c; }
+    while (L % 3 == 0) { L /= 3; ++c; }
+    return L == 1 ? c : -1;
+}
+__host__ __device__ constexpr int radix_at(int L, int i)
+{
+    int c = 0;
+    while (L % 8 == 0 && L > 4) { if (c == i) return 8; L /= 8; ++c; }
+    while (L % 4 == 0) { if (c == i) return 4; L /= 4; ++c; }
+    while (L % 2 == 0) { if (c == i) return 2; L /= 2; ++c; }
+    while (L % 3 == 0) { if (c == i) return 3; L /= 3; ++c; }
+    return 0;
+}
+
+// One Stockham pass of radix Q (stride parameter NS = product of the earlier radices)
+// over NL lines of length L held in LDS with layout buf[point * LS + line]; in place,
+// through registers (two barriers).
+template <typename R, int L, int NL, int LS, int NT, int Q, int NS>
+__device__ __forceinline__ void fft_pass(cpx<R> *buf, const cpx<R> *tw, R s)
+{
+    constexpr int NQ = L / Q;
+    constexpr int TASKS = NL * NQ;
+    constexpr int TPT = (TASKS + NT - 1) / NT;
+    constexpr int TSTEP = L / (NS * Q);
+    cpx<R> v[TPT][Q];
+    const int tid = threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < TPT; ++k) {
+        const int t = tid + k * NT;
+        if ((TASKS % NT == 0) || t < TASKS) {
+            const int line = t % NL, j = t / NL;
+#pragma unroll
+            for (int r = 0; r < Q; ++r) v[k][r] = buf[(j + r * NQ) * LS + line];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < TPT; ++k) {
+        const int t = tid + k * NT;
+        if ((TASKS % NT == 0) || t < TASKS) {
+            const int line = t % NL, j = t / NL;
+            const int jm = j % NS;
+            if (NS > 1) {
+#pragma unroll
+                for (int r = 1; r < Q; ++r) {
+                    cpx<R> w = tw[jm * TSTEP * r];
+                    w.y = w.y * (-s);     // table holds e^{-i..}; s = +1 conjugates
+                    v[k][r] = cmul(v[k][r], w);
+                }
+            }
+            Dft<R, Q>::run(v[k], s);
+            const int idxD = (j / NS) * NS * Q + jm;
+#pragma unroll
+            for (int r = 0; r < Q; ++r) buf[(idxD + r * NS) * LS + line] = v[k][r];
+        }
+    }
+    __syncthreads();
+}
+
+template <typename R, int L, int NL, int LS, int NT, int I, int NS>
+__device__ __forceinline__ void fft_plan(cpx<R> *buf, const cpx<R> *tw, R s)
+{
+    if constexpr (I < radix_count(L)) {
+        constexpr int Q = radix_at(L, I);
+        fft_pass<R, L, NL, LS, NT, Q, NS>(buf, tw, s);
+        fft_plan<R, L, NL, LS, NT, I + 1, NS * Q>(buf, tw, s);
+    }
+}
+
+// Full FFT (natural order in and out) of NL lines of length L, sign s (-1 fwd, +1 bwd).
+template <typename R, int L, int NL, int LS, int NT>
+__device__ __forceinline__ void fft_lines(cpx<R> *buf, const cpx<R> *tw, R s)
+{
+    static_assert(radix_count(L) > 0, "unsupported FFT length");
+    fft_plan<R, L, NL, LS, NT, 0, 1>(buf, tw, s);
+}
+
+// ---------------------------------------------------------------------------
+// host-side launchers implemented in the .hip files
+// ---------------------------------------------------------------------------
+int stz_fft_setup(stz_ctx *ctx);
+void stz_fft_free(stz_ctx *ctx);
+int stz_solve_launch(stz_ctx *ctx);          // enqueues the full stress solve on ctx->stream
+int stz_rates_launch(stz_ctx *ctx);          // enqueues rates + group sums + total
+int stz_select_launch(stz_ctx *ctx, double target, double dice);
+int stz_apply_launch(stz_ctx *ctx, int site, int mode);
+int stz_athermal_launch(stz_ctx *ctx);
+int stz_athermal_apply_launch(stz_ctx *ctx, int n);
+int stz_energy_launch(stz_ctx *ctx);
+bool stz_solve_supported(int dim, const int *n, int prec);

@@ -1,0 +1,685 @@
+// stz_solve.hip -- Green's-function stress solve on the GPU.
+//
+// Replaces InhomogeneousStressSolver(uflag=0) with uniform moduli
+// (STZ/elastic.c:977-1066), i.e. one HomogeneousStressSolver (STZ/elastic.c:779-828)
+// on the round-tripped eigenstrain C0^-1(C0:eps0) (STZ/elastic.c:999-1009).
+//
+// Data flow (one launch per box; C = 3 or 6 components):
+//   k_fwd_x   eps0 (FP64) -> round trip -> R2C along x (two real rows packed per
+//             complex line)                          -> spec[c][row][k0], k0 <= n0/2
+//   k_ypass   complex FFT along y, in place (3-D only)
+//   k_fused   last axis (z in 3-D, y in 2-D): forward FFT of all C components,
+//             Gamma-hat multiply, inverse FFT -- the spectrum of one tile never
+//             leaves the workgroup between the two transforms
+//   k_ypass   inverse along y (3-D only)
+//   k_inv_x   C2R along x + epilogue sigma = C0:(eps/N + epsavg - eps0_rt), block
+//             partial sums of sigma (FP64)
+//   k_mean    fixed-order reduction of the partials -> <sigma>
+//
+// Half-spectrum equivalence: the reference transforms the full complex spectrum and
+// keeps Re(IFFT) (STZ/elastic.c:811-824).  Its k vectors use round() (STZ/init.c:
+// 126-129), so on Nyquist planes Gamma-hat(n(k)) != Gamma-hat(n(-k)); Re() then applies
+// the average of the two.  The half-spectrum path applies that average explicitly on
+// Nyquist bins, which reproduces the reference to rounding.
+#include "stz_internal.h"
+
+#include <cmath>
+
+namespace {
+
+constexpr int NT_SOLVE = 256;
+
+struct SolveConsts {
+    double lambda, mu, alpha, s0, sh, sa, invN;
+    double epsavg[6];
+    int n[3];
+    long long N;
+    int rows;           // n1*n2
+    int h0, hp;
+};
+
+// --- elementwise elasticity (literal expression order of the reference) ----------
+#pragma clang fp contract(off)
+
+// Tau0 = C0:e (IsotropicMul_RF with uniform moduli, STZ/elastic.c:905-920), then
+// C0^-1(Tau0 + Tau1) with Tau1 = (C0 - C0):(e + epsavg) (IsotropicMul_RF_D :922-938,
+// IsotropicInv :940-955).
+template <int C>
+__device__ __forceinline__ void round_trip(const double *e, const double *ea, double lam, double mu, double *r)
+{
+    double t0[C], t1[C], tau[C];
+    double z[C];
+#pragma unroll
+    for (int c = 0; c < C; ++c) z[c] = e[c] + ea[c];
+    if constexpr (C == 3) {
+        double tra = lam * (e[0] + e[1]);
+        t0[0] = tra + 2 * mu * e[0];
+        t0[1] = tra + 2 * mu * e[1];
+        t0[2] = 2 * mu * e[2];
+        double tra1 = (lam - lam) * (z[0] + z[1]);
+        t1[0] = tra1 + 2 * (mu - mu) * z[0];
+        t1[1] = tra1 + 2 * (mu - mu) * z[1];
+        t1[2] = 2 * (mu - mu) * z[2];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) tau[c] = t0[c] + t1[c];
+        double tr = -1.0 * lam / 2.0 / mu / (2.0 * lam + 2.0 * mu) * (tau[0] + tau[1]);
+        r[0] = tr + tau[0] / 2.0 / mu;
+        r[1] = tr + tau[1] / 2.0 / mu;
+        r[2] = tau[2] / 2.0 / mu;
+    } else {
+        double tra = lam * ((e[0] + e[1]) + e[2]);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) t0[c] = tra + 2 * mu * e[c];
+#pragma unroll
+        for (int c = 3; c < 6; ++c) t0[c] = 2 * mu * e[c];
+        double tra1 = (lam - lam) * ((z[0] + z[1]) + z[2]);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) t1[c] = tra1 + 2 * (mu - mu) * z[c];
+#pragma unroll
+        for (int c = 3; c < 6; ++c) t1[c] = 2 * (mu - mu) * z[c];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) tau[c] = t0[c] + t1[c];
+        double tr = -1.0 * lam / 2.0 / mu / (3.0 * lam + 2.0 * mu) * ((tau[0] + tau[1]) + tau[2]);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) r[c] = tr + tau[c] / 2.0 / mu;
+#pragma unroll
+        for (int c = 3; c < 6; ++c) r[c] = tau[c] / 2.0 / mu;
+    }
+}
+
+// IsotropicMul_R2 (STZ/elastic.c:686-701): sigma = C0:d
+template <int C>
+__device__ __forceinline__ void stress_of(const double *d, double lam, double mu, double *s)
+{
+    if constexpr (C == 3) {
+        double tra = lam * (d[0] + d[1]);
+        s[0] = tra + 2.0 * mu * d[0];
+        s[1] = tra + 2.0 * mu * d[1];
+        s[2] = 2.0 * mu * d[2];
+    } else {
+        double tra = lam * ((d[0] + d[1]) + d[2]);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) s[c] = tra + 2.0 * mu * d[c];
+#pragma unroll
+        for (int c = 3; c < 6; ++c) s[c] = 2.0 * mu * d[c];
+    }
+}
+
+// Gamma-hat applied to one real part (re or im) of the eigenstrain spectrum:
+// IsotropicMul_C, HadamardProdnSum_C x3, HadamardProdnSSum_C x2, HadamardProd2nSSum_C
+// (STZ/elastic.c:802-809); g = unit k vector, gg its products.
+template <typename R, int C>
+__device__ __forceinline__ void gamma_part(const R *e, const R *g, const R *gg, R lam, R mu, R s0, R sh, R sa, R *o)
+{
+    R t[C];
+    if constexpr (C == 3) {
+        R tra = lam * (e[0] + e[1]);
+        t[0] = tra + 2 * mu * e[0];
+        t[1] = tra + 2 * mu * e[1];
+        t[2] = 2 * mu * e[2];
+        R F0 = t[0] * g[0] + t[2] * g[1];
+        R F1 = t[2] * g[0] + t[1] * g[1];
+        R G = F0 * g[0] + F1 * g[1];
+        o[0] = s0 * (F0 * g[0]) + sa * (G * gg[0]);
+        o[1] = s0 * (F1 * g[1]) + sa * (G * gg[1]);
+        o[2] = sh * (F0 * g[1] + F1 * g[0]) + sa * (G * gg[2]);
+    } else {
+        R tra = lam * ((e[0] + e[1]) + e[2]);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) t[c] = tra + 2 * mu * e[c];
+#pragma unroll
+        for (int c = 3; c < 6; ++c) t[c] = 2 * mu * e[c];
+        // Voigt (xx, yy, zz, yz, xz, xy)
+        R F0 = t[0] * g[0] + t[5] * g[1] + t[4] * g[2];
+        R F1 = t[5] * g[0] + t[1] * g[1] + t[3] * g[2];
+        R F2 = t[4] * g[0] + t[3] * g[1] + t[2] * g[2];
+        R G = F0 * g[0] + F1 * g[1] + F2 * g[2];
+        o[0] = s0 * (F0 * g[0]) + sa * (G * gg[0]);
+        o[1] = s0 * (F1 * g[1]) + sa * (G * gg[1]);
+        o[2] = s0 * (F2 * g[2]) + sa * (G * gg[2]);
+        o[3] = sh * (F1 * g[2] + F2 * g[1]) + sa * (G * gg[3]);
+        o[4] = sh * (F0 * g[2] + F2 * g[0]) + sa * (G * gg[4]);
+        o[5] = sh * (F0 * g[1] + F1 * g[0]) + sa * (G * gg[5]);
+    }
+}
+
+// unit vector of the reference k grid (Grad, STZ/init.c:116-137) for integer indices
+template <int DIM>
+__device__ __forceinline__ void unit_k(const int *idx, const int *n, double *g)
+{
+    g[0] = g[1] = g[2] = 0.0;
+#pragma unroll
+    for (int a = 0; a < DIM; ++a) {
+        double s = (double)(idx[a]) * (double)(1.0 / n[a]);
+        g[a] = (s - round(s)) * (2 * STZ_PI_REF);
+    }
+    double g2;
+    if constexpr (DIM == 2) g2 = sqrt(g[0] * g[0] + g[1] * g[1]);
+    else g2 = sqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]);
+    if (idx[0] + idx[1] + (DIM == 3 ? idx[2] : 0) == 0) g2 = 1.0;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) g[a] /= g2;
+}
+
+template <typename R, int DIM>
+__device__ __forceinline__ void products(const R *g, R *gg)
+{
+    if constexpr (DIM == 2) {
+        gg[0] = g[0] * g[0];
+        gg[1] = g[1] * g[1];
+        gg[2] = g[0] * g[1];
+    } else {
+        gg[0] = g[0] * g[0];
+        gg[1] = g[1] * g[1];
+        gg[2] = g[2] * g[2];
+        gg[3] = g[1] * g[2];
+        gg[4] = g[0] * g[2];
+        gg[5] = g[0] * g[1];
+    }
+}
+
+#pragma clang fp contract(on)
+
+// ---------------------------------------------------------------------------------
+// k_fwd_x: eps0 -> round trip -> R2C along x for NR rows and all C components
+// ---------------------------------------------------------------------------------
+template <typename R, int L0, int NR, int C>
+__global__ __launch_bounds__(NT_SOLVE) void k_fwd_x(const double *__restrict__ eps0, cpx<R> *__restrict__ spec,
+                                                    const cpx<R> *__restrict__ twg, SolveConsts k)
+{
+    constexpr int NP = NR / 2;            // packed complex lines per component
+    constexpr int NL = C * NP;
+    constexpr int LS = NL + 1;            // odd stride against bank conflicts
+    __shared__ cpx<R> buf[L0 * LS];
+    __shared__ cpx<R> tw[L0];
+    const int tid = threadIdx.x;
+    for (int i = tid; i < L0; i += NT_SOLVE) tw[i] = twg[i];
+    const long long row0 = (long long)blockIdx.x * NR;
+    R *bufr = reinterpret_cast<R *>(buf);
+    for (int e = tid; e < NR * L0; e += NT_SOLVE) {
+        const int rl = e / L0, i0 = e % L0;
+        const long long v = (row0 + rl) * L0 + i0;
+        double ev[C], rt[C];
+#pragma unroll
+        for (int c = 0; c < C; ++c) ev[c] = eps0[(long long)c * k.N + v];
+        round_trip<C>(ev, k.epsavg, k.lambda, k.mu, rt);
+#pragma unroll
+        for (int c = 0; c < C; ++c) bufr[2 * (i0 * LS + c * NP + (rl >> 1)) + (rl & 1)] = (R)rt[c];
+    }
+    __syncthreads();
+    fft_lines<R, L0, NL, LS, NT_SOLVE>(buf, tw, (R)-1);
+    constexpr int H0 = L0 / 2 + 1;
+    for (int e = tid; e < C * NR * H0; e += NT_SOLVE) {
+        const int kk = e % H0;
+        const int rl = (e / H0) % NR;
+        const int c = e / (H0 * NR);
+        const int p = rl >> 1;
+        const cpx<R> Z = buf[kk * LS + c * NP + p];
+        const cpx<R> Zm = buf[((L0 - kk) % L0) * LS + c * NP + p];
+        cpx<R> X;
+        if ((rl & 1) == 0) X = {(Z.x + Zm.x) * (R)0.5, (Z.y - Zm.y) * (R)0.5};
+        else X = {(Z.y + Zm.y) * (R)0.5, -(Z.x - Zm.x) * (R)0.5};
+        spec[((long long)c * k.rows + row0 + rl) * k.hp + kk] = X;
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// k_ypass: complex FFT along y (stride hp) for a tile of W k0-columns, in place
+// ---------------------------------------------------------------------------------
+template <typename R, int L1, int W>
+__global__ __launch_bounds__(NT_SOLVE) void k_ypass(cpx<R> *__restrict__ spec, const cpx<R> *__restrict__ twg,
+                                                    SolveConsts k, R sgn)
+{
+    __shared__ cpx<R> buf[L1 * W];
+    __shared__ cpx<R> tw[L1];
+    const int tid = threadIdx.x;
+    for (int i = tid; i < L1; i += NT_SOLVE) tw[i] = twg[i];
+    const int k0base = blockIdx.x * W;
+    const int i2 = blockIdx.y;
+    const int c = blockIdx.z;
+    cpx<R> *base = spec + ((long long)c * k.rows + (long long)i2 * L1) * k.hp + k0base;
+    for (int e = tid; e < L1 * W; e += NT_SOLVE) {
+        const int col = e % W, p = e / W;
+        cpx<R> val = {0, 0};
+        if (k0base + col < k.h0) val = base[(long long)p * k.hp + col];
+        buf[p * W + col] = val;
+    }
+    __syncthreads();
+    fft_lines<R, L1, W, W, NT_SOLVE>(buf, tw, sgn);
+    for (int e = tid; e < L1 * W; e += NT_SOLVE) {
+        const int col = e % W, p = e / W;
+        if (k0base + col < k.h0) base[(long long)p * k.hp + col] = buf[p * W + col];
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// k_fused: last axis forward FFT, Gamma-hat, inverse FFT for one tile of W k0-columns
+// (3-D: at fixed y index blockIdx.y; 2-D: the y axis itself is the last axis)
+// ---------------------------------------------------------------------------------
+template <typename R, int LZ, int W, int C, int DIM>
+__global__ __launch_bounds__(NT_SOLVE) void k_fused(cpx<R> *__restrict__ spec, const cpx<R> *__restrict__ twg,
+                                                    SolveConsts k)
+{
+    constexpr int NE = LZ * W;
+    constexpr int EPT = (NE + NT_SOLVE - 1) / NT_SOLVE;
+    __shared__ cpx<R> buf[NE];
+    __shared__ cpx<R> tw[LZ];
+    const int tid = threadIdx.x;
+    for (int i = tid; i < LZ; i += NT_SOLVE) tw[i] = twg[i];
+    const int k0base = blockIdx.x * W;
+    const int i1 = (DIM == 3) ? blockIdx.y : 0;
+    // element (p, col) of component c lives at spec[(c*rows + p*pstride + i1) * hp + k0]
+    const long long pstride = (DIM == 3) ? k.n[1] : 1;
+    cpx<R> reg[C][EPT];
+    for (int c = 0; c < C; ++c) {
+        cpx<R> *base = spec + ((long long)c * k.rows + i1) * k.hp + k0base;
+        for (int e = tid; e < NE; e += NT_SOLVE) {
+            const int col = e % W, p = e / W;
+            cpx<R> val = {0, 0};
+            if (k0base + col < k.h0) val = base[(long long)p * pstride * k.hp + col];
+            buf[e] = val;
+        }
+        __syncthreads();
+        fft_lines<R, LZ, W, W, NT_SOLVE>(buf, tw, (R)-1);
+#pragma unroll
+        for (int j = 0; j < EPT; ++j) {
+            const int e = tid + j * NT_SOLVE;
+            if (NE % NT_SOLVE == 0 || e < NE) reg[c][j] = buf[e];
+        }
+        __syncthreads();
+    }
+    // Gamma-hat on the elements this thread owns
+    const R lam = (R)k.lambda, mu = (R)k.mu, s0 = (R)k.s0, sh = (R)k.sh, sa = (R)k.sa;
+#pragma unroll
+    for (int j = 0; j < EPT; ++j) {
+        const int e = tid + j * NT_SOLVE;
+        if (!(NE % NT_SOLVE == 0 || e < NE)) continue;
+        const int col = e % W, p = e / W;
+        int idx[3];
+        idx[0] = k0base + col;
+        if constexpr (DIM == 3) { idx[1] = i1; idx[2] = p; }
+        else { idx[1] = p; idx[2] = 0; }
+        double gd[3];
+        unit_k<DIM>(idx, k.n, gd);
+        bool nyq = false;
+#pragma unroll
+        for (int a = 0; a < DIM; ++a) nyq |= (2 * idx[a] == k.n[a]);
+        R g[3] = {(R)gd[0], (R)gd[1], (R)gd[2]};
+        R gg[6];
+        products<R, DIM>(g, gg);
+        R er[C], ei[C], orr[C], oi[C];
+#pragma unroll
+        for (int c = 0; c < C; ++c) { er[c] = reg[c][j].x; ei[c] = reg[c][j].y; }
+        gamma_part<R, C>(er, g, gg, lam, mu, s0, sh, sa, orr);
+        gamma_part<R, C>(ei, g, gg, lam, mu, s0, sh, sa, oi);
+        if (nyq) {
+            // average with the mirror bin's unit vector (Nyquist components negated)
+            R gm[3] = {g[0], g[1], g[2]};
+#pragma unroll
+            for (int a = 0; a < DIM; ++a)
+                if (2 * idx[a] == k.n[a]) gm[a] = -gm[a];
+            R ggm[6];
+            products<R, DIM>(gm, ggm);
+            R o2r[C], o2i[C];
+            gamma_part<R, C>(er, gm, ggm, lam, mu, s0, sh, sa, o2r);
+            gamma_part<R, C>(ei, gm, ggm, lam, mu, s0, sh, sa, o2i);
+#pragma unroll
+            for (int c = 0; c < C; ++c) {
+                orr[c] = (orr[c] + o2r[c]) * (R)0.5;
+                oi[c] = (oi[c] + o2i[c]) * (R)0.5;
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < C; ++c) reg[c][j] = {orr[c], oi[c]};
+    }
+    for (int c = 0; c < C; ++c) {
+#pragma unroll
+        for (int j = 0; j < EPT; ++j) {
+            const int e = tid + j * NT_SOLVE;
+            if (NE % NT_SOLVE == 0 || e < NE) buf[e] = reg[c][j];
+        }
+        __syncthreads();
+        fft_lines<R, LZ, W, W, NT_SOLVE>(buf, tw, (R)1);
+        cpx<R> *base = spec + ((long long)c * k.rows + i1) * k.hp + k0base;
+        for (int e = tid; e < NE; e += NT_SOLVE) {
+            const int col = e % W, p = e / W;
+            if (k0base + col < k.h0) base[(long long)p * pstride * k.hp + col] = buf[e];
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// k_inv_x: C2R along x + stress epilogue + per-block partial sums of sigma
+// ---------------------------------------------------------------------------------
+template <typename R, int L0, int NR, int C>
+__global__ __launch_bounds__(NT_SOLVE) void k_inv_x(const cpx<R> *__restrict__ spec, const double *__restrict__ eps0,
+                                                    R *__restrict__ sigma, double *__restrict__ spart,
+                                                    int *__restrict__ nanflag, const cpx<R> *__restrict__ twg,
+                                                    SolveConsts k)
+{
+    constexpr int NP = NR / 2;
+    constexpr int NL = C * NP;
+    constexpr int LS = NL + 1;
+    constexpr int H = L0 / 2;
+    __shared__ cpx<R> buf[L0 * LS];
+    __shared__ cpx<R> tw[L0];
+    __shared__ double red[NT_SOLVE];
+    const int tid = threadIdx.x;
+    for (int i = tid; i < L0; i += NT_SOLVE) tw[i] = twg[i];
+    const long long row0 = (long long)blockIdx.x * NR;
+    for (int e = tid; e < C * NP * L0; e += NT_SOLVE) {
+        const int kk = e % L0;
+        const int p = (e / L0) % NP;
+        const int c = e / (L0 * NP);
+        const cpx<R> *ra = spec + ((long long)c * k.rows + row0 + 2 * p) * k.hp;
+        const cpx<R> *rb = ra + k.hp;
+        cpx<R> Z;
+        if (kk == 0 || kk == H) {
+            Z = {ra[kk].x, rb[kk].x};
+        } else if (kk < H) {
+            const cpx<R> a = ra[kk], b = rb[kk];
+            Z = {a.x - b.y, a.y + b.x};
+        } else {
+            const cpx<R> a = ra[L0 - kk], b = rb[L0 - kk];
+            Z = {a.x + b.y, -a.y + b.x};
+        }
+        buf[kk * LS + c * NP + p] = Z;
+    }
+    __syncthreads();
+    fft_lines<R, L0, NL, LS, NT_SOLVE>(buf, tw, (R)1);
+    const R *bufr = reinterpret_cast<const R *>(buf);
+    double acc[C];
+#pragma unroll
+    for (int c = 0; c < C; ++c) acc[c] = 0.0;
+    int bad = 0;
+    for (int e = tid; e < NR * L0; e += NT_SOLVE) {
+        const int rl = e / L0, i0 = e % L0;
+        const long long v = (row0 + rl) * L0 + i0;
+        double ev[C], rt[C], d[C], s[C];
+#pragma unroll
+        for (int c = 0; c < C; ++c) ev[c] = eps0[(long long)c * k.N + v];
+        round_trip<C>(ev, k.epsavg, k.lambda, k.mu, rt);
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+            const double z = (double)bufr[2 * (i0 * LS + c * NP + (rl >> 1)) + (rl & 1)];
+            bad |= (z != z);
+            const double tot = z / (double)k.N + k.epsavg[c];   // STZ/elastic.c:822
+            d[c] = tot - rt[c];
+        }
+        stress_of<C>(d, k.lambda, k.mu, s);
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+            sigma[(long long)c * k.N + v] = (R)s[c];
+            acc[c] += s[c];
+        }
+    }
+    if (bad) atomicOr(nanflag, 1);
+    // fixed-shape tree reduction of each component
+    for (int c = 0; c < C; ++c) {
+        red[tid] = acc[c];
+        __syncthreads();
+        for (int w = NT_SOLVE / 2; w > 0; w >>= 1) {
+            if (tid < w) red[tid] = red[tid] + red[tid + w];
+            __syncthreads();
+        }
+        if (tid == 0) spart[(long long)blockIdx.x * C + c] = red[0];
+        __syncthreads();
+    }
+}
+
+template <int C>
+__global__ __launch_bounds__(NT_SOLVE) void k_mean(const double *__restrict__ spart, int nblocks, double invN,
+                                                   double *__restrict__ out)
+{
+    __shared__ double red[NT_SOLVE];
+    const int tid = threadIdx.x;
+    for (int c = 0; c < C; ++c) {
+        double a = 0.0;
+        for (int b = tid; b < nblocks; b += NT_SOLVE) a += spart[(long long)b * C + c];
+        red[tid] = a;
+        __syncthreads();
+        for (int w = NT_SOLVE / 2; w > 0; w >>= 1) {
+            if (tid < w) red[tid] = red[tid] + red[tid + w];
+            __syncthreads();
+        }
+        if (tid == 0) out[c] = red[0] * invN;
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------------
+bool supported_len(int L)
+{
+    switch (L) {
+    case 4: case 8: case 16: case 32: case 64: case 96: case 128: case 256: case 512:
+        return true;
+    default:
+        return false;
+    }
+}
+
+// rows per workgroup for the x passes: about 1024 complex values per component group
+constexpr int rows_for(int L0) { return L0 >= 512 ? 2 : (L0 >= 256 ? 4 : (L0 >= 128 ? 8 : (L0 >= 64 ? 16 : 32))); }
+// k0 columns per tile for the strided passes
+constexpr int cols_for(int L) { return L >= 512 ? 4 : (L >= 256 ? 8 : 16); }
+constexpr int fused_cols_for(int L, int C) { return C == 6 ? (L >= 256 ? 4 : 8) : (L >= 512 ? 4 : (L >= 256 ? 8 : 16)); }
+
+template <typename R>
+struct Launch {
+    stz_ctx *ctx;
+    SolveConsts k;
+    int *nanflag;
+
+    // largest power-of-two row count <= rows_for(L0) that divides the row total
+    int pick_rows(int L0max) const
+    {
+        int NR = L0max;
+        while (NR > 2 && (k.rows % NR) != 0) NR /= 2;
+        return (k.rows % NR) == 0 ? NR : 0;
+    }
+
+    template <int L0, int NRV, int C>
+    void go_fwd_x(dim3 grid)
+    {
+        if constexpr (NRV <= rows_for(L0))
+            hipLaunchKernelGGL((k_fwd_x<R, L0, NRV, C>), grid, dim3(NT_SOLVE), 0, ctx->stream, ctx->eps0,
+                               (cpx<R> *)ctx->spec, (const cpx<R> *)ctx->tw.tw[0], k);
+    }
+
+    template <int L0, int NRV, int C>
+    void go_inv_x(dim3 grid, double *spart)
+    {
+        if constexpr (NRV <= rows_for(L0))
+            hipLaunchKernelGGL((k_inv_x<R, L0, NRV, C>), grid, dim3(NT_SOLVE), 0, ctx->stream, (const cpx<R> *)ctx->spec,
+                               ctx->eps0, (R *)ctx->sigma, spart, nanflag, (const cpx<R> *)ctx->tw.tw[0], k);
+    }
+
+    template <int L0, int C>
+    int fwd_x_n()
+    {
+        const int NR = pick_rows(rows_for(L0));
+        if (!NR) return STZ_E_ARG;
+        const dim3 grid(k.rows / NR);
+        switch (NR) {
+        case 32: go_fwd_x<L0, 32, C>(grid); break;
+        case 16: go_fwd_x<L0, 16, C>(grid); break;
+        case 8: go_fwd_x<L0, 8, C>(grid); break;
+        case 4: go_fwd_x<L0, 4, C>(grid); break;
+        default: go_fwd_x<L0, 2, C>(grid); break;
+        }
+        STZ_CHECK_LAUNCH();
+        return STZ_OK;
+    }
+
+    template <int L0, int C>
+    int inv_x_n(double *spart, int *nblocks)
+    {
+        const int NR = pick_rows(rows_for(L0));
+        if (!NR) return STZ_E_ARG;
+        *nblocks = k.rows / NR;
+        const dim3 grid(k.rows / NR);
+        switch (NR) {
+        case 32: go_inv_x<L0, 32, C>(grid, spart); break;
+        case 16: go_inv_x<L0, 16, C>(grid, spart); break;
+        case 8: go_inv_x<L0, 8, C>(grid, spart); break;
+        case 4: go_inv_x<L0, 4, C>(grid, spart); break;
+        default: go_inv_x<L0, 2, C>(grid, spart); break;
+        }
+        STZ_CHECK_LAUNCH();
+        return STZ_OK;
+    }
+
+    template <int L1>
+    int ypass_n(R sgn)
+    {
+        constexpr int W = cols_for(L1);
+        const dim3 grid((k.h0 + W - 1) / W, k.n[2], ctx->C);
+        hipLaunchKernelGGL((k_ypass<R, L1, W>), grid, dim3(NT_SOLVE), 0, ctx->stream, (cpx<R> *)ctx->spec,
+                           (const cpx<R> *)ctx->tw.tw[1], k, sgn);
+        STZ_CHECK_LAUNCH();
+        return STZ_OK;
+    }
+
+    template <int LZ, int C, int DIM>
+    int fused_n()
+    {
+        constexpr int W = fused_cols_for(LZ, C);
+        const dim3 grid((k.h0 + W - 1) / W, DIM == 3 ? k.n[1] : 1);
+        hipLaunchKernelGGL((k_fused<R, LZ, W, C, DIM>), grid, dim3(NT_SOLVE), 0, ctx->stream, (cpx<R> *)ctx->spec,
+                           (const cpx<R> *)ctx->tw.tw[DIM - 1], k);
+        STZ_CHECK_LAUNCH();
+        return STZ_OK;
+    }
+};
+
+#define STZ_LEN_DISPATCH(L, ...)                                \
+    switch (L) {                                                \
+    case 4: { constexpr int LL = 4; __VA_ARGS__; } break;       \
+    case 8: { constexpr int LL = 8; __VA_ARGS__; } break;       \
+    case 16: { constexpr int LL = 16; __VA_ARGS__; } break;     \
+    case 32: { constexpr int LL = 32; __VA_ARGS__; } break;     \
+    case 64: { constexpr int LL = 64; __VA_ARGS__; } break;     \
+    case 96: { constexpr int LL = 96; __VA_ARGS__; } break;     \
+    case 128: { constexpr int LL = 128; __VA_ARGS__; } break;   \
+    case 256: { constexpr int LL = 256; __VA_ARGS__; } break;   \
+    case 512: { constexpr int LL = 512; __VA_ARGS__; } break;   \
+    default: return STZ_E_ARG;                                  \
+    }
+
+template <typename R>
+int solve_impl(stz_ctx *ctx)
+{
+    Launch<R> Lr;
+    Lr.ctx = ctx;
+    SolveConsts &k = Lr.k;
+    k.lambda = ctx->k.lambda;
+    k.mu = ctx->k.mu;
+    k.alpha = ctx->k.alpha;
+    k.s0 = 1.0 / ctx->k.mu;
+    k.sh = 0.5 / ctx->k.mu;
+    k.sa = -1.0 * ctx->k.alpha / ctx->k.mu;
+    k.invN = 1.0 / (double)ctx->N;
+    for (int c = 0; c < 6; ++c) k.epsavg[c] = ctx->epsavg[c];
+    for (int a = 0; a < 3; ++a) k.n[a] = ctx->n[a];
+    k.N = ctx->N;
+    k.rows = ctx->n[1] * ctx->n[2];
+    k.h0 = ctx->h0;
+    k.hp = ctx->hp;
+    Lr.nanflag = reinterpret_cast<int *>(ctx->res + 15);
+    int rc = STZ_OK;
+    const int C = ctx->C;
+    tmark(ctx, 0);
+    if (C == 3) {
+        STZ_LEN_DISPATCH(ctx->n[0], rc = Lr.template fwd_x_n<LL, 3>());
+        if (rc) return rc;
+        tmark(ctx, 1);
+        tmark(ctx, 2);
+        STZ_LEN_DISPATCH(ctx->n[1], rc = (Lr.template fused_n<LL, 3, 2>()));
+        if (rc) return rc;
+        tmark(ctx, 3);
+        tmark(ctx, 4);
+    } else {
+        STZ_LEN_DISPATCH(ctx->n[0], rc = Lr.template fwd_x_n<LL, 6>());
+        if (rc) return rc;
+        tmark(ctx, 1);
+        STZ_LEN_DISPATCH(ctx->n[1], rc = Lr.template ypass_n<LL>((R)-1));
+        if (rc) return rc;
+        tmark(ctx, 2);
+        STZ_LEN_DISPATCH(ctx->n[2], rc = (Lr.template fused_n<LL, 6, 3>()));
+        if (rc) return rc;
+        tmark(ctx, 3);
+        STZ_LEN_DISPATCH(ctx->n[1], rc = Lr.template ypass_n<LL>((R)1));
+        if (rc) return rc;
+        tmark(ctx, 4);
+    }
+    int nblocks = 0;
+    if (C == 3) {
+        STZ_LEN_DISPATCH(ctx->n[0], rc = (Lr.template inv_x_n<LL, 3>(ctx->spart, &nblocks)));
+    } else {
+        STZ_LEN_DISPATCH(ctx->n[0], rc = (Lr.template inv_x_n<LL, 6>(ctx->spart, &nblocks)));
+    }
+    if (rc) return rc;
+    tmark(ctx, 5);
+    if (C == 3) hipLaunchKernelGGL((k_mean<3>), dim3(1), dim3(NT_SOLVE), 0, ctx->stream, ctx->spart, nblocks, k.invN, ctx->res);
+    else hipLaunchKernelGGL((k_mean<6>), dim3(1), dim3(NT_SOLVE), 0, ctx->stream, ctx->spart, nblocks, k.invN, ctx->res);
+    STZ_CHECK_LAUNCH();
+    tmark(ctx, 6);
+    return STZ_OK;
+}
+
+template <typename R>
+int upload_twiddles(stz_ctx *ctx, int axis, int L)
+{
+    std::vector<cpx<R>> h(L);
+    for (int t = 0; t < L; ++t) {
+        long double a = 2.0L * 3.141592653589793238462643383279502884L * (long double)t / (long double)L;
+        h[t].x = (R)(double)cosl(a);
+        h[t].y = (R)(-(double)sinl(a));
+    }
+    STZ_HIP(hipMalloc(&ctx->tw.tw[axis], sizeof(cpx<R>) * L));
+    STZ_HIP(hipMemcpy(ctx->tw.tw[axis], h.data(), sizeof(cpx<R>) * L, hipMemcpyHostToDevice));
+    return STZ_OK;
+}
+
+}  // namespace
+
+bool stz_solve_supported(int dim, const int *n, int prec)
+{
+    (void)prec;
+    for (int a = 0; a < dim; ++a)
+        if (!supported_len(n[a])) return false;
+    if (dim == 2 && n[2] != 1) return false;
+    return true;
+}
+
+int stz_fft_setup(stz_ctx *ctx)
+{
+    for (int a = 0; a < ctx->dim; ++a) {
+        int rc = ctx->prec == STZ_FP64 ? upload_twiddles<double>(ctx, a, ctx->n[a])
+                                       : upload_twiddles<float>(ctx, a, ctx->n[a]);
+        if (rc) return rc;
+    }
+    return STZ_OK;
+}
+
+void stz_fft_free(stz_ctx *ctx)
+{
+    for (int a = 0; a < 3; ++a) {
+        if (ctx->tw.tw[a]) (void)hipFree(ctx->tw.tw[a]);
+        ctx->tw.tw[a] = nullptr;
+    }
+}
+
+int stz_solve_launch(stz_ctx *ctx)
+{
+    return ctx->prec == STZ_FP64 ? solve_impl<double>(ctx) : solve_impl<float>(ctx);
+}
+
+// rows of the stress-mean partial array: the largest block count of k_inv_x
+long long stz_solve_partials(const stz_ctx *ctx)
+{
+    return (long long)ctx->n[1] * ctx->n[2];
+}

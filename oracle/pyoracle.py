@@ -91,6 +91,9 @@ def lib():
         L.ora_elastic_energy.restype = C.c_double
         L.ora_step.argtypes = [C.c_void_p, P(OraStepInfo)]
         L.ora_set_print.argtypes = [C.c_void_p, C.c_int, C.c_double]
+        L.ora_set_strainhist.argtypes = [C.c_void_p, C.c_double]
+        L.ora_set_threads.argtypes = [C.c_int]
+        L.ora_set_threads.restype = C.c_int
         _lib = L
     return _lib
 
@@ -196,6 +199,11 @@ class Oracle:
         info = OraStepInfo()
         rc = lib().ora_step(self._h, C.byref(info))
         return rc, info
+
+
+def set_threads(n):
+    """OpenMP threads of the oracle; results do not depend on the count."""
+    return lib().ora_set_threads(int(n))
 
 
 def mt_stream(seed, n):

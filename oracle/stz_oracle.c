@@ -44,6 +44,10 @@
  *   ora_step          main loop body, STZ/bmg.c:138-411, KMC loading
  *                     STZ/evolution.c:262-358.
  *
+ * Threads: element-wise loops and FFT lines run under OpenMP; every floating-point
+ * reduction (rate total, prefix, <sigma>) stays sequential in the reference order, so
+ * results are bit-identical for any thread count (tests/test_oracle_kmc.py checks it).
+ *
  * Deliberate restatement choices (each observable-equivalent):
  *   - PreDEps0[M][C][N] + PreSelectMode[N] are stored as deps_last[S][N]:
  *     the reference only ever reads PreDEps0[PreSelectMode[v]][.][v]
@@ -63,6 +67,9 @@
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 /* ------------------------------------------------------------------ */
 /* constants of STZ/bmg.h:42,68-74                                       */
@@ -228,25 +235,37 @@ int ora_fft3d(double *data, int n0, int n1, int n2, int sign)
         if (n == 1) continue;
         plan1d p;
         if (plan_init(&p, n)) return -1;
-        double *line = (double *)malloc(sizeof(double) * 4 * (size_t)n);
-        if (!line) { plan_free(&p); return -1; }
-        double *work = line + 2 * n;
         const long st = strides[ax];
-        for (long base = 0; base < ntot; base++) {
-            /* visit each line once: base has index 0 along ax */
-            if ((base / st) % n != 0) continue;
-            for (int i = 0; i < n; i++) {
-                line[2 * i] = data[2 * (base + i * st)];
-                line[2 * i + 1] = data[2 * (base + i * st) + 1];
-            }
-            fft1d(&p, line, work, sign);
-            for (int i = 0; i < n; i++) {
-                data[2 * (base + i * st)] = line[2 * i];
-                data[2 * (base + i * st) + 1] = line[2 * i + 1];
+        const long nlines = ntot / n;
+        int bad = 0;
+#pragma omp parallel
+        {
+            double *line = (double *)malloc(sizeof(double) * 4 * (size_t)n);
+            if (!line) {
+#pragma omp atomic write
+                bad = 1;
+            } else {
+                double *work = line + 2 * n;
+#pragma omp for schedule(static)
+                for (long li = 0; li < nlines; li++) {
+                    /* line li: index 0 along ax; lower axes fastest, then higher */
+                    const long lo = li % st, hi = li / st;
+                    const long base = hi * st * n + lo;
+                    for (int i = 0; i < n; i++) {
+                        line[2 * i] = data[2 * (base + i * st)];
+                        line[2 * i + 1] = data[2 * (base + i * st) + 1];
+                    }
+                    fft1d(&p, line, work, sign);
+                    for (int i = 0; i < n; i++) {
+                        data[2 * (base + i * st)] = line[2 * i];
+                        data[2 * (base + i * st) + 1] = line[2 * i + 1];
+                    }
+                }
+                free(line);
             }
         }
-        free(line);
         plan_free(&p);
+        if (bad) return -1;
     }
     return 0;
 }
@@ -550,6 +569,19 @@ void ora_set_print(ora_state *s, int printperiod, double endstrain)
     s->endstrain = endstrain;
 }
 
+void ora_set_strainhist(ora_state *s, double last) { s->strainhist_last = last; }
+
+int ora_set_threads(int n)
+{
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+    return omp_get_max_threads();
+#else
+    (void)n;
+    return 1;
+#endif
+}
+
 /* ------------------------------------------------------------------ */
 /* stress solve                                                          */
 /* ------------------------------------------------------------------ */
@@ -564,10 +596,12 @@ static void hom_solve(ora_state *s)
     double *sp = s->spec;
     for (int c = 0; c < C; c++) {
         double *f = sp + 2 * (long)c * N;
+#pragma omp parallel for schedule(static)
         for (long v = 0; v < N; v++) { f[2 * v] = s->epsolast[(long)c * N + v]; f[2 * v + 1] = 0.; }
         ora_fft3d(f, s->p.n[0], s->p.n[1], s->p.n[2], -1);
     }
     const double s0 = 1.0 / mu, sh = 0.5 / mu, sa = -1.0 * alpha / mu;
+#pragma omp parallel for schedule(static)
     for (long v = 0; v < N; v++) {
         for (int ri = 0; ri < 2; ri++) {
             double e[6] = {0}, t[6];
@@ -602,9 +636,11 @@ static void hom_solve(ora_state *s)
     for (int c = 0; c < C; c++) {
         double *f = sp + 2 * (long)c * N;
         ora_fft3d(f, s->p.n[0], s->p.n[1], s->p.n[2], +1);
+#pragma omp parallel for schedule(static)
         for (long v = 0; v < N; v++) s->epso[(long)c * N + v] = f[2 * v] / (double)N + s->epsavg[c]; /* :822 */
     }
     /* IsotropicMul_R2, STZ/elastic.c:686-701 */
+#pragma omp parallel for schedule(static)
     for (long v = 0; v < N; v++) {
         double d[6] = {0};
         for (int c = 0; c < C; c++) d[c] = s->epso[(long)c * N + v] - s->epsolast[(long)c * N + v];
@@ -633,6 +669,7 @@ void ora_stress_solve(ora_state *s)
     const long N = s->N;
     const int C = s->C;
     const double lambda = s->lambda, mu = s->mu;
+#pragma omp parallel for schedule(static)
     for (long v = 0; v < N; v++) {
         double e[6] = {0}, t0[6], t1[6], tau[6], z[6] = {0}, r[6];
         for (int c = 0; c < C; c++) { e[c] = s->eps0[(long)c * N + v]; z[c] = e[c] + s->epsavg[c]; }
@@ -665,6 +702,7 @@ void ora_stress_solve(ora_state *s)
 /* ------------------------------------------------------------------ */
 void ora_softening(ora_state *s)
 {
+#pragma omp parallel for schedule(static)
     for (long v = 0; v < s->N; v++)
         s->soft[v] = s->perm[v] + s->temp[v] * exp((s->tlast[v] - s->totime) / s->tTemp); /* :731 */
 }
@@ -675,6 +713,7 @@ long long ora_act_energy(ora_state *s)
     const int C = s->C;
     long long nneg = 0;
     for (int m = 0; m < s->M; m++) {
+#pragma omp parallel for schedule(static) reduction(+ : nneg)
         for (long v = 0; v < N; v++) {
             double sg[6] = {0}, d[6] = {0}, pd[6] = {0};
             for (int c = 0; c < C; c++) sg[c] = s->sigma[(long)c * N + v];
@@ -698,10 +737,9 @@ double ora_total_rate(ora_state *s)
 {
     const long MN = (long)s->M * s->N;
     double t = 0;
-    for (long i = 0; i < MN; i++) {
-        s->rate[i] = s->ratepre * exp(-s->q[i] / s->kT);   /* STZ/evolution.c:125 */
-        t += s->rate[i];                                  /* :126 */
-    }
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < MN; i++) s->rate[i] = s->ratepre * exp(-s->q[i] / s->kT);   /* STZ/evolution.c:125 */
+    for (long i = 0; i < MN; i++) t += s->rate[i];                                  /* :126, sequential */
     return t;
 }
 
@@ -712,6 +750,7 @@ int ora_select(ora_state *s, double dice, int *site, int *mode)
     double R = 0;
     for (long i = 0; i < MN; i++) R += s->rate[i];
     double tk = 1 / R;
+#pragma omp parallel for schedule(static)
     for (long i = 0; i < MN; i++) s->rate[i] *= tk;
     for (long i = 1; i < MN; i++) s->rate[i] += s->rate[i - 1];
     long ev;
@@ -780,12 +819,18 @@ static int athermal(ora_state *s, ora_step_info *info)
     const long N = s->N;
     int ntr = 0;
     s->astep++;
+    int *pick = (int *)malloc(sizeof(int) * (size_t)N);
+#pragma omp parallel for schedule(static)
     for (long v = 0; v < N; v++) {
         int mm = 0;
         double minq = s->q[v];
         for (int m = 0; m < s->M; m++)
             if (minq > s->q[(long)m * N + v]) { mm = m; minq = s->q[(long)m * N + v]; }
-        if (minq < 0 || fabs(minq) < s->p.athermal_threshold) {
+        pick[v] = (minq < 0 || fabs(minq) < s->p.athermal_threshold) ? mm : -1;
+    }
+    for (long v = 0; v < N; v++) {
+        const int mm = pick[v];
+        if (mm >= 0) {
             double sg[6] = {0}, d[6] = {0};
             for (int c = 0; c < s->C; c++) sg[c] = s->sigma[(long)c * N + v];
             deps_of(s, mm, v, d);
@@ -796,6 +841,7 @@ static int athermal(ora_state *s, ora_step_info *info)
             ntr++;
         }
     }
+    free(pick);
     ora_stress_solve(s);
     athermal_correction(s);
     info->kind = ORA_ATHERMAL;
@@ -828,6 +874,7 @@ static int kmc(ora_state *s, ora_step_info *info)
                 if (c != lc) s->epsavg[c] += -te[c];
             s->epsavg[lc] += sr * t_kmc;
             s->stress[lc] += 2 * s->mu * sr * t_kmc;
+#pragma omp parallel for schedule(static)
             for (long v = 0; v < N; v++) s->sigma[(long)lc * N + v] += 2 * s->mu * sr * t_kmc;
         } else {                              /* TENSION, STZ/evolution.c:281-297 */
             ts[0] = 0.0;
@@ -841,6 +888,7 @@ static int kmc(ora_state *s, ora_step_info *info)
                 for (int c = 3; c < 6; c++) s->epsavg[c] += (-te[c]);
             }
             s->stress[0] += s->E * sr * t_kmc;
+#pragma omp parallel for schedule(static)
             for (long v = 0; v < N; v++) s->sigma[v] += s->E * sr * t_kmc;
         }
         s->totime += t_kmc;
@@ -855,6 +903,7 @@ static int kmc(ora_state *s, ora_step_info *info)
         if (s->p.loading == 1) {              /* STZ/evolution.c:349-356 */
             s->epsavg[lc] += sr * 1;
             s->stress[lc] += 2 * s->mu * inc;
+#pragma omp parallel for schedule(static)
             for (long v = 0; v < N; v++) s->sigma[(long)lc * N + v] += 2 * s->mu * inc;
         } else {                              /* STZ/evolution.c:335-347 */
             s->epsavg[0] += (inc * 1);
@@ -862,6 +911,7 @@ static int kmc(ora_state *s, ora_step_info *info)
             if (C == 3) s->epsavg[2] += (inc * 0.0);
             else { s->epsavg[2] += (inc * (-s->nu)); for (int c = 3; c < 6; c++) s->epsavg[c] += (inc * 0.0); }
             s->stress[0] += s->E * inc;
+#pragma omp parallel for schedule(static)
             for (long v = 0; v < N; v++) s->sigma[v] += s->E * inc;
         }
         info->kind = ORA_ELASTIC;

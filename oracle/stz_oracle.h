@@ -108,6 +108,8 @@ double ora_elastic_energy(ora_state *s);             /* ElasticEnergy (sum) */
 /* one iteration of the main loop (STZ/bmg.c:138-411); returns 0, or <0 on error, 1 on termination */
 int ora_step(ora_state *s, ora_step_info *info);
 void ora_set_print(ora_state *s, int printperiod, double endstrain);
+void ora_set_strainhist(ora_state *s, double last); /* strainhist[N_print-1], STZ/bmg.c:182 */
+int  ora_set_threads(int n);   /* OpenMP threads (n <= 0: query); returns the count in use */
 
 #ifdef __cplusplus
 }

@@ -1,14 +1,34 @@
-"""Writes the small golden fixtures under tests/golden/.
+"""Writes the golden fixtures under tests/golden/.
 
-fft_vectors.npz: numpy.fft.fftn of fixed seeded inputs (2-D and 3-D, radix 2/3).
+fft_vectors.npz      numpy.fft.fftn of fixed seeded inputs (2-D and 3-D, radix 2/3).
+trace_<name>.npz     event traces of the CPU FP64 oracle (oracle/liboracle.so) for the
+                     workloads below: per step kind, site, mode, transformed count,
+                     clock and loaded-component mean stress.  The GPU parity tests
+                     replay the same seed and compare step by step.
+
 Run: python tests/golden/make_golden.py fft
+     python tests/golden/make_golden.py trace ref2d_512 [events]
 """
 import os
 import sys
+import time
 
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+
+# reference input file (STZ/input) on the reference's 512x512 2-D grid, tension
+REF2D = dict(dim=2, n=(512, 512, 1), seed=123, loading=0)
+# 3-D simple shear, mu = 35 GPa, nu = 0.35, gamma0 = 0.1, Q0 ~ N(2.0, 0.2), seed 1234
+SHEAR3D = dict(dim=3, E=94.5, nu=0.35, double_nu=0, gamma0=0.1, q0_std=0.2, seed=1234, loading=1,
+               athermal_correction=1)
+TRACES = {
+    "ref2d_512": dict(REF2D),
+    "shear3d_64": dict(SHEAR3D, n=(64, 64, 64)),
+    "shear3d_128": dict(SHEAR3D, n=(128, 128, 128)),
+    "shear2d_256": dict(dim=2, n=(256, 256, 1), seed=7, loading=1),
+}
 
 
 def make_fft():
@@ -22,7 +42,39 @@ def make_fft():
     np.savez_compressed(os.path.join(HERE, "fft_vectors.npz"), **out)
 
 
+def make_trace(name, events, threads=8, printperiod=5, endstrain=1.0):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as po
+    po.set_threads(threads)
+    kw = TRACES[name]
+    t0 = time.time()
+    o = po.Oracle(**kw)
+    o.set_print(printperiod, endstrain)
+    rows = []
+    nev = 0
+    lc = 0 if kw.get("loading", 0) == 0 else (2 if kw["dim"] == 2 else 5)
+    while nev < events:
+        rc, info = o.step()
+        rows.append((info.kind, info.site, info.mode, info.n_transformed, info.totime, info.stress[lc],
+                     info.printed))
+        nev += info.kind == 1
+        if len(rows) % 500 == 0:
+            print(f"{name}: {len(rows)} steps, {nev} events, {time.time() - t0:.0f}s", flush=True)
+        if rc:
+            break
+    a = np.array(rows, dtype=object)
+    np.savez_compressed(os.path.join(HERE, f"trace_{name}.npz"),
+                        kind=a[:, 0].astype(np.int8), site=a[:, 1].astype(np.int32),
+                        mode=a[:, 2].astype(np.int8), ntrans=a[:, 3].astype(np.int32),
+                        totime=a[:, 4].astype(np.float64), stress=a[:, 5].astype(np.float64),
+                        printed=a[:, 6].astype(np.int8), printperiod=printperiod, endstrain=endstrain,
+                        params=np.array(repr(sorted(kw.items()))))
+    print(f"{name}: done, {len(rows)} steps, {nev} events, {time.time() - t0:.0f}s", flush=True)
+
+
 if __name__ == "__main__":
     what = sys.argv[1:] or ["fft"]
-    if "fft" in what:
+    if what[0] == "fft":
         make_fft()
+    elif what[0] == "trace":
+        make_trace(what[1], int(what[2]) if len(what) > 2 else 10000)

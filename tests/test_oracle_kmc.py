@@ -170,3 +170,17 @@ def test_trajectory_deterministic_and_sane():
             assert ntr >= 1
     c = a.counters()
     assert c[0] == len(ta)
+
+
+def test_thread_count_does_not_change_results():
+    kw = dict(dim=3, n=(16, 16, 8), modes=20, seed=77, loading=1, E=94.5, nu=0.35, double_nu=0,
+              gamma0=0.1, q0_std=0.2, athermal_correction=1)
+    traces = []
+    for th in (1, 4):
+        po.set_threads(th)
+        o = po.Oracle(**kw)
+        o.set_print(5, 1.0)
+        traces.append((_trace(o, 300), o.field(po.F_SIGMA).copy()))
+    po.set_threads(0)
+    assert traces[0][0] == traces[1][0]
+    assert np.array_equal(traces[0][1], traces[1][1])
