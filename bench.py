@@ -66,17 +66,15 @@ def restore(g, snap, mg):
 
 
 def run_steps(g, k):
+    t0 = time.perf_counter()
+    rc, res = g.run_steps(k)          # the loop runs inside the library; each step synchronises
+    dt = time.perf_counter() - t0
     kinds = np.zeros(3, dtype=np.int64)
     events = []
-    t0 = time.perf_counter()
-    for _ in range(k):
-        rc, r = g.step()
+    for r in res:
         kinds[r.kind] += 1
         if r.kind == 1:
             events.append((r.site, r.mode))
-        if rc:
-            break
-    dt = time.perf_counter() - t0
     return dt, kinds, events
 
 

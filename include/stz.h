@@ -167,6 +167,11 @@ int stz_step(stz_ctx *ctx, stz_step_result *res);
 int stz_step_kmc(stz_ctx *ctx, stz_step_result *res);
 int stz_step_print(stz_ctx *ctx, stz_step_result *res);
 
+/* Runs up to nsteps iterations of stz_step (the reference's while(1) loop body), stopping
+   early on termination (return 1) or error (< 0).  results (optional) receives one entry
+   per completed step; done receives the number of completed steps. */
+int stz_run_steps(stz_ctx *ctx, int nsteps, stz_step_result *results, int *done);
+
 /* Host RNG of the context (GSL mt19937 + gsl_ran_gaussian polar method). */
 int stz_rng_seed(stz_ctx *ctx, unsigned long seed);
 int stz_rng_uniform(stz_ctx *ctx, double *u);

@@ -63,11 +63,51 @@ __global__ void k_add(R *p, long long n, double d)
     if (i < n) p[i] = p[i] + (R)d;
 }
 
+// waits for the stream by polling, which returns sooner than a blocking wait
+int spin_sync(stz_ctx *ctx)
+{
+    for (;;) {
+        const hipError_t e = hipStreamQuery(ctx->stream);
+        if (e == hipSuccess) return STZ_OK;
+        if (e != hipErrorNotReady) {
+            ctx->err = std::string("hipStreamQuery: ") + hipGetErrorString(e);
+            return STZ_E_HIP;
+        }
+    }
+}
+
 float elapsed(stz_ctx *ctx, int a, int b)
 {
     float ms = 0.f;
     if (hipEventElapsedTime(&ms, ctx->tev[a], ctx->tev[b]) != hipSuccess) return 0.f;
     return ms;
+}
+
+void acc_rates(stz_ctx *ctx)
+{
+    if (!ctx->timing) return;
+    ctx->tms[0] += elapsed(ctx, 8, 9);
+    ctx->tn[0]++;
+    ctx->tms[5] += elapsed(ctx, 9, 10);
+    ctx->tn[5]++;
+}
+
+void acc_select(stz_ctx *ctx)
+{
+    if (!ctx->timing) return;
+    ctx->tms[2] += elapsed(ctx, 11, 12);
+    ctx->tn[2]++;
+}
+
+void acc_solve(stz_ctx *ctx)
+{
+    if (!ctx->timing) return;
+    for (int i = 0; i < 6; ++i) {
+        ctx->sms[i] += elapsed(ctx, i, i + 1);
+        ctx->sn[i]++;
+    }
+    ctx->tms[1] += elapsed(ctx, 0, 6);
+    ctx->tn[1]++;
 }
 
 int dalloc(stz_ctx *ctx, void **p, size_t bytes)
@@ -132,6 +172,19 @@ int flush_pending(stz_ctx *ctx)
     return STZ_OK;
 }
 
+// marks every cached Q0*exp(-D*S) stale (after an upload of any of its inputs)
+int invalidate_cache(stz_ctx *ctx)
+{
+    const long long n = ctx->N;
+    const dim3 g((unsigned)((n + 255) / 256));
+    if (ctx->rsz == 8)
+        hipLaunchKernelGGL((k_fill<double>), g, dim3(256), 0, ctx->stream, (double *)ctx->scache, n, std::nan(""));
+    else
+        hipLaunchKernelGGL((k_fill<float>), g, dim3(256), 0, ctx->stream, (float *)ctx->scache, n, std::nanf(""));
+    STZ_CHECK_LAUNCH();
+    return STZ_OK;
+}
+
 int ensure_dnew(stz_ctx *ctx, size_t count)
 {
     if (count <= ctx->dnew_cap) return STZ_OK;
@@ -162,7 +215,7 @@ int solve(stz_ctx *ctx)
     ctx->pend_delta = 0.0;
     for (int c = 0; c < 6; ++c) ctx->soff[c] = 0.0;
     STZ_HIP(hipMemsetAsync(ctx->res + 15, 0, sizeof(double), ctx->stream));
-    int rc = stz_solve_launch(ctx);
+    int rc = stz_solve_launch(ctx, 0);
     if (rc) return rc;
     STZ_HIP(hipMemcpyAsync(ctx->hres, ctx->res, 16 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     STZ_HIP(hipStreamSynchronize(ctx->stream));
@@ -174,14 +227,7 @@ int solve(stz_ctx *ctx)
     }
     for (int c = 0; c < ctx->C; ++c) ctx->stress[c] = ctx->hres[c];
     ctx->nsolve++;
-    if (ctx->timing) {
-        for (int i = 0; i < 6; ++i) {
-            ctx->sms[i] += elapsed(ctx, i, i + 1);
-            ctx->sn[i]++;
-        }
-        ctx->tms[1] += elapsed(ctx, 0, 6);
-        ctx->tn[1]++;
-    }
+    acc_solve(ctx);
     return STZ_OK;
 }
 
@@ -202,19 +248,14 @@ int add_uniform(stz_ctx *ctx, int comp, double delta)
 
 int rates(stz_ctx *ctx, long long *nneg, double *R)
 {
-    int rc = stz_rates_launch(ctx);
+    int rc = stz_rates_launch(ctx, 0);
     if (rc) return rc;
     ctx->pend_comp = -1;     // consumed by the rate pass
     ctx->pend_delta = 0.0;
     STZ_HIP(hipMemcpyAsync(ctx->hres + 8, ctx->res + 8, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     STZ_HIP(hipStreamSynchronize(ctx->stream));
     ctx->rate_total = ctx->hres[8];
-    if (ctx->timing) {
-        ctx->tms[0] += elapsed(ctx, 8, 9);
-        ctx->tn[0]++;
-        ctx->tms[5] += elapsed(ctx, 9, 10);
-        ctx->tn[5]++;
-    }
+    acc_rates(ctx);
     *R = ctx->hres[8];
     *nneg = (long long)ctx->hres[9];
     return STZ_OK;
@@ -226,14 +267,11 @@ int select_event(stz_ctx *ctx, double dice, int *site, int *mode, double *q)
         ctx->err = "KMC failed to locate an activation mode";
         return STZ_E_SELECT;
     }
-    int rc = stz_select_launch(ctx, dice * ctx->rate_total, dice);
+    int rc = stz_select_launch(ctx, dice * ctx->rate_total, 0);
     if (rc) return rc;
     STZ_HIP(hipMemcpyAsync(ctx->hres + 10, ctx->res + 10, 3 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     STZ_HIP(hipStreamSynchronize(ctx->stream));
-    if (ctx->timing) {
-        ctx->tms[2] += elapsed(ctx, 11, 12);
-        ctx->tn[2]++;
-    }
+    acc_select(ctx);
     if (ctx->hres[10] < 0) {
         ctx->err = "KMC failed to locate an activation mode";
         return STZ_E_SELECT;
@@ -250,7 +288,7 @@ int apply_event(stz_ctx *ctx, int site, int mode, const double *dnew)
     int rc = ensure_dnew(ctx, ms);
     if (rc) return rc;
     STZ_HIP(hipMemcpyAsync(ctx->dnew, dnew, ms * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
-    rc = stz_apply_launch(ctx, site, mode);
+    rc = stz_apply_launch(ctx, site, mode, 0);
     if (rc) return rc;
     ctx->ntrans++;
     return STZ_OK;
@@ -417,9 +455,11 @@ int stz_create(const stz_params *p, stz_ctx **out)
     rc = rc ? rc : dalloc(ctx, (void **)&ctx->tlast, N * 8);
     rc = rc ? rc : dalloc(ctx, &ctx->spec, (size_t)ctx->C * rows * ctx->hp * 2 * R);
     rc = rc ? rc : dalloc(ctx, &ctx->minq, N * R);
+    rc = rc ? rc : dalloc(ctx, &ctx->q0eff, ctx->M * N * R);
+    rc = rc ? rc : dalloc(ctx, &ctx->scache, N * R);
     rc = rc ? rc : dalloc(ctx, (void **)&ctx->argm, N);
     rc = rc ? rc : dalloc(ctx, (void **)&ctx->bsum, (size_t)ctx->ngrp * STZ_GRP * 8);
-    rc = rc ? rc : dalloc(ctx, (void **)&ctx->bneg, ctx->nblk * sizeof(int));
+    rc = rc ? rc : dalloc(ctx, (void **)&ctx->astatus, ctx->nblk * sizeof(unsigned long long));
     rc = rc ? rc : dalloc(ctx, (void **)&ctx->gsum, ctx->ngrp * 8);
     rc = rc ? rc : dalloc(ctx, (void **)&ctx->res, 64 * 8);
     rc = rc ? rc : dalloc(ctx, (void **)&ctx->spart, rows * ctx->C * 8);
@@ -430,10 +470,20 @@ int stz_create(const stz_params *p, stz_ctx **out)
     if (rc) return fail(rc);
     if (hipHostMalloc((void **)&ctx->hres, 64 * sizeof(double), hipHostMallocDefault) != hipSuccess) return fail(STZ_E_HIP);
     std::memset(ctx->hres, 0, 64 * sizeof(double));
+    // host buffers that kernels read directly (fine-grained, coherent)
+    const unsigned hflags = hipHostMallocCoherent | hipHostMallocMapped;
+    const size_t ms = (size_t)ctx->M * ctx->S;
+    if (hipHostMalloc((void **)&ctx->hspec, (1 + ms) * sizeof(double), hflags) != hipSuccess) return fail(STZ_E_HIP);
+    if (hipHostMalloc((void **)&ctx->hath, (size_t)ctx->acap_max * ms * sizeof(double), hflags) != hipSuccess)
+        return fail(STZ_E_HIP);
+    if (hipHostGetDevicePointer((void **)&ctx->dspec_dev, ctx->hspec, 0) != hipSuccess) return fail(STZ_E_HIP);
+    if (hipHostGetDevicePointer((void **)&ctx->dath_dev, ctx->hath, 0) != hipSuccess) return fail(STZ_E_HIP);
+    std::memset(ctx->hspec, 0, (1 + ms) * sizeof(double));
     rc = ensure_dnew(ctx, (size_t)ctx->M * ctx->S);
     if (rc) return fail(rc);
     rc = stz_fft_setup(ctx);
     if (rc) return fail(rc);
+    if (invalidate_cache(ctx)) return fail(STZ_E_HIP);
     // tlast = -100*tTemp (STZ/init.c:312)
     hipLaunchKernelGGL((k_fill<double>), dim3((unsigned)((N + 255) / 256)), dim3(256), 0, ctx->stream, ctx->tlast,
                        (long long)N, -100 * ctx->k.tTemp);
@@ -446,8 +496,8 @@ void stz_destroy(stz_ctx *ctx)
 {
     if (!ctx) return;
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-    void *ptrs[] = {ctx->eps0, ctx->sigma, ctx->q0, ctx->deps, ctx->deps_last, ctx->perm, ctx->temp, ctx->tlast,
-                    ctx->spec, ctx->minq, ctx->argm, ctx->bsum, ctx->bneg, ctx->gsum, ctx->res, ctx->spart,
+    void *ptrs[] = {ctx->q0eff, ctx->scache, ctx->eps0, ctx->sigma, ctx->q0, ctx->deps, ctx->deps_last, ctx->perm, ctx->temp, ctx->tlast,
+                    ctx->spec, ctx->minq, ctx->argm, ctx->bsum, ctx->astatus, ctx->gsum, ctx->res, ctx->spart,
                     ctx->acnt, ctx->aoff, ctx->alist, ctx->dnew, ctx->socp, ctx->gneg};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
@@ -455,6 +505,8 @@ void stz_destroy(stz_ctx *ctx)
     for (int i = 0; i < 24; ++i)
         if (ctx->tev[i]) (void)hipEventDestroy(ctx->tev[i]);
     if (ctx->hres) (void)hipHostFree(ctx->hres);
+    if (ctx->hspec) (void)hipHostFree(ctx->hspec);
+    if (ctx->hath) (void)hipHostFree(ctx->hath);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -516,6 +568,10 @@ int stz_upload(stz_ctx *ctx, int field, const double *host)
     default: return STZ_E_ARG;
     }
     if (rc) return rc;
+    if (field != STZ_F_EPS0 && field != STZ_F_SIGMA) {
+        rc = invalidate_cache(ctx);
+        if (rc) return rc;
+    }
     STZ_HIP(hipStreamSynchronize(ctx->stream));
     return STZ_OK;
 }
@@ -672,64 +728,120 @@ int stz_step_kmc(stz_ctx *ctx, stz_step_result *res)
     std::memset(res, 0, sizeof(*res));
     res->site = res->mode = -1;
     ctx->step++;
-    long long nneg;
-    double R;
-    int rc = rates(ctx, &nneg, &R);               // SofteningFactor + ActEnergy + count (bmg.c:141-157)
+    const size_t ms = (size_t)ctx->M * ctx->S;
+    // The random numbers a thermal event would consume -- the dice (STZ/evolution.c:142)
+    // and the M*S redraw values (STZ/elastic.c:508-512) -- and those of an athermal
+    // avalanche of up to acap voxels (row-major, M*S each) are drawn before the step
+    // kind is known; afterwards the stream is set to exactly what the reference consumed.
+    const StzRng s0 = ctx->rng;
+    ctx->hspec[0] = ctx->rng.uniform();
+    draw_voxel_deps(ctx, ctx->hspec + 1);
+    const StzRng s_th = ctx->rng;
+    int rc = stz_rates_launch(ctx, 1);            // rates + group sums + step kind on the device
     if (rc) return rc;
+    ctx->pend_comp = -1;
+    ctx->pend_delta = 0.0;
+    ctx->rng = s0;                                // host drawing overlaps the rate pass
+    for (long long i = 0; i < ctx->acap; ++i) draw_voxel_deps(ctx, ctx->hath + (size_t)i * ms);
+    ctx->rng = s0;
+    rc = stz_select_launch(ctx, 0.0, 1);          // thermal: selection + transform
+    if (rc) return rc;
+    rc = stz_athermal_fused_launch(ctx, 1);       // athermal: compaction + transforms
+    if (rc) return rc;
+    rc = stz_solve_launch(ctx, 1);                // thermal or athermal: stress solve
+    if (rc) return rc;
+    STZ_HIP(hipMemcpyAsync(ctx->hres, ctx->res, 32 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    rc = spin_sync(ctx);
+    if (rc) return rc;
+    const double R = ctx->hres[8];
+    const long long nneg = (long long)ctx->hres[9];
+    int kind;
+    std::memcpy(&kind, ctx->hres + RES_KIND, sizeof(int));
+    ctx->rate_total = R;
     res->n_negative = nneg;
     res->total_rate = R;
-    const size_t ms = (size_t)ctx->M * ctx->S;
-    if (nneg > 0) {
-        // athermal_plasticity (STZ/evolution.c:47-100)
-        ctx->astep++;
-        int n;
-        rc = athermal_scan(ctx, &n);
-        if (rc) return rc;
-        ctx->hbuf.resize((size_t)n * ms + 1);
-        for (int i = 0; i < n; ++i) draw_voxel_deps(ctx, ctx->hbuf.data() + (size_t)i * ms);
-        double soc = 0.0;
-        rc = athermal_apply(ctx, ctx->hbuf.data(), &soc);
-        if (rc) return rc;
-        ctx->avlan += n;
-        ctx->soc_elast += soc;
-        rc = solve(ctx);
-        if (rc) return rc;
-        athermal_correction(ctx);
-        res->kind = STZ_STEP_ATHERMAL;
-        res->n_transformed = n;
-        res->soc_elast = soc;
-    } else {
+    acc_rates(ctx);
+    if (kind == KIND_FAIL) {
+        ctx->err = "KMC failed to locate an activation mode";
+        return STZ_E_SELECT;
+    }
+    if (kind == KIND_THERMAL || kind == KIND_ATHERMAL) {
+        int nan;
+        std::memcpy(&nan, ctx->hres + 15, sizeof(int));
+        if (nan) {
+            ctx->err = "NaN in the stress solve";
+            return STZ_E_NAN;
+        }
+        acc_solve(ctx);
+        ctx->nsolve++;
+        for (int c = 0; c < ctx->C; ++c) ctx->stress[c] = ctx->hres[c];
+        for (int c = 0; c < 6; ++c) ctx->soff[c] = 0.0;   // the solve rewrote the stress field
+    }
+    if (kind == KIND_THERMAL) {
+        acc_select(ctx);
+        ctx->rng = s_th;
         ctx->avlan = 0;                           // UpdateSOC (STZ/evolution.c:5-45)
         ctx->soc_elast = 0.0;
         const double t = 1 / R;                   // STZ/evolution.c:134
         res->dt_kmc = t;
-        if (t < ctx->k.dt) {
-            ctx->kstep++;
-            const double dice = ctx->rng.uniform();   // STZ/evolution.c:142
-            int site, mode;
-            double q;
-            rc = select_event(ctx, dice, &site, &mode, &q);
+        ctx->kstep++;
+        ctx->ntrans++;
+        res->kind = STZ_STEP_THERMAL;
+        res->site = (int)ctx->hres[10];
+        res->mode = (int)ctx->hres[11];
+        res->q_chosen = ctx->hres[12];
+        rc = load_thermal(ctx, t);
+        if (rc) return rc;
+        ctx->totime += t;                         // STZ/evolution.c:317
+    } else if (kind == KIND_ATHERMAL || kind == KIND_ATH_OVERFLOW) {
+        // athermal_plasticity (STZ/evolution.c:47-100)
+        ctx->astep++;
+        const long long n = (long long)ctx->hres[13];
+        double soc = ctx->hres[14];
+        ctx->rng = s0;
+        if (kind == KIND_ATHERMAL) {
+            for (long long i = 0; i < n; ++i) draw_voxel_deps(ctx, ctx->hath);   // advance the stream
+        } else {
+            // more voxels than pre-drawn: transform the rest with the continuing stream
+            ctx->hbuf.resize((size_t)n * ms);
+            for (long long i = 0; i < n; ++i) draw_voxel_deps(ctx, ctx->hbuf.data() + (size_t)i * ms);
+            const long long rest = n - ctx->acap;
+            rc = ensure_dnew(ctx, (size_t)rest * ms);
             if (rc) return rc;
-            ctx->hbuf.resize(ms);
-            draw_voxel_deps(ctx, ctx->hbuf.data());
-            rc = apply_event(ctx, site, mode, ctx->hbuf.data());
+            STZ_HIP(hipMemcpyAsync(ctx->dnew, ctx->hbuf.data() + (size_t)ctx->acap * ms, (size_t)rest * ms * sizeof(double),
+                                   hipMemcpyHostToDevice, ctx->stream));
+            rc = upload_soff(ctx);
             if (rc) return rc;
+            rc = stz_athermal_apply_range(ctx, ctx->acap, rest);
+            if (rc) return rc;
+            std::vector<double> part((size_t)((rest + 255) / 256));
+            STZ_HIP(hipMemcpyAsync(part.data(), ctx->socp, part.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
             rc = solve(ctx);
             if (rc) return rc;
-            rc = load_thermal(ctx, t);
-            if (rc) return rc;
-            ctx->totime += t;                     // STZ/evolution.c:317
-            res->kind = STZ_STEP_THERMAL;
-            res->site = site;
-            res->mode = mode;
-            res->q_chosen = q;
-        } else {
-            ctx->estep++;
-            ctx->totime += ctx->k.dt;             // STZ/evolution.c:324
-            rc = load_elastic(ctx);
-            if (rc) return rc;
-            res->kind = STZ_STEP_ELASTIC;
+            for (double x : part) soc += x;
         }
+        ctx->ntrans += n;
+        ctx->avlan += n;
+        ctx->soc_elast += soc;
+        athermal_correction(ctx);
+        // pre-draw enough redraws for the avalanches seen recently
+        long long want = 2 * n;
+        if (want < 8) want = 8;
+        if (want > ctx->acap_max) want = ctx->acap_max;
+        if (want > ctx->acap) ctx->acap = want;
+        res->kind = STZ_STEP_ATHERMAL;
+        res->n_transformed = (int)n;
+        res->soc_elast = soc;
+    } else {
+        ctx->rng = s0;                            // nothing consumed
+        ctx->avlan = 0;                           // UpdateSOC
+        ctx->soc_elast = 0.0;
+        res->dt_kmc = 1 / R;
+        ctx->estep++;
+        ctx->totime += ctx->k.dt;                 // STZ/evolution.c:324
+        rc = load_elastic(ctx);
+        if (rc) return rc;
+        res->kind = STZ_STEP_ELASTIC;
     }
     int term = 0;
     if (ctx->printperiod > 0) {
@@ -763,6 +875,22 @@ int stz_step(stz_ctx *ctx, stz_step_result *res)
     int rc = stz_step_kmc(ctx, res);
     if (rc < 0 || !res->printed) return rc;
     return stz_step_print(ctx, res);
+}
+
+int stz_run_steps(stz_ctx *ctx, int nsteps, stz_step_result *results, int *done)
+{
+    if (!ctx || nsteps < 0) return STZ_E_ARG;
+    stz_step_result tmp;
+    int rc = 0, i = 0;
+    for (; i < nsteps; ++i) {
+        rc = stz_step(ctx, results ? results + i : &tmp);
+        if (rc != 0) {
+            if (rc > 0) ++i;
+            break;
+        }
+    }
+    if (done) *done = i;
+    return rc;
 }
 
 int stz_rng_seed(stz_ctx *ctx, unsigned long seed)

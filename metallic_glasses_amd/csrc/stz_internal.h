@@ -16,6 +16,23 @@
 #define STZ_BLK 256          // voxels per rate block (one per thread)
 #define STZ_GRP 256          // rate blocks per selection group
 
+// control block (ctx->res, doubles unless noted), shared by kernels and host:
+//   [0..5] <sigma>  [8] R  [9] #(Q<0)  [10..12] site, mode, Q  [13] #athermal
+//   [15] NaN flag (int)  [16..21] mixed stress offsets  [24] Q<0 counter (u64)
+//   [25] group arrival counter (u32)  [26] step kind (int)  [27] dice*R  [28] dice
+//   [29] athermal block ticket (u32)
+#define RES_NNEG 24
+#define RES_ARRIVE 25
+#define RES_KIND 26
+#define RES_TAU 27
+#define RES_DICE 28
+#define RES_TICKET 29
+#define KIND_ELASTIC 0
+#define KIND_THERMAL 1
+#define KIND_ATHERMAL 2
+#define KIND_FAIL 3
+#define KIND_ATH_OVERFLOW 4   // athermal avalanche larger than the pre-drawn redraw buffer
+
 // physical constants of the reference, STZ/bmg.h:68-74
 #define STZ_EV_IN_J 1.60217733e-19
 #define STZ_J_IN_EV (1 / STZ_EV_IN_J)
@@ -62,9 +79,19 @@ struct stz_ctx {
     double *perm = nullptr, *temp = nullptr, *tlast = nullptr;
     void *spec = nullptr;         // [C][n1*n2][hp] cpx<R>
     void *minq = nullptr;         // [N] R
+    void *q0eff = nullptr;        // [M][N] R, cached Q0*exp(-D*S)
+    void *scache = nullptr;       // [N] R, softening of the cache entries (NaN: invalid)
     uint8_t *argm = nullptr;      // [N]
     double *bsum = nullptr;       // [M*nblk] block rate sums
-    int *bneg = nullptr;          // [nblk] Q<0 counts per voxel block
+    double *dspec = nullptr;      // unused placeholder (kept zero)
+    double *hspec = nullptr;      // pinned, device-visible: dice + thermal redraw [1 + M*S]
+    double *dspec_dev = nullptr;  // device address of hspec
+    double *hath = nullptr;       // pinned, device-visible: athermal redraws [acap_max * M*S]
+    double *dath_dev = nullptr;   // device address of hath
+    long long acap = 16;          // voxels of an avalanche transformed without a host round trip
+    long long acap_max = 256;
+    unsigned long long *astatus = nullptr;   // look-back status per voxel block
+    unsigned long long aepoch = 0;
     double *gsum = nullptr;       // [ngrp]
     long long *gneg = nullptr;    // [ngrp]
     double *res = nullptr;        // small result area (device)
@@ -307,11 +334,13 @@ __device__ __forceinline__ void fft_lines(cpx<R> *buf, const cpx<R> *tw, R s)
 // ---------------------------------------------------------------------------
 int stz_fft_setup(stz_ctx *ctx);
 void stz_fft_free(stz_ctx *ctx);
-int stz_solve_launch(stz_ctx *ctx);          // enqueues the full stress solve on ctx->stream
-int stz_rates_launch(stz_ctx *ctx);          // enqueues rates + group sums + total
-int stz_select_launch(stz_ctx *ctx, double target, double dice);
-int stz_apply_launch(stz_ctx *ctx, int site, int mode);
+int stz_solve_launch(stz_ctx *ctx, int gated); // enqueues the full stress solve on ctx->stream
+int stz_rates_launch(stz_ctx *ctx, int decide); // enqueues rates + group sums (+ step-kind decision)
+int stz_select_launch(stz_ctx *ctx, double target, int gated);
+int stz_apply_launch(stz_ctx *ctx, int site, int mode, int gated);
 int stz_athermal_launch(stz_ctx *ctx);
 int stz_athermal_apply_launch(stz_ctx *ctx, int n);
+int stz_athermal_fused_launch(stz_ctx *ctx, int gated);
+int stz_athermal_apply_range(stz_ctx *ctx, long long first, long long count);
 int stz_energy_launch(stz_ctx *ctx);
 bool stz_solve_supported(int dim, const int *n, int prec);

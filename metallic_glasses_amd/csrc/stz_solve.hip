@@ -30,6 +30,7 @@ namespace {
 constexpr int NT_SOLVE = 256;
 
 struct SolveConsts {
+    const int *gate;    // fused step: run only when the step kind is thermal
     double lambda, mu, alpha, s0, sh, sa, invN;
     double epsavg[6];
     int n[3];
@@ -180,6 +181,13 @@ __device__ __forceinline__ void products(const R *g, R *gg)
 
 #pragma clang fp contract(on)
 
+// fused step: the solve runs after a thermal event or an athermal avalanche
+__device__ __forceinline__ bool gated_off_p(const int *gate)
+{
+    return gate && *gate != KIND_THERMAL && *gate != KIND_ATHERMAL;
+}
+__device__ __forceinline__ bool gated_off(const SolveConsts &k) { return gated_off_p(k.gate); }
+
 // ---------------------------------------------------------------------------------
 // k_fwd_x: eps0 -> round trip -> R2C along x for NR rows and all C components
 // ---------------------------------------------------------------------------------
@@ -188,6 +196,7 @@ __global__ __launch_bounds__(NT_SOLVE) void k_fwd_x(const double *__restrict__ e
                                                     const cpx<R> *__restrict__ twg, SolveConsts k)
 {
     constexpr int NP = NR / 2;            // packed complex lines per component
+    if (gated_off(k)) return;
     constexpr int NL = C * NP;
     constexpr int LS = NL + 1;            // odd stride against bank conflicts
     __shared__ cpx<R> buf[L0 * LS];
@@ -230,6 +239,7 @@ template <typename R, int L1, int W>
 __global__ __launch_bounds__(NT_SOLVE) void k_ypass(cpx<R> *__restrict__ spec, const cpx<R> *__restrict__ twg,
                                                     SolveConsts k, R sgn)
 {
+    if (gated_off(k)) return;
     __shared__ cpx<R> buf[L1 * W];
     __shared__ cpx<R> tw[L1];
     const int tid = threadIdx.x;
@@ -261,6 +271,7 @@ __global__ __launch_bounds__(NT_SOLVE) void k_fused(cpx<R> *__restrict__ spec, c
                                                     SolveConsts k)
 {
     constexpr int NE = LZ * W;
+    if (gated_off(k)) return;
     constexpr int EPT = (NE + NT_SOLVE - 1) / NT_SOLVE;
     __shared__ cpx<R> buf[NE];
     __shared__ cpx<R> tw[LZ];
@@ -362,6 +373,7 @@ __global__ __launch_bounds__(NT_SOLVE) void k_inv_x(const cpx<R> *__restrict__ s
     constexpr int NL = C * NP;
     constexpr int LS = NL + 1;
     constexpr int H = L0 / 2;
+    if (gated_off(k)) return;
     __shared__ cpx<R> buf[L0 * LS];
     __shared__ cpx<R> tw[L0];
     __shared__ double red[NT_SOLVE];
@@ -428,24 +440,30 @@ __global__ __launch_bounds__(NT_SOLVE) void k_inv_x(const cpx<R> *__restrict__ s
     }
 }
 
+constexpr int NT_MEAN = 1024;
 template <int C>
-__global__ __launch_bounds__(NT_SOLVE) void k_mean(const double *__restrict__ spart, int nblocks, double invN,
-                                                   double *__restrict__ out)
+__global__ __launch_bounds__(NT_MEAN) void k_mean(const double *__restrict__ spart, int nblocks, double invN,
+                                                  double *__restrict__ out, const int *gate)
 {
-    __shared__ double red[NT_SOLVE];
+    if (gated_off_p(gate)) return;
+    __shared__ double red[C][NT_MEAN];
     const int tid = threadIdx.x;
-    for (int c = 0; c < C; ++c) {
-        double a = 0.0;
-        for (int b = tid; b < nblocks; b += NT_SOLVE) a += spart[(long long)b * C + c];
-        red[tid] = a;
-        __syncthreads();
-        for (int w = NT_SOLVE / 2; w > 0; w >>= 1) {
-            if (tid < w) red[tid] = red[tid] + red[tid + w];
-            __syncthreads();
-        }
-        if (tid == 0) out[c] = red[0] * invN;
+    double a[C];
+#pragma unroll
+    for (int c = 0; c < C; ++c) a[c] = 0.0;
+    for (int b = tid; b < nblocks; b += NT_MEAN)
+#pragma unroll
+        for (int c = 0; c < C; ++c) a[c] += spart[(long long)b * C + c];
+#pragma unroll
+    for (int c = 0; c < C; ++c) red[c][tid] = a[c];
+    __syncthreads();
+    for (int w = NT_MEAN / 2; w > 0; w >>= 1) {
+        if (tid < w)
+#pragma unroll
+            for (int c = 0; c < C; ++c) red[c][tid] = red[c][tid] + red[c][tid + w];
         __syncthreads();
     }
+    if (tid < C) out[tid] = red[tid][0] * invN;
 }
 
 // ---------------------------------------------------------------------------------
@@ -570,11 +588,12 @@ struct Launch {
     }
 
 template <typename R>
-int solve_impl(stz_ctx *ctx)
+int solve_impl(stz_ctx *ctx, int gated)
 {
     Launch<R> Lr;
     Lr.ctx = ctx;
     SolveConsts &k = Lr.k;
+    k.gate = gated ? reinterpret_cast<const int *>(ctx->res + RES_KIND) : nullptr;
     k.lambda = ctx->k.lambda;
     k.mu = ctx->k.mu;
     k.alpha = ctx->k.alpha;
@@ -623,8 +642,8 @@ int solve_impl(stz_ctx *ctx)
     }
     if (rc) return rc;
     tmark(ctx, 5);
-    if (C == 3) hipLaunchKernelGGL((k_mean<3>), dim3(1), dim3(NT_SOLVE), 0, ctx->stream, ctx->spart, nblocks, k.invN, ctx->res);
-    else hipLaunchKernelGGL((k_mean<6>), dim3(1), dim3(NT_SOLVE), 0, ctx->stream, ctx->spart, nblocks, k.invN, ctx->res);
+    if (C == 3) hipLaunchKernelGGL((k_mean<3>), dim3(1), dim3(NT_MEAN), 0, ctx->stream, ctx->spart, nblocks, k.invN, ctx->res, k.gate);
+    else hipLaunchKernelGGL((k_mean<6>), dim3(1), dim3(NT_MEAN), 0, ctx->stream, ctx->spart, nblocks, k.invN, ctx->res, k.gate);
     STZ_CHECK_LAUNCH();
     tmark(ctx, 6);
     return STZ_OK;
@@ -673,9 +692,9 @@ void stz_fft_free(stz_ctx *ctx)
     }
 }
 
-int stz_solve_launch(stz_ctx *ctx)
+int stz_solve_launch(stz_ctx *ctx, int gated)
 {
-    return ctx->prec == STZ_FP64 ? solve_impl<double>(ctx) : solve_impl<float>(ctx);
+    return ctx->prec == STZ_FP64 ? solve_impl<double>(ctx, gated) : solve_impl<float>(ctx, gated);
 }
 
 // rows of the stress-mean partial array: the largest block count of k_inv_x

@@ -106,6 +106,7 @@ def lib():
         L.stz_step.argtypes = [vp, P(StepResult)]
         L.stz_step_kmc.argtypes = [vp, P(StepResult)]
         L.stz_step_print.argtypes = [vp, P(StepResult)]
+        L.stz_run_steps.argtypes = [vp, C.c_int, P(StepResult), P(C.c_int)]
         L.stz_rng_seed.argtypes = [vp, C.c_ulong]
         L.stz_rng_uniform.argtypes = [vp, dp]
         L.stz_rng_gaussian.argtypes = [vp, C.c_double, dp]
@@ -252,6 +253,13 @@ class Stz:
         r = StepResult()
         rc = self._check(lib().stz_step(self._h, C.byref(r)))
         return rc, r
+
+    def run_steps(self, n):
+        """Runs n steps in the library; returns (rc, results[:done])."""
+        arr = (StepResult * max(n, 1))()
+        done = C.c_int()
+        rc = self._check(lib().stz_run_steps(self._h, int(n), arr, C.byref(done)))
+        return rc, arr[:done.value]
 
     def step_kmc(self):
         r = StepResult()

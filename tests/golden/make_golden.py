@@ -63,12 +63,14 @@ def make_trace(name, events, threads=8, printperiod=5, endstrain=1.0):
         if rc:
             break
     a = np.array(rows, dtype=object)
-    np.savez_compressed(os.path.join(HERE, f"trace_{name}.npz"),
+    tmp = os.path.join(HERE, f".tmp_trace_{name}.npz")
+    np.savez_compressed(tmp,
                         kind=a[:, 0].astype(np.int8), site=a[:, 1].astype(np.int32),
                         mode=a[:, 2].astype(np.int8), ntrans=a[:, 3].astype(np.int32),
                         totime=a[:, 4].astype(np.float64), stress=a[:, 5].astype(np.float64),
                         printed=a[:, 6].astype(np.int8), printperiod=printperiod, endstrain=endstrain,
                         params=np.array(repr(sorted(kw.items()))))
+    os.replace(tmp, os.path.join(HERE, f"trace_{name}.npz"))
     print(f"{name}: done, {len(rows)} steps, {nev} events, {time.time() - t0:.0f}s", flush=True)
 
 
