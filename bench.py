@@ -39,6 +39,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--seed", type=int, default=1234)
+    ap.add_argument("--timing-steps", type=int, default=200)
     return ap.parse_args()
 
 
@@ -65,9 +66,9 @@ def restore(g, snap, mg):
     g.set_print_state(pr[0])
 
 
-def run_steps(g, k):
+def run_steps(g, k, curve=None):
     t0 = time.perf_counter()
-    rc, res = g.run_steps(k)          # the loop runs inside the library; each step synchronises
+    rc, res = g.run_steps(k)          # device-resident steps; the host synchronises once per batch
     dt = time.perf_counter() - t0
     kinds = np.zeros(3, dtype=np.int64)
     events = []
@@ -75,6 +76,8 @@ def run_steps(g, k):
         kinds[r.kind] += 1
         if r.kind == 1:
             events.append((r.site, r.mode))
+        if curve is not None:
+            curve.append((r.kind, r.site, r.mode, r.n_transformed, r.epsavg[5], r.stress[5]))
     return dt, kinds, events
 
 
@@ -110,19 +113,38 @@ def main():
     if not a.no_cpu or not a.no_mixed:
         snap = snapshot(g, mg)
 
-    g.timing(True)
-    dt, kinds, events = run_steps(g, a.steps)
-    ms, n, sms, sn = g.timing_get()
+    # timed region: device-resident steps replayed from a hipGraph
+    curve64 = []
+    dt, kinds, events = run_steps(g, a.steps, curve64)
+    sig64_end = g.download(mg.F_SIGMA) if not a.no_mixed else None
     N = g.N
     C, M, S = g.C, g.M, g.S
     ev_fp64 = kinds[1] / dt
+    # kernel timing with HIP events on the library's stream, continuing the same run:
+    # the resident step in direct launches (events around every rate pass), then the
+    # per-call step (events around every solve kernel)
+    g.run_config(use_graph=0)
+    g.timing(True)
+    t0 = time.perf_counter()
+    _, tk, _ = run_steps(g, a.timing_steps)
+    dt_direct = time.perf_counter() - t0
+    ms, n, _, _ = g.timing_get()
     k4_ms = ms[0] / max(n[0], 1)
+    g.timing(False)
+    g.timing(True)
+    for _ in range(a.timing_steps // 4):
+        rc_, _r = g.step()
+        if rc_:
+            break
+    ms2, n2, sms, sn = g.timing_get()
+    g.timing(False)
     k4_bytes = rate_bytes_per_voxel(C, M, S, 8) * N
     achieved = k4_bytes / (k4_ms * 1e-3) / 1e9
-    solve_ms = ms[1] / max(n[1], 1)
+    solve_ms = ms2[1] / max(n2[1], 1)
     print(f"# fp64: {a.steps} steps in {dt:.3f}s kinds(el,th,ath)={kinds.tolist()} "
           f"k_rates {k4_ms:.3f} ms ({achieved:.0f} GB/s) solve {solve_ms:.3f} ms "
-          f"per-kernel {np.round(sms[:5] / np.maximum(sn[:5], 1), 4).tolist()}", file=sys.stderr, flush=True)
+          f"per-kernel {np.round(sms[:5] / np.maximum(sn[:5], 1), 4).tolist()}; direct launches "
+          f"{tk[1] / dt_direct:.1f} events/s", file=sys.stderr, flush=True)
 
     out = {
         "metric": "kMC events/s on 1 GPU, mixed vs FP64 at fixed stress err; FFT HBM roofline %",
@@ -140,7 +162,8 @@ def main():
         "config": {"workload": f"STZ kMC {dims[0]}x{dims[1]}x{dims[2]} FP64, M=20, simple shear, seed {a.seed}",
                    "grid": list(dims), "modes": M, "precision": "fp64", "parallelism": "single GPU"},
         "steps_by_kind": {"elastic": int(kinds[0]), "thermal": int(kinds[1]), "athermal": int(kinds[2])},
-        "roofline": {"bound": "hbm", "kernel": "k_rates", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+        "events_per_s_direct_launches": round(float(tk[1] / dt_direct), 3),
+        "roofline": {"bound": "hbm", "kernel": "kb_rates", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "bytes_per_launch": int(k4_bytes), "avg_ms": round(k4_ms, 5)},
         "solve_ms": round(solve_ms, 5),
@@ -173,9 +196,14 @@ def main():
         smix = gm.download(mg.F_SIGMA)
         err = float(np.linalg.norm(smix - sref) / np.linalg.norm(sref))
         restore(gm, snap, mg)
+        curvem = []
+        dtm, kinds_m, events_m = run_steps(gm, a.steps, curvem)
+        smix_end = gm.download(mg.F_SIGMA)
+        gm.run_config(use_graph=0)
         gm.timing(True)
-        dtm, kinds_m, events_m = run_steps(gm, a.steps)
+        run_steps(gm, a.timing_steps)
         msm, nm, _, _ = gm.timing_get()
+        gm.timing(False)
         k4m = msm[0] / max(nm[0], 1)
         kb = rate_bytes_per_voxel(C, M, S, 4) * N
         same = 0
@@ -183,8 +211,21 @@ def main():
             if x != y:
                 break
             same += 1
+        # stress-strain curve (loaded component of <sigma> per step) over the steps whose
+        # step sequence is identical in both runs
+        nsame = 0
+        for x, y in zip(curve64, curvem):
+            if x[:4] != y[:4]:
+                break
+            nsame += 1
+        c64 = np.array([x[5] for x in curve64[:nsame]])
+        cm = np.array([y[5] for y in curvem[:nsame]])
+        ss_err = float(np.max(np.abs(cm - c64)) / max(np.max(np.abs(c64)), 1e-30)) if nsame else None
+        end_err = (float(np.linalg.norm(smix_end - sig64_end) / np.linalg.norm(sig64_end))
+                   if nsame == len(curve64) == len(curvem) else None)
         out["mixed"] = {"value": round(float(kinds_m[1] / dtm), 3), "unit": "events/s",
-                        "stress_rel_err": err, "ms_per_step": round(dtm * 1e3 / a.steps, 4),
+                        "stress_rel_err": err, "stress_rel_err_end": end_err, "ss_curve_rel_err": ss_err,
+                        "steps_identical": nsame, "ms_per_step": round(dtm * 1e3 / a.steps, 4),
                         "speedup_vs_fp64": round(float(kinds_m[1] / dtm) / ev_fp64, 3),
                         "k_rates_GBs": round(kb / (k4m * 1e-3) / 1e9, 1),
                         "events_identical_to_fp64": same}

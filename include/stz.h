@@ -167,10 +167,41 @@ int stz_step(stz_ctx *ctx, stz_step_result *res);
 int stz_step_kmc(stz_ctx *ctx, stz_step_result *res);
 int stz_step_print(stz_ctx *ctx, stz_step_result *res);
 
-/* Runs up to nsteps iterations of stz_step (the reference's while(1) loop body), stopping
-   early on termination (return 1) or error (< 0).  results (optional) receives one entry
-   per completed step; done receives the number of completed steps. */
+/* Runs up to nsteps iterations of the reference's while(1) loop body (STZ/bmg.c:138-411)
+   device-resident: rates, step kind, selection, transforms, stress solve, loading, clock
+   and the print/termination tests run on the GPU without a host round trip, consuming
+   the context's MT19937 stream by index, so the results equal those of nsteps calls of
+   stz_step.  Print steps are re-solved on the host as in stz_step.  Stops early on
+   termination (return 1) or error (< 0).  results (optional) receives one entry per
+   completed step; done receives the number of completed steps. */
 int stz_run_steps(stz_ctx *ctx, int nsteps, stz_step_result *results, int *done);
+
+/* Resident-run settings: steps per host synchronisation (batch, 0 = keep; default 64),
+   hipGraph replay of the step (1) or direct launches (0; -1 = keep; default 1), words of
+   the MT19937 lookup window (0 = keep; default 2^22).  No counterpart in the reference. */
+int stz_run_config(stz_ctx *ctx, int batch, int use_graph, long long window_words);
+/* With on = 1, stz_run_steps returns right after a print step (last res.printed = 1) and
+   before its re-solve, like stz_step_kmc: the caller writes the print block's outputs
+   (STZ/bmg.c:181-287) and then calls stz_step_print. */
+int stz_run_stop_at_print(stz_ctx *ctx, int on);
+
+/* Batched ensemble: replicas (contexts with their own grid, seed, temperature and barrier
+   spread) advanced together, every kernel of a step launched once per shape class over
+   all its replicas, the step replayed from one hipGraph.  The contexts stay owned by the
+   caller and must outlive the ensemble.  stz_ensemble_run advances each replica by up to
+   nsteps steps exactly as stz_run_steps would: results[i] (optional, nsteps entries)
+   receives replica i's records, done[i] their count and rcs[i] its return code.  No
+   counterpart in the reference (one run per process there). */
+typedef struct stz_ensemble stz_ensemble;
+int stz_ensemble_create(stz_ctx *const *members, int n, stz_ensemble **out);
+int stz_ensemble_config(stz_ensemble *e, int batch, int use_graph);
+int stz_ensemble_run(stz_ensemble *e, int nsteps, stz_step_result *const *results, int *done, int *rcs);
+void stz_ensemble_destroy(stz_ensemble *e);
+
+/* Host-only check of the MT19937 lookup window used by the resident step: window values
+   and stream positions against the same draws taken in order (no GPU needed). */
+int stz_mt_window_probe(unsigned long seed, long long skip, long long window_words, long long offset, int with_uniform,
+                        int ngauss, double *win_out, double *seq_out, long long *win_end, long long *seq_end);
 
 /* Host RNG of the context (GSL mt19937 + gsl_ran_gaussian polar method). */
 int stz_rng_seed(stz_ctx *ctx, unsigned long seed);

@@ -459,7 +459,6 @@ int stz_create(const stz_params *p, stz_ctx **out)
     rc = rc ? rc : dalloc(ctx, &ctx->scache, N * R);
     rc = rc ? rc : dalloc(ctx, (void **)&ctx->argm, N);
     rc = rc ? rc : dalloc(ctx, (void **)&ctx->bsum, (size_t)ctx->ngrp * STZ_GRP * 8);
-    rc = rc ? rc : dalloc(ctx, (void **)&ctx->astatus, ctx->nblk * sizeof(unsigned long long));
     rc = rc ? rc : dalloc(ctx, (void **)&ctx->gsum, ctx->ngrp * 8);
     rc = rc ? rc : dalloc(ctx, (void **)&ctx->res, 64 * 8);
     rc = rc ? rc : dalloc(ctx, (void **)&ctx->spart, rows * ctx->C * 8);
@@ -496,8 +495,9 @@ void stz_destroy(stz_ctx *ctx)
 {
     if (!ctx) return;
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    stz_resident_free(ctx);
     void *ptrs[] = {ctx->q0eff, ctx->scache, ctx->eps0, ctx->sigma, ctx->q0, ctx->deps, ctx->deps_last, ctx->perm, ctx->temp, ctx->tlast,
-                    ctx->spec, ctx->minq, ctx->argm, ctx->bsum, ctx->astatus, ctx->gsum, ctx->res, ctx->spart,
+                    ctx->spec, ctx->minq, ctx->argm, ctx->bsum, ctx->gsum, ctx->res, ctx->spart,
                     ctx->acnt, ctx->aoff, ctx->alist, ctx->dnew, ctx->socp, ctx->gneg};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
@@ -516,6 +516,7 @@ const char *stz_last_error(const stz_ctx *ctx) { return ctx ? ctx->err.c_str() :
 int stz_init_state(stz_ctx *ctx)
 {
     if (!ctx) return STZ_E_ARG;
+    stz_rng_sync(ctx);
     const long long N = ctx->N;
     // MRO map: one uniform per voxel, glass unless u <= xMROFrac = 0 (STZ/init.c:355-396)
     for (long long v = 0; v < N; ++v) {
@@ -725,6 +726,7 @@ int stz_set_print(stz_ctx *ctx, int printperiod, double endstrain)
 int stz_step_kmc(stz_ctx *ctx, stz_step_result *res)
 {
     if (!ctx || !res) return STZ_E_ARG;
+    stz_rng_sync(ctx);
     std::memset(res, 0, sizeof(*res));
     res->site = res->mode = -1;
     ctx->step++;
@@ -877,32 +879,18 @@ int stz_step(stz_ctx *ctx, stz_step_result *res)
     return stz_step_print(ctx, res);
 }
 
-int stz_run_steps(stz_ctx *ctx, int nsteps, stz_step_result *results, int *done)
-{
-    if (!ctx || nsteps < 0) return STZ_E_ARG;
-    stz_step_result tmp;
-    int rc = 0, i = 0;
-    for (; i < nsteps; ++i) {
-        rc = stz_step(ctx, results ? results + i : &tmp);
-        if (rc != 0) {
-            if (rc > 0) ++i;
-            break;
-        }
-    }
-    if (done) *done = i;
-    return rc;
-}
-
 int stz_rng_seed(stz_ctx *ctx, unsigned long seed)
 {
     if (!ctx) return STZ_E_ARG;
     ctx->rng.seed(seed);
+    stz_rng_replaced(ctx);
     return STZ_OK;
 }
 
 int stz_rng_uniform(stz_ctx *ctx, double *u)
 {
     if (!ctx || !u) return STZ_E_ARG;
+    stz_rng_sync(ctx);
     *u = ctx->rng.uniform();
     return STZ_OK;
 }
@@ -910,6 +898,7 @@ int stz_rng_uniform(stz_ctx *ctx, double *u)
 int stz_rng_gaussian(stz_ctx *ctx, double sigma, double *g)
 {
     if (!ctx || !g) return STZ_E_ARG;
+    stz_rng_sync(ctx);
     *g = ctx->rng.gaussian(sigma);
     return STZ_OK;
 }
@@ -917,6 +906,7 @@ int stz_rng_gaussian(stz_ctx *ctx, double sigma, double *g)
 int stz_rng_get_state(stz_ctx *ctx, unsigned int words[625])
 {
     if (!ctx || !words) return STZ_E_ARG;
+    stz_rng_sync(ctx);
     for (int i = 0; i < 624; ++i) words[i] = ctx->rng.state[i];
     words[624] = (unsigned int)ctx->rng.pos;
     return STZ_OK;
@@ -927,6 +917,8 @@ int stz_rng_set_state(stz_ctx *ctx, const unsigned int words[625])
     if (!ctx || !words || words[624] > 624) return STZ_E_ARG;
     for (int i = 0; i < 624; ++i) ctx->rng.state[i] = words[i];
     ctx->rng.pos = (int)words[624];
+    ctx->rng.count = 0;
+    stz_rng_replaced(ctx);
     return STZ_OK;
 }
 

@@ -2,7 +2,8 @@
 //
 // Reads the reference's input file (STZ/io.c:4-101, labels matched literally), plus an
 // optional extension block after the last reference field, runs the main loop
-// (STZ/bmg.c:138-411) through stz_step_kmc / stz_step_print, and writes the reference's
+// (STZ/bmg.c:138-411) device-resident through stz_run_steps (stopping at print steps)
+// and stz_step_print, and writes the reference's
 // outputs: actQ.out (per thermal event), SS.out, SOC.out, ElastSOC.out,
 // ElasticEnergy.out, EA.out, EpsAvg.out, avg.out, and the Strain%03d_P00.out /
 // Stress%03d_P00.out field dumps at print steps.
@@ -226,77 +227,89 @@ int main(int argc, char **argv)
     std::vector<double> eps0((size_t)(C * N)), sig((size_t)(C * N)), eq((size_t)N);
     long long step = 0, astep = 0, kstep = 0, estep = 0;
     double prev_time = totime;
+    // the main loop runs device-resident in batches; each batch returns after a print
+    // step, before its re-solve, so the print block sees the reference's state
+    stz_run_stop_at_print(ctx, 1);
+    std::vector<stz_step_result> recs(4096);
     stz_step_result r;
-    for (;;) {
-        rc = stz_step_kmc(ctx, &r);
-        if (rc < 0) {
-            std::fprintf(stderr, "step %lld failed (%d): %s\n", step + 1, rc, stz_last_error(ctx));
+    std::memset(&r, 0, sizeof(r));
+    bool finished = false;
+    while (!finished) {
+        int done = 0;
+        const int brc = stz_run_steps(ctx, (int)recs.size(), recs.data(), &done);
+        if (brc < 0) {
+            std::fprintf(stderr, "step %lld failed (%d): %s\n", step + done + 1, brc, stz_last_error(ctx));
             return 2;
         }
-        ++step;
-        if (r.kind == STZ_STEP_ATHERMAL) {
-            ++astep;
-            soc.counter += r.n_transformed;
-            soc.elast += r.soc_elast;
-        } else {
-            soc.update(prev_time);                      // UpdateSOC before KMC, STZ/bmg.c:167
-            if (r.kind == STZ_STEP_THERMAL) {
-                ++kstep;
-                std::fprintf(fq, "%d\t%.6e\n", 0, r.q_chosen);   // MRO type (glass) and Q, STZ/evolution.c:243
-                std::fflush(fq);
+        for (int i = 0; i < done && !finished; ++i) {
+            r = recs[(size_t)i];
+            rc = (i + 1 == done && !r.printed) ? brc : 0;
+            ++step;
+            if (r.kind == STZ_STEP_ATHERMAL) {
+                ++astep;
+                soc.counter += r.n_transformed;
+                soc.elast += r.soc_elast;
             } else {
-                ++estep;
+                soc.update(prev_time);                      // UpdateSOC before KMC, STZ/bmg.c:167
+                if (r.kind == STZ_STEP_THERMAL) {
+                    ++kstep;
+                    std::fprintf(fq, "%d\t%.6e\n", 0, r.q_chosen);   // MRO type (glass) and Q, STZ/evolution.c:243
+                } else {
+                    ++estep;
+                }
+            }
+            prev_time = r.totime;
+            if (r.printed) {
+                std::fflush(fq);
+                double el = 0.0;
+                stz_elastic_energy(ctx, &el);               // STZ/bmg.c:174-178, before the re-solve
+                strainhist.push_back(r.epsavg[lc]);
+                stresshist.push_back(r.stress[lc]);
+                elhist.push_back(el);
+                const int np = (int)strainhist.size();
+                std::printf("PE#0:step=%lld\ttime=%lf\tstrain=[%.3e, %.3e, %.3e]\nstress=[%.3e, %.3e, %.3e]\n", step,
+                            r.totime, r.epsavg[0], r.epsavg[1], r.epsavg[lc], r.stress[0], r.stress[1], r.stress[lc]);
+                std::printf("step=%lld\nastep=%lld\nkstep=%lld\nestep=%lld\nN_print=%d\n\n", step, astep, kstep, estep, np);
+                stz_download(ctx, STZ_F_EPS0, eps0.data());
+                double avg[6] = {0, 0, 0, 0, 0, 0};
+                for (int c = 0; c < C; ++c) {
+                    for (long long v = 0; v < N; ++v) avg[c] += eps0[(size_t)(c * N + v)];
+                    avg[c] /= (double)N;
+                }
+                if (in.field_output) {
+                    // von Mises-like magnitudes of STZ/io.c:260-273: sqrt(xx^2 + xy^2)
+                    stz_download(ctx, STZ_F_SIGMA, sig.data());
+                    const int xy = C == 3 ? 2 : 5;
+                    for (long long v = 0; v < N; ++v)
+                        eq[(size_t)v] = std::sqrt(eps0[(size_t)v] * eps0[(size_t)v] +
+                                                  eps0[(size_t)(xy * N + v)] * eps0[(size_t)(xy * N + v)]);
+                    write_field("Strain", np - 1, eq, p.n[0], rows);
+                    for (long long v = 0; v < N; ++v)
+                        eq[(size_t)v] = std::sqrt(sig[(size_t)v] * sig[(size_t)v] +
+                                                  sig[(size_t)(xy * N + v)] * sig[(size_t)(xy * N + v)]);
+                    write_field("Stress", np - 1, eq, p.n[0], rows);
+                }
+                rc = stz_step_print(ctx, &r);               // re-solve, STZ/bmg.c:289-295
+                if (rc < 0) {
+                    std::fprintf(stderr, "print re-solve failed (%d): %s\n", rc, stz_last_error(ctx));
+                    return 2;
+                }
+                write_summaries(strainhist, stresshist, elhist, soc, r.epsavg, C);
+                std::printf("step=%lld\nastep=%lld\nkstep=%lld\nestep=%lld\n", step, astep, kstep, estep);
+                FILE *f = std::fopen("EpsAvg.out", "a");
+                for (int c = 0; c < C; ++c) std::fprintf(f, c + 1 < C ? "%lf " : "%lf\n", r.epsavg[c]);
+                std::fclose(f);
+                f = std::fopen("avg.out", "a");
+                for (int c = 0; c < C; ++c) std::fprintf(f, c + 1 < C ? "%lf " : "%lf\n", avg[c]);
+                std::fclose(f);
+            }
+            if (rc == 1) {
+                if (std::fabs(r.epsavg[lc]) > in.endstrain) std::printf("PE#0: macro strain exceeds %lf\n", in.endstrain);
+                else std::printf("\n\nUnstable system! Solution is divergent (occilating)\n");
+                finished = true;
             }
         }
-        prev_time = r.totime;
-        if (r.printed) {
-            double el = 0.0;
-            stz_elastic_energy(ctx, &el);               // STZ/bmg.c:174-178, before the re-solve
-            strainhist.push_back(r.epsavg[lc]);
-            stresshist.push_back(r.stress[lc]);
-            elhist.push_back(el);
-            const int np = (int)strainhist.size();
-            std::printf("PE#0:step=%lld\ttime=%lf\tstrain=[%.3e, %.3e, %.3e]\nstress=[%.3e, %.3e, %.3e]\n", step,
-                        r.totime, r.epsavg[0], r.epsavg[1], r.epsavg[lc], r.stress[0], r.stress[1], r.stress[lc]);
-            std::printf("step=%lld\nastep=%lld\nkstep=%lld\nestep=%lld\nN_print=%d\n\n", step, astep, kstep, estep, np);
-            stz_download(ctx, STZ_F_EPS0, eps0.data());
-            double avg[6] = {0, 0, 0, 0, 0, 0};
-            for (int c = 0; c < C; ++c) {
-                for (long long v = 0; v < N; ++v) avg[c] += eps0[(size_t)(c * N + v)];
-                avg[c] /= (double)N;
-            }
-            if (in.field_output) {
-                // von Mises-like magnitudes of STZ/io.c:260-273: sqrt(xx^2 + xy^2)
-                stz_download(ctx, STZ_F_SIGMA, sig.data());
-                const int xy = C == 3 ? 2 : 5;
-                for (long long v = 0; v < N; ++v)
-                    eq[(size_t)v] = std::sqrt(eps0[(size_t)v] * eps0[(size_t)v] +
-                                              eps0[(size_t)(xy * N + v)] * eps0[(size_t)(xy * N + v)]);
-                write_field("Strain", np - 1, eq, p.n[0], rows);
-                for (long long v = 0; v < N; ++v)
-                    eq[(size_t)v] = std::sqrt(sig[(size_t)v] * sig[(size_t)v] +
-                                              sig[(size_t)(xy * N + v)] * sig[(size_t)(xy * N + v)]);
-                write_field("Stress", np - 1, eq, p.n[0], rows);
-            }
-            rc = stz_step_print(ctx, &r);               // re-solve, STZ/bmg.c:289-295
-            if (rc < 0) {
-                std::fprintf(stderr, "print re-solve failed (%d): %s\n", rc, stz_last_error(ctx));
-                return 2;
-            }
-            write_summaries(strainhist, stresshist, elhist, soc, r.epsavg, C);
-            std::printf("step=%lld\nastep=%lld\nkstep=%lld\nestep=%lld\n", step, astep, kstep, estep);
-            FILE *f = std::fopen("EpsAvg.out", "a");
-            for (int c = 0; c < C; ++c) std::fprintf(f, c + 1 < C ? "%lf " : "%lf\n", r.epsavg[c]);
-            std::fclose(f);
-            f = std::fopen("avg.out", "a");
-            for (int c = 0; c < C; ++c) std::fprintf(f, c + 1 < C ? "%lf " : "%lf\n", avg[c]);
-            std::fclose(f);
-        }
-        if (rc == 1) {
-            if (std::fabs(r.epsavg[lc]) > in.endstrain) std::printf("PE#0: macro strain exceeds %lf\n", in.endstrain);
-            else std::printf("\n\nUnstable system! Solution is divergent (occilating)\n");
-            break;
-        }
+        std::fflush(fq);
     }
     write_summaries(strainhist, stresshist, elhist, soc, r.epsavg, C);
     std::printf("step=%lld\nastep=%lld\nkstep=%lld\nestep=%lld\n", step, astep, kstep, estep);

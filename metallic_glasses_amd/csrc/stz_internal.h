@@ -20,7 +20,7 @@
 //   [0..5] <sigma>  [8] R  [9] #(Q<0)  [10..12] site, mode, Q  [13] #athermal
 //   [15] NaN flag (int)  [16..21] mixed stress offsets  [24] Q<0 counter (u64)
 //   [25] group arrival counter (u32)  [26] step kind (int)  [27] dice*R  [28] dice
-//   [29] athermal block ticket (u32)
+//   [29] athermal arrival counter (u32)
 #define RES_NNEG 24
 #define RES_ARRIVE 25
 #define RES_KIND 26
@@ -32,6 +32,20 @@
 #define KIND_ATHERMAL 2
 #define KIND_FAIL 3
 #define KIND_ATH_OVERFLOW 4   // athermal avalanche larger than the pre-drawn redraw buffer
+#define KIND_STALL 5          // resident step: the random-number window does not hold the step's draws
+#define KIND_NONE (-1)        // no step decided (resident step halted)
+// resident step: the random-number positions of the decided step
+#define RES_K0 30             // first accepted-pair index of the step's gaussians (double)
+#define RES_PAR 31            // their parity (int)
+#define RES_RPOS 32           // stream position after the step (long long)
+
+// halt states of a resident replica
+#define HALT_NONE 0
+#define HALT_DONE 1           // EndStrain or instability reached
+#define HALT_PRINT 2          // print step: the host re-solves and writes output
+#define HALT_RNG 3            // the random-number window must be refilled
+#define HALT_ERR 4            // error code in RepState::err
+#define HALT_BUDGET 5         // the batch's step budget is used up
 
 // physical constants of the reference, STZ/bmg.h:68-74
 #define STZ_EV_IN_J 1.60217733e-19
@@ -54,6 +68,67 @@ struct StzConsts {
 struct FftTwiddles {
     void *tw[3] = {nullptr, nullptr, nullptr};   // per axis, L complex e^{-2 pi i t / L}
 };
+
+// MT19937 lookup window in device memory (see StzMtWindow)
+struct MtDev {
+    const uint32_t *w;
+    const double *g[2];
+    const int *rank[2];
+    const int *pair[2];
+    long long base, len;
+    long long nacc[2];
+};
+
+// first accepted-pair index of the gaussians that start at stream position pos, and its
+// parity p; the stream position after the accepted pair k of parity p
+__host__ __device__ inline long long mt_first(const MtDev &t, long long pos, int &p)
+{
+    const long long r = pos - t.base;
+    p = (int)(r & 1);
+    return t.rank[p][r >> 1];
+}
+__host__ __device__ inline long long mt_after(const MtDev &t, int p, long long k)
+{
+    return t.base + p + 2 * ((long long)t.pair[p][k] + 1);
+}
+
+// run state of one replica for the resident step (device memory): the reference's
+// globals EpsAvg, Stress, totime, the step counters and the stream position
+struct RepState {
+    double epsavg[6], stress[6], soff[6];
+    double totime, pend_delta, strainhist_last, soc_elast, endstrain;
+    long long step, kstep, astep, estep, ntrans, nsolve, avlan;
+    long long rpos;
+    long long budget;        // steps this batch may record
+    int pend_comp, halt, err, nrec, printperiod, pad;
+};
+
+// everything the resident kernels need for one replica (device memory)
+struct RepDev {
+    double *eps0;
+    void *sigma, *q0, *deps, *deps_last, *q0eff, *scache, *minq, *spec;
+    double *perm, *temp, *tlast;
+    uint8_t *argm;
+    double *bsum, *gsum, *socp, *spart, *res;
+    int *acnt, *aoff, *alist;
+    RepState *st;
+    stz_step_result *rec;
+    MtDev mt;
+    // constants (STZ/init.c:6-75)
+    double kT, tTemp, ratepre, lnpre, voxel, thr, dt, permsoft, tempsoft, permcap, half_gamma0;
+    double E, nu, mu, lambda, strainrate, strainincrement;
+    int loading, lc, athermal_correction, softening;
+};
+
+// per-replica pointers of the batched stress solve
+struct SolveRep {
+    const double *eps0;
+    void *spec, *sigma;
+    double *spart, *res;
+    const double *epsavg;
+};
+
+struct StzResident;
 
 struct stz_ctx {
     stz_params p;
@@ -90,8 +165,6 @@ struct stz_ctx {
     double *dath_dev = nullptr;   // device address of hath
     long long acap = 16;          // voxels of an avalanche transformed without a host round trip
     long long acap_max = 256;
-    unsigned long long *astatus = nullptr;   // look-back status per voxel block
-    unsigned long long aepoch = 0;
     double *gsum = nullptr;       // [ngrp]
     long long *gneg = nullptr;    // [ngrp]
     double *res = nullptr;        // small result area (device)
@@ -124,6 +197,8 @@ struct stz_ctx {
     double soc_elast = 0.0;
 
     StzRng rng;
+    long long rng_epoch = 0;       // bumped when the stream is reseeded or replaced
+    StzResident *rs = nullptr;     // resident-step resources (created on first use)
     std::vector<double> hbuf;      // host staging for redraws
     std::string err;
 
@@ -344,3 +419,15 @@ int stz_athermal_fused_launch(stz_ctx *ctx, int gated);
 int stz_athermal_apply_range(stz_ctx *ctx, long long first, long long count);
 int stz_energy_launch(stz_ctx *ctx);
 bool stz_solve_supported(int dim, const int *n, int prec);
+int stz_solve_launch_batched(stz_ctx *proto, int nrep, const SolveRep *reps, hipStream_t stream);
+
+// resident step (stz_resident.hip)
+struct StzPlan;
+void stz_rng_sync(stz_ctx *ctx);        // moves ctx->rng to the resident stream position
+void stz_rng_replaced(stz_ctx *ctx);    // the host stream was reseeded or overwritten
+void stz_resident_free(stz_ctx *ctx);
+int stz_plan_create(stz_ctx *const *ctxs, int n, StzPlan **out);
+void stz_plan_destroy(StzPlan *pl);
+void stz_plan_config(StzPlan *pl, int batch, int use_graph);
+int stz_plan_run(StzPlan *pl, int nsteps, stz_step_result *const *results, int *done, int *rcs, int stop_at_print);
+StzPlan *stz_self_plan(stz_ctx *ctx);

@@ -1,0 +1,585 @@
+// stz_kmc_dev.h -- device code shared by the per-call kernels (stz_rates.hip) and the
+// resident batched step (stz_resident.hip): the per-voxel physics of SofteningFactor and
+// ActEnergy (STZ/init.c:561-739) and of the KMC rates (STZ/evolution.c:120-128), the
+// deterministic wavefront/block scans of the residence-time selection
+// (STZ/evolution.c:136-224), the athermal compaction (STZ/evolution.c:52-77) and the glass
+// branch of VoxelShearTransform (STZ/elastic.c:361-520).  Every caller evaluates the same
+// inline code, so the per-call and the resident paths produce identical bits.
+#pragma once
+
+#include "stz_internal.h"
+
+namespace {
+
+constexpr int NT = 256;
+constexpr int WAVE = 64;
+constexpr int NW = NT / WAVE;
+
+template <typename R>
+struct RateArgs {
+    R *sigma;
+    const R *q0, *deps, *deps_last;
+    R *q0eff;                  // cached Q0*exp(-D*S) [M][N]
+    R *scache;                 // S the cache entry was computed with [N] (NaN: invalid)
+    const double *perm, *temp, *tlast;
+    R *minq;
+    uint8_t *argm;
+    double *bsum;
+    int *acnt;                 // per voxel block: voxels meeting the athermal criterion
+    unsigned long long *nneg;  // Q<0 counter (exact integer atomics)
+    double thr;
+    long long N;
+    long long nblk;
+    int M;
+    int softening;
+    int pend_comp;
+    double pend_delta;
+    double soff[6];
+    double totime, tTemp, kT, ratepre, voxel, lnpre;
+};
+
+// ---------------------------------------------------------------------------------
+// per-voxel physics, evaluated with every operation rounded separately so that any
+// two call sites produce identical bits
+// ---------------------------------------------------------------------------------
+#pragma clang fp contract(off)
+
+template <int DIM>
+struct Comp {
+    static constexpr int C = DIM == 2 ? 3 : 6;
+    static constexpr int S = DIM == 2 ? 2 : 5;
+};
+
+// stored S components -> C tensor components (yy = -xx in 2-D; zz = -(xx + yy) in 3-D)
+template <typename R, int DIM>
+__device__ __forceinline__ void expand(const R *st, R *t)
+{
+    if constexpr (DIM == 2) {
+        t[0] = st[0]; t[1] = -st[0]; t[2] = st[1];
+    } else {
+        t[0] = st[0]; t[1] = st[1]; t[2] = -(st[0] + st[1]); t[3] = st[2]; t[4] = st[3]; t[5] = st[4];
+    }
+}
+
+// a:b with weights (1,1,2) or (1,1,1,2,2,2), left to right (STZ/init.c:585-586)
+template <typename R, int DIM>
+__device__ __forceinline__ R contract(const R *a, const R *b)
+{
+    constexpr int C = Comp<DIM>::C;
+    R acc = a[0] * b[0];
+#pragma unroll
+    for (int c = 1; c < C; ++c) {
+        const bool two = (DIM == 2) ? (c == 2) : (c >= 3);
+        acc = acc + (two ? ((R)2 * a[c]) * b[c] : a[c] * b[c]);
+    }
+    return acc;
+}
+
+template <typename R>
+struct VoxelState {
+    R sg[6];
+    R pd[6];
+    R norm2;
+    R soft;
+    bool fresh;     // the cached Q0*exp(-D*S) of this voxel is valid for this step's S
+};
+
+// Softening S (STZ/init.c:731) and stress of one voxel.  Q0*exp(-D*S) (STZ/init.c:684)
+// changes only when S changes or the voxel transforms, so it is cached per (mode,
+// voxel) with the S it was computed for; S equals Perm bit for bit except shortly after
+// a transform, so almost every voxel takes the cached value, which is the bit-identical
+// result of the same expression.
+template <typename R, int DIM>
+__device__ __forceinline__ void load_voxel(const RateArgs<R> &a, long long v, bool apply_pending, VoxelState<R> &s)
+{
+    constexpr int C = Comp<DIM>::C, S = Comp<DIM>::S;
+    double soft = 0.0;
+    if (a.softening) soft = a.perm[v] + a.temp[v] * exp((a.tlast[v] - a.totime) / a.tTemp);   // :731
+    s.soft = (R)soft;
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+        R x = a.sigma[(long long)c * a.N + v];
+        if constexpr (sizeof(R) == 8) {
+            if (apply_pending && c == a.pend_comp) {
+                x = x + (R)a.pend_delta;      // Sigma[c][idx] += delta (STZ/evolution.c:311-314)
+                a.sigma[(long long)c * a.N + v] = x;
+            }
+        } else {
+            x = (R)((double)x + a.soff[c]);
+        }
+        s.sg[c] = x;
+    }
+    s.fresh = (a.scache[v] == s.soft);       // false for the NaN of an invalidated entry
+    if (!s.fresh) {
+        R st[5];
+#pragma unroll
+        for (int k = 0; k < S; ++k) st[k] = a.deps_last[(long long)k * a.N + v];
+        expand<R, DIM>(st, s.pd);
+        s.norm2 = sqrt(contract<R, DIM>(s.pd, s.pd));                    // :617-618
+    }
+}
+
+template <typename R>
+struct ModeData {
+    R q0;           // cached Q0*exp(-D*S) when the voxel is fresh, else Q0
+    R st[5];
+};
+
+template <typename R, int DIM>
+__device__ __forceinline__ void load_mode(const RateArgs<R> &a, long long v, int m, bool fresh, ModeData<R> &d)
+{
+    constexpr int S = Comp<DIM>::S;
+    d.q0 = fresh ? a.q0eff[(long long)m * a.N + v] : a.q0[(long long)m * a.N + v];
+#pragma unroll
+    for (int k = 0; k < S; ++k) d.st[k] = a.deps[((long long)m * S + k) * a.N + v];
+}
+
+// Q and rate of one mode at a voxel; returns Q0*exp(-D*S)
+template <typename R, int DIM>
+__device__ __forceinline__ R mode_q(const RateArgs<R> &a, const ModeData<R> &md, const VoxelState<R> &s, R &q,
+                                    double &rate)
+{
+    constexpr int C = Comp<DIM>::C;
+    R d[C];
+    expand<R, DIM>(md.st, d);
+    const R dwork = contract<R, DIM>(s.sg, d) * (R)a.voxel * (R)1e-18 * (R)STZ_J_IN_EV;   // :585-586
+    R qe;
+    if (s.fresh) {
+        qe = md.q0;
+    } else {
+        const R dot = contract<R, DIM>(s.pd, d);                                           // :613-614
+        const R norm1 = sqrt(contract<R, DIM>(d, d));                                      // :615-616
+        R dm = dot / (norm1 + (R)STZ_EPSILON) / (s.norm2 + (R)STZ_EPSILON);                // :620
+        dm = ((R)1 + dm) * ((R)1 + dm) / (R)4;                                             // :621
+        qe = md.q0 * exp(-dm * s.soft);
+    }
+    q = qe - dwork / (R)2;                                                                 // :684
+    if constexpr (sizeof(R) == 8) rate = a.ratepre * exp(-q / a.kT);                       // evolution.c:125
+    else rate = (double)expf((R)a.lnpre - q / (R)a.kT);
+    return qe;
+}
+
+#pragma clang fp contract(on)
+
+__device__ __forceinline__ int wave_sum_int(int x)
+{
+#pragma unroll
+    for (int d = 1; d < WAVE; d <<= 1) x += __shfl_xor(x, d, WAVE);
+    return x;
+}
+
+__device__ __forceinline__ double wave_sum_xor(double x)
+{
+#pragma unroll
+    for (int d = 1; d < WAVE; d <<= 1) x = x + __shfl_xor(x, d, WAVE);
+    return x;
+}
+
+// inclusive Hillis-Steele scan inside a wavefront; lane 63 equals wave_sum_xor
+__device__ __forceinline__ double wave_scan(double x, int lane)
+{
+#pragma unroll
+    for (int d = 1; d < WAVE; d <<= 1) {
+        const double y = __shfl_up(x, d, WAVE);
+        if (lane >= d) x = y + x;
+    }
+    return x;
+}
+
+// block-wide inclusive scan of NT values: wave scans + sequential wave offsets.
+// Its last element equals block_sum() below bit for bit.
+__device__ __forceinline__ double block_scan(double x, double *sh)
+{
+    const int tid = threadIdx.x, lane = tid % WAVE, w = tid / WAVE;
+    x = wave_scan(x, lane);
+    if (lane == WAVE - 1) sh[w] = x;
+    __syncthreads();
+    double off = 0.0;
+    for (int i = 0; i < w; ++i) off = off + sh[i];
+    __syncthreads();
+    return w == 0 ? x : off + x;
+}
+
+
+
+// ---------------------------------------------------------------------------------
+// rate pass over one block of NT voxels (all modes): block rate sums per mode, the
+// per-voxel minimum Q and its first argmin, the athermal flag count of the block, the
+// Q<0 count, and the Q0*exp(-D*S) cache refresh
+// ---------------------------------------------------------------------------------
+template <typename R, int DIM>
+__device__ __forceinline__ void rates_block(const RateArgs<R> &a, long long blk)
+{
+    __shared__ double wsum[STZ_MAXM][NW];
+    __shared__ int wcnt[2][NW];
+    const int tid = threadIdx.x, lane = tid % WAVE, w = tid / WAVE;
+    const long long v = blk * NT + tid;
+    const bool valid = v < a.N;
+    VoxelState<R> s;
+    ModeData<R> cur, nxt;
+    if (valid) {
+        load_voxel<R, DIM>(a, v, true, s);
+        load_mode<R, DIM>(a, v, 0, s.fresh, cur);
+    }
+    R minq = (R)0;
+    int am = 0, nneg = 0;
+    for (int m = 0; m < a.M; ++m) {
+        double rate = 0.0;
+        if (valid) {
+            // the next mode's loads are in flight while this mode is evaluated
+            if (m + 1 < a.M) load_mode<R, DIM>(a, v, m + 1, s.fresh, nxt);
+            R q;
+            const R qe = mode_q<R, DIM>(a, cur, s, q, rate);
+            if (!s.fresh) a.q0eff[(long long)m * a.N + v] = qe;
+            if (m == 0 || minq > q) { minq = q; am = m; }     // first argmin (STZ/evolution.c:56-63)
+            nneg += (q < (R)0);
+            cur = nxt;
+        }
+        const double t = wave_sum_xor(rate);
+        if (lane == 0) wsum[m][w] = t;
+    }
+    const int flag = valid && (minq < (R)0 || fabs((double)minq) < a.thr);   // STZ/evolution.c:65
+    const int wn = wave_sum_int(nneg);
+    const int wa = __popcll(__ballot(flag));
+    if (lane == 0) {
+        wcnt[0][w] = wn;
+        wcnt[1][w] = wa;
+    }
+    if (valid) {
+        a.minq[v] = minq;
+        a.argm[v] = (uint8_t)am;
+        if (!s.fresh) a.scache[v] = s.soft;
+    }
+    __syncthreads();
+    if (tid < a.M) {
+        double t = wsum[tid][0];
+        for (int i = 1; i < NW; ++i) t = t + wsum[tid][i];
+        a.bsum[(long long)tid * a.nblk + blk] = t;
+    }
+    if (tid == 0) {
+        int tn = 0, ta = 0;
+        for (int i = 0; i < NW; ++i) {
+            tn += wcnt[0][i];
+            ta += wcnt[1][i];
+        }
+        if (tn) atomicAdd(a.nneg, (unsigned long long)tn);
+        a.acnt[blk] = ta;
+    }
+}
+
+// fixed-tree sum of NT consecutive block sums starting at b0 (one per thread)
+__device__ __forceinline__ double group_sum(const double *bsum, long long nb, long long b0, double *ws)
+{
+    const int tid = threadIdx.x, lane = tid % WAVE, w = tid / WAVE;
+    const long long b = b0 + tid;
+    double x = b < nb ? bsum[b] : 0.0;
+    x = wave_sum_xor(x);
+    if (lane == 0) ws[w] = x;
+    __syncthreads();
+    double t = ws[0];
+    for (int i = 1; i < NW; ++i) t = t + ws[i];
+    __syncthreads();
+    return t;
+}
+
+// Running scan over the group sums, chunk by chunk, by one workgroup.  The same code
+// computes R and the level-2 prefixes of the selection, so they agree bit for bit.
+__device__ double group_total(const double *gsum, long long ng, double *sh)
+{
+    __shared__ double last;
+    double carry = 0.0;
+    for (long long c0 = 0; c0 < ng; c0 += NT) {
+        const long long g = c0 + threadIdx.x;
+        const double x = g < ng ? gsum[g] : 0.0;
+        const double incl = block_scan(x, sh);
+        if (threadIdx.x == NT - 1) last = incl;
+        __syncthreads();
+        carry = (c0 == 0) ? last : carry + last;
+        __syncthreads();
+    }
+    return carry;
+}
+
+// Exclusive offsets of the per-block athermal counts (exact integers), by one
+// workgroup: per-thread runs over contiguous chunks, one scan, per-thread writes.
+// Returns the total to every thread.
+__device__ long long block_offsets(const int *cnt, long long nblk, int *off)
+{
+    __shared__ long long sh[NT];
+    const int tid = threadIdx.x;
+    const long long per = (nblk + NT - 1) / NT;
+    const long long b0 = (long long)tid * per;
+    const long long b1 = b0 + per < nblk ? b0 + per : nblk;
+    long long run = 0;
+    for (long long b = b0; b < b1; ++b) run += cnt[b];
+    sh[tid] = run;
+    __syncthreads();
+    for (int d = 1; d < NT; d <<= 1) {
+        const long long y = tid >= d ? sh[tid - d] : 0;
+        __syncthreads();
+        sh[tid] += y;
+        __syncthreads();
+    }
+    long long base = sh[tid] - run;
+    for (long long b = b0; b < b1; ++b) {
+        off[b] = (int)base;
+        base += cnt[b];
+    }
+    const long long total = sh[NT - 1];
+    __syncthreads();
+    return total;
+}
+
+// ---------------------------------------------------------------------------------
+// voxel transform (glass branch of VoxelShearTransform).  The M*S new transformation
+// strains are dscale * dnew[i] (dscale = 1 for values drawn already scaled).
+// ---------------------------------------------------------------------------------
+struct ApplyArgs {
+    double *eps0;
+    void *deps, *deps_last;
+    void *scache;
+    double *perm, *temp, *tlast;
+    long long N;
+    int M;
+    double permsoft, tempsoft, permcap, totime;
+};
+
+#pragma clang fp contract(off)
+template <typename R, int DIM>
+__device__ void transform_voxel(const ApplyArgs &a, long long v, int mm, const double *dnew, double dscale)
+{
+    constexpr int C = Comp<DIM>::C, S = Comp<DIM>::S;
+    R *deps = (R *)a.deps;
+    R *last = (R *)a.deps_last;
+    R st[5], d[C];
+#pragma unroll
+    for (int k = 0; k < S; ++k) st[k] = deps[((long long)mm * S + k) * a.N + v];
+#pragma unroll
+    for (int k = 0; k < S; ++k) last[(long long)k * a.N + v] = st[k];                  // PreDEps0 snapshot :369-374
+    expand<R, DIM>(st, d);
+    double dd[C];
+#pragma unroll
+    for (int c = 0; c < C; ++c) dd[c] = (double)d[c];
+#pragma unroll
+    for (int c = 0; c < C; ++c) a.eps0[(long long)c * a.N + v] += dd[c];             // :377-379
+    const double ds = contract<double, DIM>(dd, dd);                                   // :385-386
+    double pv = a.perm[v] + ds * a.permsoft;                                          // :390
+    if (pv > a.permcap) pv = a.permcap;                                               // :391
+    a.perm[v] = pv;
+    a.temp[v] = ds * a.tempsoft;                                                      // :396
+    a.tlast[v] = a.totime;                                                            // :397
+    ((R *)a.scache)[v] = (R)__builtin_nan("");                                        // Q0*exp(-D*S) must be recomputed
+    for (int m = 0; m < a.M; ++m)
+#pragma unroll
+        for (int k = 0; k < S; ++k)
+            deps[((long long)m * S + k) * a.N + v] = (R)(dscale * dnew[m * S + k]);   // :508-512
+}
+#pragma clang fp contract(on)
+
+// ---------------------------------------------------------------------------------
+// residence-time selection by one workgroup
+// ---------------------------------------------------------------------------------
+__device__ __forceinline__ int first_true(bool f, int *sh_min)
+{
+    if (threadIdx.x == 0) *sh_min = NT;
+    __syncthreads();
+    if (f) atomicMin(sh_min, (int)threadIdx.x);
+    __syncthreads();
+    const int r = *sh_min;
+    __syncthreads();
+    return r;
+}
+
+// Three-level search for the first event index whose cumulative rate reaches tau
+// (STZ/evolution.c:150-224): groups -> blocks -> voxels, each level a wavefront-shuffle
+// scan whose last element reproduces the sums of the rate pass bit for bit, the voxel
+// level re-evaluating the NT rates of one block with the rate pass's own code.  Writes
+// site, mode, Q to res[10..12] and, when dnew is given, transforms the chosen voxel.
+// Returns false (res[10] = -1) when tau lies beyond the total.
+template <typename R, int DIM>
+__device__ bool select_block(const RateArgs<R> &a, const double *gsum, long long ng, double tau, double *res,
+                             const ApplyArgs &ap, const double *dnew, double dscale)
+{
+    __shared__ double sh[NW];
+    __shared__ int smin;
+    __shared__ double vals[NT];
+    const int tid = threadIdx.x;
+    const long long nb = (long long)a.M * a.nblk;
+    // level 2: groups
+    double carry = 0.0;
+    long long gstar = -1;
+    double base2 = 0.0;
+    for (long long c0 = 0; c0 < ng && gstar < 0; c0 += NT) {
+        const long long g = c0 + tid;
+        const double x = g < ng ? gsum[g] : 0.0;
+        const double incl = block_scan(x, sh);
+        const double P = (c0 == 0) ? incl : carry + incl;
+        vals[tid] = P;
+        const int f = first_true(g < ng && P >= tau, &smin);
+        if (f < NT) {
+            gstar = c0 + f;
+            base2 = f == 0 ? (c0 == 0 ? 0.0 : carry) : vals[f - 1];
+        } else {
+            carry = vals[NT - 1];
+        }
+        __syncthreads();
+    }
+    if (gstar < 0) {                    // tau beyond the total: "failed to locate"
+        if (tid == 0) {
+            res[10] = -1;
+            res[11] = -1;
+        }
+        return false;
+    }
+    // level 1: blocks of group gstar
+    const long long b = gstar * STZ_GRP + tid;
+    const double x = b < nb ? a.bsum[b] : 0.0;
+    const double incl = block_scan(x, sh);
+    const double P = base2 + incl;
+    vals[tid] = P;
+    __syncthreads();
+    int f = first_true(b < nb && P >= tau, &smin);
+    if (f == NT) {
+        const long long lastb = (nb - 1 < gstar * STZ_GRP + NT - 1) ? nb - 1 : gstar * STZ_GRP + NT - 1;
+        f = (int)(lastb - gstar * STZ_GRP);
+    }
+    const double base1 = f == 0 ? base2 : vals[f - 1];
+    __syncthreads();
+    const long long bstar = gstar * STZ_GRP + f;
+    // level 0: voxels of block bstar, re-evaluated
+    const int m = (int)(bstar / a.nblk);
+    const long long blk = bstar % a.nblk;
+    const long long v = blk * NT + tid;
+    double rate = 0.0;
+    R q = (R)0;
+    if (v < a.N) {
+        VoxelState<R> s;
+        ModeData<R> md;
+        load_voxel<R, DIM>(a, v, false, s);
+        load_mode<R, DIM>(a, v, m, s.fresh, md);
+        (void)mode_q<R, DIM>(a, md, s, q, rate);
+    }
+    const double incl0 = block_scan(rate, sh);
+    const double P0 = base1 + incl0;
+    int f0 = first_true(v < a.N && P0 >= tau, &smin);
+    if (f0 == NT) {
+        const long long lastv = (a.N - 1 < blk * NT + NT - 1) ? a.N - 1 : blk * NT + NT - 1;
+        f0 = (int)(lastv - blk * NT);
+    }
+    if (tid == f0) {
+        res[10] = (double)(blk * NT + f0);
+        res[11] = (double)m;
+        res[12] = (double)q;
+        if (dnew) transform_voxel<R, DIM>(ap, blk * NT + f0, m, dnew, dscale);   // VoxelShearTransform right away
+    }
+    return true;
+}
+
+// ---------------------------------------------------------------------------------
+// athermal_plasticity's compaction and transforms (STZ/evolution.c:52-77) for one block
+// of NT voxels.  Voxels whose minimum Q over modes is < 0 or within the threshold take
+// list positions off[blk] + (rank inside the block), i.e. row-major order; the first
+// `cap` of them transform with redraw values dsrc[idx*M*S ...] (the reference draws them
+// voxel by voxel in that order).  The block's sum of |DWork| goes to socp[blk].
+// ---------------------------------------------------------------------------------
+template <typename R>
+struct AthArgs {
+    ApplyArgs ap;
+    const R *minq;
+    const uint8_t *argm;
+    const R *sigma;
+    double soff[6];
+    double voxel, thr;
+    int *list;
+    const int *off;
+    const double *dsrc;
+    double dscale;
+    long long cap;
+    long long N;
+    double *socp;
+};
+
+template <typename R, int DIM>
+__device__ void athermal_block(const AthArgs<R> &a, long long blk)
+{
+    constexpr int C = Comp<DIM>::C, S = Comp<DIM>::S;
+    __shared__ int ws[NW];
+    __shared__ double red[NT];
+    const int tid = threadIdx.x, lane = tid % WAVE, w = tid / WAVE;
+    const long long v = blk * NT + tid;
+    int f = 0;
+    if (v < a.N) {
+        const R q = a.minq[v];
+        f = (q < (R)0 || fabs((double)q) < a.thr);
+    }
+    const unsigned long long bal = __ballot(f);
+    const int before = __popcll(bal & ((1ull << lane) - 1ull));
+    if (lane == 0) ws[w] = __popcll(bal);
+    __syncthreads();
+    int loff = 0;
+    for (int i = 0; i < w; ++i) loff += ws[i];
+    double dw = 0.0;
+    if (f) {
+        const long long idx = (long long)a.off[blk] + loff + before;
+        a.list[idx] = (int)v;
+        if (idx < a.cap) {
+            const int mm = a.argm[v];
+            R sg[C], st[5], d[C];
+#pragma unroll
+            for (int c = 0; c < C; ++c) {
+                R x = a.sigma[(long long)c * a.N + v];
+                if constexpr (sizeof(R) == 4) x = (R)((double)x + a.soff[c]);
+                sg[c] = x;
+            }
+            const R *deps = (const R *)a.ap.deps;
+#pragma unroll
+            for (int k = 0; k < S; ++k) st[k] = deps[((long long)mm * S + k) * a.N + v];
+            expand<R, DIM>(st, d);
+            dw = fabs((double)(contract<R, DIM>(sg, d) * (R)a.voxel * (R)1e-18 * (R)STZ_J_IN_EV));   // :66-67
+            transform_voxel<R, DIM>(a.ap, v, mm, a.dsrc + idx * (long long)a.ap.M * S, a.dscale);
+        }
+    }
+    red[tid] = dw;
+    __syncthreads();
+    for (int s2 = NT / 2; s2 > 0; s2 >>= 1) {
+        if (tid < s2) red[tid] = red[tid] + red[tid + s2];
+        __syncthreads();
+    }
+    if (tid == 0) a.socp[blk] = red[0];
+}
+
+// Arrival of one workgroup of a grid of `total`; true in the last one to arrive (its
+// threads then see every write the others made before arriving).  Resets the counter.
+__device__ __forceinline__ bool last_arrival(unsigned int *counter, unsigned int total, int *flag_sh)
+{
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        const bool last = atomicAdd(counter, 1u) == total - 1;
+        if (last) atomicExch(counter, 0u);
+        *flag_sh = last;
+    }
+    __syncthreads();
+    const bool last = *flag_sh != 0;
+    if (last) __threadfence();
+    return last;
+}
+
+// fixed-order sum of the athermal block partials (by the last workgroup)
+__device__ double ordered_sum(const double *p, long long n, double *red)
+{
+    const int tid = threadIdx.x;
+    double a = 0.0;
+    for (long long i = tid; i < n; i += NT) a = a + p[i];
+    red[tid] = a;
+    __syncthreads();
+    for (int s2 = NT / 2; s2 > 0; s2 >>= 1) {
+        if (tid < s2) red[tid] = red[tid] + red[tid + s2];
+        __syncthreads();
+    }
+    const double r = red[0];
+    __syncthreads();
+    return r;
+}
+
+}  // namespace

@@ -1,0 +1,1255 @@
+// stz_resident.hip -- the device-resident kMC step: one iteration of the reference's main
+// loop (STZ/bmg.c:138-411: SofteningFactor + ActEnergy, the Q<0 test, athermal_plasticity
+// or KMC with its loading and clock update, the stress solves) with no host round trip,
+// for a batch of replicas of one grid shape at a time.
+//
+// Per step and shape class the stream holds ten launches, every one batched over the
+// replicas of the class (grid y/z = replica) and gated on the device:
+//   kb_rates      rate pass (shared code with k_rates)
+//   kb_group      group sums; in the last workgroup of each replica: R, the Q<0 count,
+//                 the step kind, the residence-time selection and the transform
+//                 (thermal), or the avalanche list offsets (athermal)
+//   kb_athermal   athermal compaction + transforms
+//   5 solve kernels + k_mean (stz_solve.hip, batched)
+//   kb_finish     clock, loading (STZ/evolution.c:262-358), athermal correction
+//                 (STZ/evolution.c:91-97), print-step and termination tests
+//                 (STZ/bmg.c:181-201, 382-410), the step record
+// The random numbers are the reference's own MT19937 stream, taken by index from a
+// lookup window built on the host (StzMtWindow), so a resident run draws exactly what
+// the per-call path and the reference draw.  A replica halts (every later kernel of
+// it exits at once) on termination, on a print step (the host re-solves and writes
+// output), when the window runs short, or on error; the host services it between
+// batches.  The step is captured once into a hipGraph and replayed.
+#include "stz_kmc_dev.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <thread>
+#include <tuple>
+#include <vector>
+
+namespace {
+
+// ---------------------------------------------------------------------------------
+// kernels
+// ---------------------------------------------------------------------------------
+template <typename R>
+__device__ __forceinline__ RateArgs<R> rate_args_of(const RepDev &d, const RepState *st, long long N, long long nblk,
+                                                    int M)
+{
+    RateArgs<R> a;
+    a.sigma = (R *)d.sigma;
+    a.q0 = (const R *)d.q0;
+    a.deps = (const R *)d.deps;
+    a.deps_last = (const R *)d.deps_last;
+    a.q0eff = (R *)d.q0eff;
+    a.scache = (R *)d.scache;
+    a.perm = d.perm;
+    a.temp = d.temp;
+    a.tlast = d.tlast;
+    a.minq = (R *)d.minq;
+    a.argm = d.argm;
+    a.bsum = d.bsum;
+    a.acnt = d.acnt;
+    a.nneg = reinterpret_cast<unsigned long long *>(d.res + RES_NNEG);
+    a.thr = d.thr;
+    a.N = N;
+    a.nblk = nblk;
+    a.M = M;
+    a.softening = d.softening;
+    a.pend_comp = st->pend_comp;
+    a.pend_delta = st->pend_delta;
+    for (int c = 0; c < 6; ++c) a.soff[c] = st->soff[c];
+    a.totime = st->totime;
+    a.tTemp = d.tTemp;
+    a.kT = d.kT;
+    a.ratepre = d.ratepre;
+    a.voxel = d.voxel;
+    a.lnpre = d.lnpre;
+    return a;
+}
+
+__device__ __forceinline__ ApplyArgs apply_args_of(const RepDev &d, const RepState *st, long long N, int M)
+{
+    ApplyArgs a;
+    a.eps0 = d.eps0;
+    a.deps = d.deps;
+    a.deps_last = d.deps_last;
+    a.scache = d.scache;
+    a.perm = d.perm;
+    a.temp = d.temp;
+    a.tlast = d.tlast;
+    a.N = N;
+    a.M = M;
+    a.permsoft = d.permsoft;
+    a.tempsoft = d.tempsoft;
+    a.permcap = d.permcap;
+    a.totime = st->totime;
+    return a;
+}
+
+template <typename R, int DIM>
+__global__ __launch_bounds__(NT) void kb_rates(const RepDev *__restrict__ reps, long long N, long long nblk, int M)
+{
+    const RepDev &d = reps[blockIdx.y];
+    const RepState *st = d.st;
+    if (st->halt) return;
+    const RateArgs<R> a = rate_args_of<R>(d, st, N, nblk, M);
+    rates_block<R, DIM>(a, blockIdx.x);
+}
+
+template <typename R, int DIM>
+__global__ __launch_bounds__(NT) void kb_group(const RepDev *__restrict__ reps, long long N, long long nblk, long long ngrp,
+                                               int M)
+{
+    constexpr int S = Comp<DIM>::S;
+    __shared__ double ws[NW];
+    __shared__ int is_last;
+    __shared__ int s_kind;
+    __shared__ unsigned long long s_nn;
+    __shared__ double s_tau;
+    __shared__ long long s_k0;
+    __shared__ int s_par;
+    const RepDev &d = reps[blockIdx.y];
+    RepState *st = d.st;
+    if (st->halt) return;
+    double *res = d.res;
+    const int tid = threadIdx.x;
+    const long long nb = (long long)M * nblk;
+    const double t = group_sum(d.bsum, nb, (long long)blockIdx.x * STZ_GRP, ws);
+    if (tid == 0) d.gsum[blockIdx.x] = t;
+    if (!last_arrival(reinterpret_cast<unsigned int *>(res + RES_ARRIVE), (unsigned int)ngrp, &is_last)) return;
+    const double Rtot = group_total(d.gsum, ngrp, ws);
+    // rate args for the selection: the pending stress increment is in the field now
+    RateArgs<R> a = rate_args_of<R>(d, st, N, nblk, M);
+    a.pend_comp = -1;
+    a.pend_delta = 0.0;
+    const MtDev &mt = d.mt;
+    const long long MS = (long long)M * S;
+    if (tid == 0) {
+        unsigned long long *cnt = reinterpret_cast<unsigned long long *>(res + RES_NNEG);
+        s_nn = atomicAdd(cnt, 0ull);
+        atomicExch(cnt, 0ull);
+        res[8] = Rtot;
+        res[9] = (double)s_nn;
+        st->pend_comp = -1;          // consumed by the rate pass
+        st->pend_delta = 0.0;
+        reinterpret_cast<int *>(res + 15)[0] = 0;
+        res[14] = 0.0;
+        res[13] = 0.0;
+        const long long rpos = st->rpos;
+        int kind;
+        if (s_nn > 0) {
+            kind = KIND_ATHERMAL;                        // STZ/bmg.c:161
+        } else if (1 / Rtot < d.dt) {                    // STZ/evolution.c:134-136
+            kind = KIND_THERMAL;
+            // the dice (STZ/evolution.c:142), then M*S gaussians (STZ/elastic.c:508-512)
+            if (rpos - mt.base >= mt.len) {
+                kind = KIND_STALL;
+            } else {
+                const double dice = mt.w[rpos - mt.base] / 4294967296.0;
+                int p;
+                const long long k0 = mt_first(mt, rpos + 1, p);
+                if (k0 + MS > mt.nacc[p]) {
+                    kind = KIND_STALL;
+                } else if (!(dice > 0.0)) {
+                    kind = KIND_FAIL;
+                } else {
+                    s_tau = dice * Rtot;
+                    s_k0 = k0;
+                    s_par = p;
+                    reinterpret_cast<long long *>(res + RES_RPOS)[0] = mt_after(mt, p, k0 + MS - 1);
+                }
+            }
+        } else {
+            kind = KIND_ELASTIC;
+        }
+        s_kind = kind;
+    }
+    __syncthreads();
+    int kind = s_kind;
+    if (kind == KIND_ATHERMAL) {
+        const long long n = block_offsets(d.acnt, nblk, d.aoff);
+        if (tid == 0) {
+            const long long rpos = st->rpos;
+            int p = 0;
+            long long k0 = mt.nacc[0] + 1;
+            if (rpos - mt.base <= mt.len) k0 = mt_first(mt, rpos, p);
+            if (k0 + n * MS > mt.nacc[p]) {
+                kind = KIND_STALL;
+            } else {
+                res[13] = (double)n;
+                res[RES_K0] = (double)k0;
+                reinterpret_cast<int *>(res + RES_PAR)[0] = p;
+                reinterpret_cast<long long *>(res + RES_RPOS)[0] = n > 0 ? mt_after(mt, p, k0 + n * MS - 1) : rpos;
+            }
+            s_kind = kind;
+        }
+        __syncthreads();
+        kind = s_kind;
+    } else if (kind == KIND_THERMAL) {
+        const ApplyArgs ap = apply_args_of(d, st, N, M);
+        const double *src = mt.g[s_par] + s_k0;
+        const bool ok = select_block<R, DIM>(a, d.gsum, ngrp, s_tau, res, ap, src, d.half_gamma0);
+        if (!ok) kind = KIND_FAIL;
+    }
+    if (tid == 0) reinterpret_cast<int *>(res + RES_KIND)[0] = kind;
+}
+
+template <typename R, int DIM>
+__global__ __launch_bounds__(NT) void kb_athermal(const RepDev *__restrict__ reps, long long N, long long nblk, int M)
+{
+    constexpr int S = Comp<DIM>::S;
+    __shared__ int is_last;
+    __shared__ double red[NT];
+    const RepDev &d = reps[blockIdx.y];
+    const RepState *st = d.st;
+    double *res = d.res;
+    if (st->halt || reinterpret_cast<const int *>(res + RES_KIND)[0] != KIND_ATHERMAL) return;
+    AthArgs<R> a;
+    a.ap = apply_args_of(d, st, N, M);
+    a.minq = (const R *)d.minq;
+    a.argm = d.argm;
+    a.sigma = (const R *)d.sigma;
+    for (int c = 0; c < 6; ++c) a.soff[c] = st->soff[c];
+    a.voxel = d.voxel;
+    a.thr = d.thr;
+    a.list = d.alist;
+    a.off = d.aoff;
+    const int p = reinterpret_cast<const int *>(res + RES_PAR)[0];
+    a.dsrc = d.mt.g[p] + (long long)res[RES_K0];
+    a.dscale = d.half_gamma0;
+    a.cap = 1ll << 62;
+    a.N = N;
+    a.socp = d.socp;
+    athermal_block<R, DIM>(a, blockIdx.x);
+    (void)S;
+    if (!last_arrival(reinterpret_cast<unsigned int *>(res + RES_TICKET), (unsigned int)nblk, &is_last)) return;
+    const double s = ordered_sum(d.socp, nblk, red);
+    if (threadIdx.x == 0) res[14] = s;
+}
+
+// --- host-loop arithmetic of the reference, on the device, with the host's operation order
+#pragma clang fp contract(off)
+
+// isotropicinv (STZ/elastic.c:522-531)
+__device__ void iso_inv(const RepDev &d, int C, const double *st, double *e)
+{
+    const double lam = d.lambda, mu = d.mu;
+    if (C == 3) {
+        double tra = -lam / 2 / mu / (2 * lam + 2 * mu) * (st[0] + st[1]);
+        e[0] = tra + st[0] / 2 / mu;
+        e[1] = tra + st[1] / 2 / mu;
+        e[2] = st[2] / 2 / mu;
+    } else {
+        double tra = -lam / 2 / mu / (3 * lam + 2 * mu) * ((st[0] + st[1]) + st[2]);
+        for (int i = 0; i < 3; ++i) e[i] = tra + st[i] / 2 / mu;
+        for (int i = 3; i < 6; ++i) e[i] = st[i] / 2 / mu;
+    }
+}
+
+// Sigma[comp][*] += delta: FP64 defers it to the next rate pass, mixed keeps an offset
+__device__ void add_uniform_d(RepState *s, bool fp64, int comp, double delta)
+{
+    if (fp64) {
+        s->pend_comp = comp;
+        s->pend_delta = delta;
+    } else {
+        s->soff[comp] += delta;
+    }
+}
+
+// KMC loading after a thermal event (STZ/evolution.c:281-315)
+__device__ void load_thermal_d(const RepDev &d, RepState *s, int C, bool fp64, double t)
+{
+    const double sr = d.strainrate, mu = d.mu;
+    const int lc = d.lc;
+    double ts[6] = {0, 0, 0, 0, 0, 0}, te[6];
+    if (d.loading == STZ_SHEAR) {
+        for (int c = 0; c < C; ++c) ts[c] = (c == lc) ? 0 : s->stress[c];
+        iso_inv(d, C, ts, te);
+        for (int c = 0; c < C; ++c)
+            if (c != lc) s->epsavg[c] += -te[c];
+        s->epsavg[lc] += sr * t;
+        s->stress[lc] += 2 * mu * sr * t;
+        add_uniform_d(s, fp64, lc, 2 * mu * sr * t);
+        return;
+    }
+    const double nu = d.nu, E = d.E;
+    for (int c = 1; c < C; ++c) ts[c] = s->stress[c];
+    iso_inv(d, C, ts, te);
+    s->epsavg[0] += (sr * t * 1 - te[0]);
+    s->epsavg[1] += (sr * t * (-nu) - te[1]);
+    if (C == 3) {
+        s->epsavg[2] += (-te[2]);
+    } else {
+        s->epsavg[2] += (sr * t * (-nu) - te[2]);
+        for (int c = 3; c < 6; ++c) s->epsavg[c] += (-te[c]);
+    }
+    s->stress[0] += E * sr * t;
+    add_uniform_d(s, fp64, 0, E * sr * t);
+}
+
+// pure elastic increment (STZ/evolution.c:321-358)
+__device__ void load_elastic_d(const RepDev &d, RepState *s, int C, bool fp64)
+{
+    const double sr = d.strainrate, inc = d.strainincrement, mu = d.mu;
+    const int lc = d.lc;
+    if (d.loading == STZ_SHEAR) {
+        s->epsavg[lc] += sr * 1;
+        s->stress[lc] += 2 * mu * inc;
+        add_uniform_d(s, fp64, lc, 2 * mu * inc);
+        return;
+    }
+    const double nu = d.nu, E = d.E;
+    s->epsavg[0] += (inc * 1);
+    s->epsavg[1] += (inc * (-nu));
+    if (C == 3) {
+        s->epsavg[2] += (inc * 0.0);
+    } else {
+        s->epsavg[2] += (inc * (-nu));
+        for (int c = 3; c < 6; ++c) s->epsavg[c] += (inc * 0.0);
+    }
+    s->stress[0] += E * inc;
+    add_uniform_d(s, fp64, 0, E * inc);
+}
+
+// athermal error subtraction, STZ/evolution.c:91-97 (reference) or loading-consistent
+__device__ void athermal_correction_d(const RepDev &d, RepState *s, int C)
+{
+    double ts[6] = {0, 0, 0, 0, 0, 0}, te[6];
+    if (d.athermal_correction == 0) {
+        for (int c = 1; c < C; ++c) ts[c] = s->stress[c];
+    } else {
+        for (int c = 0; c < C; ++c) ts[c] = (c == d.lc) ? 0.0 : s->stress[c];
+    }
+    iso_inv(d, C, ts, te);
+    for (int c = 0; c < C; ++c) s->epsavg[c] -= te[c];
+}
+
+__device__ bool terminated_d(const RepDev &d, const RepState *s)
+{
+    if (fabs(s->epsavg[d.lc]) > s->endstrain) return true;                        // bmg.c:383,397
+    if (d.loading == STZ_TENSION && s->stress[0] < -0.5) return true;               // bmg.c:388
+    if (d.loading == STZ_SHEAR && fabs(s->stress[d.lc]) < STZ_EPSILON) return true; // bmg.c:402
+    return false;
+}
+
+// the host loop's bookkeeping for one replica, after the device decided and ran its step
+template <int DIM>
+__global__ void kb_finish(const RepDev *__restrict__ reps, int fp64)
+{
+    constexpr int C = Comp<DIM>::C;
+    if (threadIdx.x != 0) return;
+    const RepDev &d = reps[blockIdx.x];
+    RepState *s = d.st;
+    if (s->halt) return;
+    double *res = d.res;
+    int *kindp = reinterpret_cast<int *>(res + RES_KIND);
+    const int kind = kindp[0];
+    if (kind == KIND_STALL) {
+        s->halt = HALT_RNG;
+        kindp[0] = KIND_NONE;
+        return;
+    }
+    if (kind == KIND_FAIL) {
+        s->halt = HALT_ERR;
+        s->err = STZ_E_SELECT;
+        kindp[0] = KIND_NONE;
+        return;
+    }
+    if (kind != KIND_THERMAL && kind != KIND_ATHERMAL && kind != KIND_ELASTIC) {
+        s->halt = HALT_ERR;
+        s->err = STZ_E_HIP;
+        return;
+    }
+    if ((kind == KIND_THERMAL || kind == KIND_ATHERMAL) && reinterpret_cast<const int *>(res + 15)[0]) {
+        s->halt = HALT_ERR;
+        s->err = STZ_E_NAN;
+        kindp[0] = KIND_NONE;
+        return;
+    }
+    stz_step_result *r = d.rec + s->nrec;
+    r->kind = STZ_STEP_ELASTIC;
+    r->site = r->mode = -1;
+    r->n_transformed = 0;
+    r->q_chosen = 0.0;
+    r->dt_kmc = 0.0;
+    r->printed = 0;
+    r->soc_elast = 0.0;
+    s->step++;
+    const double R = res[8];
+    r->n_negative = (long long)res[9];
+    r->total_rate = R;
+    if (kind == KIND_THERMAL || kind == KIND_ATHERMAL) {
+        s->nsolve++;
+        for (int c = 0; c < C; ++c) s->stress[c] = res[c];
+        for (int c = 0; c < 6; ++c) s->soff[c] = 0.0;        // the solve rewrote the stress field
+        s->rpos = reinterpret_cast<const long long *>(res + RES_RPOS)[0];
+    }
+    if (kind == KIND_THERMAL) {
+        s->avlan = 0;                         // UpdateSOC (STZ/evolution.c:5-45)
+        s->soc_elast = 0.0;
+        const double t = 1 / R;               // STZ/evolution.c:134
+        r->dt_kmc = t;
+        s->kstep++;
+        s->ntrans++;
+        r->kind = STZ_STEP_THERMAL;
+        r->site = (int)res[10];
+        r->mode = (int)res[11];
+        r->q_chosen = res[12];
+        load_thermal_d(d, s, C, fp64 != 0, t);
+        s->totime += t;                       // STZ/evolution.c:317
+    } else if (kind == KIND_ATHERMAL) {
+        s->astep++;                           // athermal_plasticity (STZ/evolution.c:47-100)
+        const long long n = (long long)res[13];
+        const double soc = res[14];
+        s->ntrans += n;
+        s->avlan += n;
+        s->soc_elast += soc;
+        athermal_correction_d(d, s, C);
+        r->kind = STZ_STEP_ATHERMAL;
+        r->n_transformed = (int)n;
+        r->soc_elast = soc;
+    } else {
+        s->avlan = 0;
+        s->soc_elast = 0.0;
+        r->dt_kmc = 1 / R;
+        s->estep++;
+        s->totime += d.dt;                    // STZ/evolution.c:324
+        load_elastic_d(d, s, C, fp64 != 0);
+    }
+    if (s->printperiod > 0) {
+        const double cur = s->epsavg[d.lc];
+        if (fabs(cur - s->strainhist_last) >= (s->printperiod * d.strainincrement * 0.99)) {
+            s->strainhist_last = cur;         // strainhist[N_print-1], STZ/bmg.c:193-201
+            r->printed = 1;
+            s->halt = HALT_PRINT;
+        } else if (terminated_d(d, s)) {
+            s->halt = HALT_DONE;
+        }
+    }
+    r->totime = s->totime;
+    for (int c = 0; c < 6; ++c) {
+        r->epsavg[c] = s->epsavg[c];
+        r->stress[c] = s->stress[c];
+    }
+    s->nrec++;
+    if (s->halt == HALT_NONE && s->nrec >= s->budget) s->halt = HALT_BUDGET;
+    kindp[0] = KIND_NONE;
+}
+#pragma clang fp contract(on)
+
+}  // namespace
+
+// ==================================================================================
+// host side
+// ==================================================================================
+struct StzResident {
+    RepState *st = nullptr;           // device
+    stz_step_result *rec = nullptr;   // device [rec_cap]
+    int rec_cap = 0;
+    // MT19937 lookup window (device copies of StzMtWindow's arrays)
+    uint32_t *w = nullptr;
+    double *g[2] = {nullptr, nullptr};
+    int *rank[2] = {nullptr, nullptr};
+    int *pair[2] = {nullptr, nullptr};
+    long long wcap = 0;               // words the device arrays hold
+    long long wsize = 1ll << 22;      // words per window
+    StzMtWindow win;
+    bool win_ok = false;
+    long long win_epoch = -1;
+    bool force_rebuild = false;
+    bool rng_lazy = false;            // ctx->rng lags behind the resident stream position
+    long long rng_target = 0;
+    RepDev hdev;                      // host copy of the replica descriptor
+    RepState hst;                     // host copy of the run state
+    std::vector<stz_step_result> hrec;
+    int batch = 64;
+    int use_graph = 1;
+    int stop_at_print = 0;
+    int threads = 8;
+    struct StzPlan *self = nullptr;   // single-context plan
+};
+
+namespace {
+
+struct StzClass {
+    std::vector<stz_ctx *> reps;
+    RepDev *d_reps = nullptr;
+    SolveRep *d_solve = nullptr;
+};
+
+}  // namespace
+
+struct StzPlan {
+    std::vector<StzClass> cls;
+    std::vector<stz_ctx *> all;
+    hipStream_t stream = nullptr;
+    stz_ctx *owner = nullptr;         // context whose error string receives messages
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int batch = 64;
+    int use_graph = 1;
+    // rate-pass timing of direct launches (single context with timing on)
+    std::vector<hipEvent_t> eva, evb;
+    int timed = 0;
+};
+
+namespace {
+
+#define RS_HIP(ctx, call)                                                          \
+    do {                                                                           \
+        hipError_t e_ = (call);                                                    \
+        if (e_ != hipSuccess) {                                                    \
+            (ctx)->err = std::string(#call) + ": " + hipGetErrorString(e_);        \
+            return STZ_E_HIP;                                                      \
+        }                                                                          \
+    } while (0)
+
+int rs_alloc(stz_ctx *ctx, void **p, size_t bytes)
+{
+    RS_HIP(ctx, hipMalloc(p, bytes));
+    RS_HIP(ctx, hipMemset(*p, 0, bytes));
+    return STZ_OK;
+}
+
+void free_window(StzResident *rs)
+{
+    void *ptrs[] = {rs->w, rs->g[0], rs->g[1], rs->rank[0], rs->rank[1], rs->pair[0], rs->pair[1]};
+    for (void *q : ptrs)
+        if (q) (void)hipFree(q);
+    rs->w = nullptr;
+    for (int p = 0; p < 2; ++p) rs->g[p] = nullptr, rs->rank[p] = nullptr, rs->pair[p] = nullptr;
+    rs->wcap = 0;
+}
+
+int ensure_resident(stz_ctx *ctx)
+{
+    if (ctx->rs) return STZ_OK;
+    StzResident *rs = new StzResident();
+    ctx->rs = rs;
+    unsigned hw = std::thread::hardware_concurrency();
+    rs->threads = (int)std::max(1u, std::min(hw, 16u));
+    int rc = rs_alloc(ctx, (void **)&rs->st, sizeof(RepState));
+    if (rc) return rc;
+    rs->rec_cap = rs->batch;
+    rc = rs_alloc(ctx, (void **)&rs->rec, sizeof(stz_step_result) * (size_t)rs->rec_cap);
+    if (rc) return rc;
+    rs->hrec.resize((size_t)rs->rec_cap);
+    return STZ_OK;
+}
+
+// builds the lookup window that starts at the current position of `from` and uploads it
+int window_build(stz_ctx *ctx, const StzRng &from)
+{
+    StzResident *rs = ctx->rs;
+    rs->win.build(from, rs->wsize, rs->threads);
+    if (rs->wcap < rs->wsize) {
+        free_window(rs);
+        const size_t T = (size_t)rs->wsize, H = T / 2 + 1;
+        int rc = STZ_OK;
+        rc = rc ? rc : rs_alloc(ctx, (void **)&rs->w, T * sizeof(uint32_t));
+        for (int p = 0; p < 2; ++p) {
+            rc = rc ? rc : rs_alloc(ctx, (void **)&rs->g[p], H * sizeof(double));
+            rc = rc ? rc : rs_alloc(ctx, (void **)&rs->rank[p], (H + 1) * sizeof(int));
+            rc = rc ? rc : rs_alloc(ctx, (void **)&rs->pair[p], H * sizeof(int));
+        }
+        if (rc) return rc;
+        rs->wcap = rs->wsize;
+    }
+    const StzMtWindow &W = rs->win;
+    RS_HIP(ctx, hipMemcpy(rs->w, W.words.data(), (size_t)W.len * sizeof(uint32_t), hipMemcpyHostToDevice));
+    for (int p = 0; p < 2; ++p) {
+        if (W.nacc[p] > 0) {
+            RS_HIP(ctx, hipMemcpy(rs->g[p], W.g[p].data(), (size_t)W.nacc[p] * sizeof(double), hipMemcpyHostToDevice));
+            RS_HIP(ctx, hipMemcpy(rs->pair[p], W.pair[p].data(), (size_t)W.nacc[p] * sizeof(int), hipMemcpyHostToDevice));
+        }
+        RS_HIP(ctx, hipMemcpy(rs->rank[p], W.rank[p].data(), W.rank[p].size() * sizeof(int), hipMemcpyHostToDevice));
+    }
+    MtDev &m = rs->hdev.mt;
+    m.w = rs->w;
+    for (int p = 0; p < 2; ++p) {
+        m.g[p] = rs->g[p];
+        m.rank[p] = rs->rank[p];
+        m.pair[p] = rs->pair[p];
+        m.nacc[p] = W.nacc[p];
+    }
+    m.base = W.base;
+    m.len = W.len;
+    rs->win_ok = true;
+    rs->win_epoch = ctx->rng_epoch;
+    return STZ_OK;
+}
+
+void fill_desc(stz_ctx *ctx)
+{
+    StzResident *rs = ctx->rs;
+    RepDev &d = rs->hdev;
+    d.eps0 = ctx->eps0;
+    d.sigma = ctx->sigma;
+    d.q0 = ctx->q0;
+    d.deps = ctx->deps;
+    d.deps_last = ctx->deps_last;
+    d.q0eff = ctx->q0eff;
+    d.scache = ctx->scache;
+    d.minq = ctx->minq;
+    d.spec = ctx->spec;
+    d.perm = ctx->perm;
+    d.temp = ctx->temp;
+    d.tlast = ctx->tlast;
+    d.argm = ctx->argm;
+    d.bsum = ctx->bsum;
+    d.gsum = ctx->gsum;
+    d.socp = ctx->socp;
+    d.spart = ctx->spart;
+    d.res = ctx->res;
+    d.acnt = ctx->acnt;
+    d.aoff = ctx->aoff;
+    d.alist = ctx->alist;
+    d.st = rs->st;
+    d.rec = rs->rec;
+    const StzConsts &k = ctx->k;
+    d.kT = k.kT;
+    d.tTemp = k.tTemp;
+    d.ratepre = k.ratepre;
+    d.lnpre = std::log(k.ratepre);
+    d.voxel = k.voxel;
+    d.thr = k.thr;
+    d.dt = k.dt;
+    d.permsoft = k.permsoft;
+    d.tempsoft = k.tempsoft;
+    d.permcap = k.permcap;
+    d.half_gamma0 = k.half_gamma0;
+    d.E = k.E;
+    d.nu = k.nu;
+    d.mu = k.mu;
+    d.lambda = k.lambda;
+    d.strainrate = ctx->p.strainrate;
+    d.strainincrement = ctx->p.strainincrement;
+    d.loading = ctx->p.loading;
+    d.lc = ctx->lc;
+    d.athermal_correction = ctx->p.athermal_correction;
+    d.softening = ctx->p.softening == 1;
+}
+
+// host scalars -> device run state; positions the lookup window at the stream position
+int push_state(stz_ctx *ctx, long long budget)
+{
+    StzResident *rs = ctx->rs;
+    // the stream position the next step consumes from
+    long long pos;
+    if (rs->rng_lazy && rs->win_ok && rs->win_epoch == ctx->rng_epoch) {
+        pos = rs->rng_target;
+    } else {
+        rs->rng_lazy = false;
+        pos = ctx->rng.count;
+    }
+    const StzMtWindow &W = rs->win;
+    const bool inside = rs->win_ok && rs->win_epoch == ctx->rng_epoch && pos >= W.base && pos <= W.base + W.len;
+    if (!inside || rs->force_rebuild || W.base + W.len - pos < rs->wsize / 4) {
+        rs->force_rebuild = false;
+        StzRng from;
+        if (inside) {
+            from = W.at_base;
+            from.skip(pos - W.base);
+        } else {
+            from = ctx->rng;          // not lazy here: ctx->rng is at pos
+        }
+        int rc = window_build(ctx, from);
+        if (rc) return rc;
+    }
+    RepState &s = rs->hst;
+    std::memset(&s, 0, sizeof(s));
+    for (int c = 0; c < 6; ++c) {
+        s.epsavg[c] = ctx->epsavg[c];
+        s.stress[c] = ctx->stress[c];
+        s.soff[c] = ctx->soff[c];
+    }
+    s.totime = ctx->totime;
+    s.pend_comp = ctx->pend_comp;
+    s.pend_delta = ctx->pend_delta;
+    s.strainhist_last = ctx->strainhist_last;
+    s.soc_elast = ctx->soc_elast;
+    s.endstrain = ctx->endstrain;
+    s.printperiod = ctx->printperiod;
+    s.step = ctx->step;
+    s.kstep = ctx->kstep;
+    s.astep = ctx->astep;
+    s.estep = ctx->estep;
+    s.ntrans = ctx->ntrans;
+    s.nsolve = ctx->nsolve;
+    s.avlan = ctx->avlan;
+    s.rpos = pos;
+    s.budget = std::min<long long>(budget, rs->rec_cap);
+    s.halt = budget > 0 ? HALT_NONE : HALT_BUDGET;
+    s.nrec = 0;
+    RS_HIP(ctx, hipMemcpyAsync(rs->st, &s, sizeof(s), hipMemcpyHostToDevice, ctx->stream));
+    const int none = KIND_NONE;
+    RS_HIP(ctx, hipMemcpyAsync(ctx->res + RES_KIND, &none, sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+    fill_desc(ctx);
+    RS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return STZ_OK;
+}
+
+// device run state -> host scalars (rs->hst must hold the downloaded state)
+void pull_state(stz_ctx *ctx)
+{
+    StzResident *rs = ctx->rs;
+    const RepState &s = rs->hst;
+    for (int c = 0; c < 6; ++c) {
+        ctx->epsavg[c] = s.epsavg[c];
+        ctx->stress[c] = s.stress[c];
+        ctx->soff[c] = s.soff[c];
+    }
+    ctx->totime = s.totime;
+    ctx->pend_comp = s.pend_comp;
+    ctx->pend_delta = s.pend_delta;
+    ctx->strainhist_last = s.strainhist_last;
+    ctx->soc_elast = s.soc_elast;
+    ctx->step = s.step;
+    ctx->kstep = s.kstep;
+    ctx->astep = s.astep;
+    ctx->estep = s.estep;
+    ctx->ntrans = s.ntrans;
+    ctx->nsolve = s.nsolve;
+    ctx->avlan = s.avlan;
+    if (!(rs->rng_lazy == false && ctx->rng.count == s.rpos)) {
+        rs->rng_lazy = true;
+        rs->rng_target = s.rpos;
+    }
+}
+
+// --------------------------------------------------------------------------------
+// plans
+// --------------------------------------------------------------------------------
+template <typename R, int DIM>
+int launch_class(StzPlan *pl, StzClass &k, hipEvent_t ea, hipEvent_t eb)
+{
+    stz_ctx *p = k.reps[0];
+    const int nrep = (int)k.reps.size();
+    hipStream_t s = pl->stream;
+    if (ea) (void)hipEventRecord(ea, s);
+    hipLaunchKernelGGL((kb_rates<R, DIM>), dim3((unsigned)p->nblk, nrep), dim3(NT), 0, s, k.d_reps, p->N, p->nblk, p->M);
+    if (eb) (void)hipEventRecord(eb, s);
+    hipLaunchKernelGGL((kb_group<R, DIM>), dim3((unsigned)p->ngrp, nrep), dim3(NT), 0, s, k.d_reps, p->N, p->nblk,
+                       p->ngrp, p->M);
+    hipLaunchKernelGGL((kb_athermal<R, DIM>), dim3((unsigned)p->nblk, nrep), dim3(NT), 0, s, k.d_reps, p->N, p->nblk, p->M);
+    if (hipGetLastError() != hipSuccess) return STZ_E_HIP;
+    const int saved = p->timing;
+    p->timing = 0;
+    int rc = stz_solve_launch_batched(p, nrep, k.d_solve, s);
+    p->timing = saved;
+    if (rc) return rc;
+    hipLaunchKernelGGL((kb_finish<DIM>), dim3(nrep), dim3(64), 0, s, k.d_reps, (int)(p->prec == STZ_FP64));
+    return hipGetLastError() == hipSuccess ? STZ_OK : STZ_E_HIP;
+}
+
+int launch_step(StzPlan *pl, hipEvent_t ea = nullptr, hipEvent_t eb = nullptr)
+{
+    for (StzClass &k : pl->cls) {
+        stz_ctx *p = k.reps[0];
+        int rc;
+        if (p->prec == STZ_FP64)
+            rc = p->dim == 2 ? launch_class<double, 2>(pl, k, ea, eb) : launch_class<double, 3>(pl, k, ea, eb);
+        else
+            rc = p->dim == 2 ? launch_class<float, 2>(pl, k, ea, eb) : launch_class<float, 3>(pl, k, ea, eb);
+        if (rc) return rc;
+    }
+    return STZ_OK;
+}
+
+void plan_free(StzPlan *pl)
+{
+    if (!pl) return;
+    for (hipEvent_t e : pl->eva) (void)hipEventDestroy(e);
+    for (hipEvent_t e : pl->evb) (void)hipEventDestroy(e);
+    if (pl->exec) (void)hipGraphExecDestroy(pl->exec);
+    if (pl->graph) (void)hipGraphDestroy(pl->graph);
+    for (StzClass &k : pl->cls) {
+        if (k.d_reps) (void)hipFree(k.d_reps);
+        if (k.d_solve) (void)hipFree(k.d_solve);
+    }
+    delete pl;
+}
+
+// groups the contexts into classes of one shape, precision and elasticity
+int plan_make(stz_ctx *const *ctxs, int n, StzPlan **out)
+{
+    StzPlan *pl = new StzPlan();
+    pl->owner = ctxs[0];
+    pl->stream = ctxs[0]->stream;
+    std::map<std::tuple<int, int, int, int, int, int, double, double, int>, int> key2cls;
+    for (int i = 0; i < n; ++i) {
+        stz_ctx *c = ctxs[i];
+        int rc = ensure_resident(c);
+        if (rc) {
+            plan_free(pl);
+            return rc;
+        }
+        auto key = std::make_tuple(c->dim, c->n[0], c->n[1], c->n[2], c->M, c->prec, c->k.lambda, c->k.mu, c->p.loading);
+        auto it = key2cls.find(key);
+        if (it == key2cls.end()) {
+            key2cls[key] = (int)pl->cls.size();
+            pl->cls.emplace_back();
+            it = key2cls.find(key);
+        }
+        pl->cls[(size_t)it->second].reps.push_back(c);
+        pl->all.push_back(c);
+    }
+    for (StzClass &k : pl->cls) {
+        const size_t nr = k.reps.size();
+        if (hipMalloc(&k.d_reps, nr * sizeof(RepDev)) != hipSuccess || hipMalloc(&k.d_solve, nr * sizeof(SolveRep)) != hipSuccess) {
+            plan_free(pl);
+            return STZ_E_HIP;
+        }
+    }
+    *out = pl;
+    return STZ_OK;
+}
+
+// uploads the replica descriptors (they change when a lookup window is rebuilt)
+int plan_upload(StzPlan *pl)
+{
+    stz_ctx *o = pl->owner;
+    for (StzClass &k : pl->cls) {
+        std::vector<RepDev> d;
+        std::vector<SolveRep> s;
+        for (stz_ctx *c : k.reps) {
+            d.push_back(c->rs->hdev);
+            SolveRep q;
+            q.eps0 = c->eps0;
+            q.spec = c->spec;
+            q.sigma = c->sigma;
+            q.spart = c->spart;
+            q.res = c->res;
+            q.epsavg = c->rs->st->epsavg;
+            s.push_back(q);
+        }
+        RS_HIP(o, hipMemcpyAsync(k.d_reps, d.data(), d.size() * sizeof(RepDev), hipMemcpyHostToDevice, pl->stream));
+        RS_HIP(o, hipMemcpyAsync(k.d_solve, s.data(), s.size() * sizeof(SolveRep), hipMemcpyHostToDevice, pl->stream));
+    }
+    RS_HIP(o, hipStreamSynchronize(pl->stream));
+    return STZ_OK;
+}
+
+int plan_graph(StzPlan *pl)
+{
+    if (pl->exec) return STZ_OK;
+    stz_ctx *o = pl->owner;
+    RS_HIP(o, hipStreamBeginCapture(pl->stream, hipStreamCaptureModeThreadLocal));
+    int rc = launch_step(pl);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(pl->stream, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) {
+        o->err = std::string("hipStreamEndCapture: ") + hipGetErrorString(e);
+        return STZ_E_HIP;
+    }
+    pl->graph = g;
+    RS_HIP(o, hipGraphInstantiate(&pl->exec, g, nullptr, nullptr, 0));
+    return STZ_OK;
+}
+
+int plan_launch_steps(StzPlan *pl, int k)
+{
+    stz_ctx *o = pl->owner;
+    if (pl->use_graph) {
+        int rc = plan_graph(pl);
+        if (rc) return rc;
+        for (int i = 0; i < k; ++i) RS_HIP(o, hipGraphLaunch(pl->exec, pl->stream));
+        return STZ_OK;
+    }
+    stz_ctx *p = pl->all[0];
+    const bool timed = pl->all.size() == 1 && p->timing;
+    if (timed && pl->eva.size() < (size_t)k) {
+        while (pl->eva.size() < (size_t)k) {
+            hipEvent_t a, b;
+            RS_HIP(o, hipEventCreate(&a));
+            RS_HIP(o, hipEventCreate(&b));
+            pl->eva.push_back(a);
+            pl->evb.push_back(b);
+        }
+    }
+    for (int i = 0; i < k; ++i) {
+        int rc = launch_step(pl, timed ? pl->eva[(size_t)i] : nullptr, timed ? pl->evb[(size_t)i] : nullptr);
+        if (rc) return rc;
+    }
+    pl->timed = timed ? k : 0;
+    return STZ_OK;
+}
+
+}  // namespace
+
+// re-positions ctx->rng at the resident stream position when a resident run moved it
+void stz_rng_sync(stz_ctx *ctx)
+{
+    StzResident *rs = ctx->rs;
+    if (!rs || !rs->rng_lazy) return;
+    StzRng g = rs->win.at_base;
+    g.skip(rs->rng_target - rs->win.base);
+    ctx->rng = g;
+    rs->rng_lazy = false;
+}
+
+void stz_rng_replaced(stz_ctx *ctx)
+{
+    ctx->rng_epoch++;
+    if (ctx->rs) ctx->rs->rng_lazy = false;
+}
+
+void stz_resident_free(stz_ctx *ctx)
+{
+    StzResident *rs = ctx->rs;
+    if (!rs) return;
+    plan_free(rs->self);
+    free_window(rs);
+    if (rs->st) (void)hipFree(rs->st);
+    if (rs->rec) (void)hipFree(rs->rec);
+    delete rs;
+    ctx->rs = nullptr;
+}
+
+// Runs up to nsteps resident steps on every context of the plan.  results[i] (optional)
+// receives up to nsteps records of context i, done[i] their count, rcs[i] its return
+// code (0 running, 1 terminated, < 0 error).  A print step is re-solved on the host
+// (STZ/bmg.c:289-295); with stop_at_print the context stops right after the print step,
+// before that re-solve, which the caller then runs with stz_step_print.
+int stz_plan_run(StzPlan *pl, int nsteps, stz_step_result *const *results, int *done, int *rcs, int stop_at_print)
+{
+    const size_t n = pl->all.size();
+    std::vector<int> active(n, 1);
+    for (size_t i = 0; i < n; ++i) {
+        done[i] = 0;
+        rcs[i] = 0;
+    }
+    for (size_t i = 0; i < n; ++i) {
+        stz_ctx *c = pl->all[i];
+        int rc = push_state(c, nsteps);
+        if (rc) return rc;
+    }
+    int rc = plan_upload(pl);
+    if (rc) return rc;
+    for (;;) {
+        long long most = 0;
+        for (size_t i = 0; i < n; ++i)
+            if (active[i]) most = std::max<long long>(most, nsteps - done[i]);
+        if (most <= 0) break;
+        const int k = (int)std::min<long long>(most, pl->batch);
+        rc = plan_launch_steps(pl, k);
+        if (rc) return rc;
+        stz_ctx *o = pl->owner;
+        for (size_t i = 0; i < n; ++i) {
+            stz_ctx *c = pl->all[i];
+            RS_HIP(o, hipMemcpyAsync(&c->rs->hst, c->rs->st, sizeof(RepState), hipMemcpyDeviceToHost, pl->stream));
+        }
+        RS_HIP(o, hipStreamSynchronize(pl->stream));
+        if (pl->timed > 0) {
+            stz_ctx *p = pl->all[0];
+            for (int i = 0; i < pl->timed; ++i) {
+                float ms = 0.f;
+                if (hipEventElapsedTime(&ms, pl->eva[(size_t)i], pl->evb[(size_t)i]) == hipSuccess) {
+                    p->tms[0] += ms;
+                    p->tn[0] += 1;
+                }
+            }
+            pl->timed = 0;
+        }
+        bool need_upload = false;
+        for (size_t i = 0; i < n; ++i) {
+            if (!active[i]) continue;
+            stz_ctx *c = pl->all[i];
+            StzResident *rs = c->rs;
+            const int nrec = rs->hst.nrec;
+            if (nrec > 0) {
+                RS_HIP(o, hipMemcpy(rs->hrec.data(), rs->rec, (size_t)nrec * sizeof(stz_step_result), hipMemcpyDeviceToHost));
+                if (results && results[i])
+                    std::memcpy(results[i] + done[i], rs->hrec.data(), (size_t)nrec * sizeof(stz_step_result));
+                done[i] += nrec;
+            }
+            const int halt = rs->hst.halt;
+            pull_state(c);
+            long long next_budget = nsteps - done[i];
+            if (halt == HALT_ERR) {
+                c->err = rs->hst.err == STZ_E_SELECT ? "KMC failed to locate an activation mode"
+                         : rs->hst.err == STZ_E_NAN  ? "NaN in the stress solve"
+                                                     : "resident step: unexpected step kind";
+                rcs[i] = rs->hst.err;
+                active[i] = 0;
+                continue;
+            }
+            if (halt == HALT_DONE) {
+                rcs[i] = 1;
+                active[i] = 0;
+                continue;
+            }
+            if (halt == HALT_PRINT && stop_at_print) {
+                // the caller writes the print block and then calls stz_step_print
+                active[i] = 0;
+                continue;
+            }
+            if (halt == HALT_PRINT) {
+                // print-step re-solve on the host path, STZ/bmg.c:289-295
+                stz_step_result tmp;
+                int prc = stz_step_print(c, &tmp);
+                if (prc < 0) {
+                    rcs[i] = prc;
+                    active[i] = 0;
+                    continue;
+                }
+                if (results && results[i] && nrec > 0)
+                    for (int cc = 0; cc < 6; ++cc) results[i][done[i] - 1].stress[cc] = c->stress[cc];
+                if (prc == 1) {
+                    rcs[i] = 1;
+                    active[i] = 0;
+                    continue;
+                }
+            }
+            if (halt == HALT_RNG) {
+                // the window does not hold the coming step's draws
+                const long long pos = rs->hst.rpos;
+                const bool fresh = rs->win.base == pos;
+                if (fresh && rs->win.len == rs->wsize && rs->wsize < (1ll << 28)) {
+                    rs->wsize *= 4;           // a larger avalanche than the window holds
+                } else if (fresh) {
+                    // a zero word ahead (or the largest window is too small): one step on the
+                    // host path, which draws from the stream itself
+                    stz_rng_sync(c);
+                    stz_step_result r;
+                    int hrc = stz_step(c, &r);
+                    if (results && results[i]) results[i][done[i]] = r;
+                    done[i] += 1;
+                    next_budget -= 1;
+                    if (hrc != 0) {
+                        rcs[i] = hrc;
+                        active[i] = 0;
+                        continue;
+                    }
+                }
+                rs->force_rebuild = true;
+            }
+            if (next_budget <= 0) {
+                active[i] = 0;
+                continue;
+            }
+            rc = push_state(c, next_budget);
+            if (rc) return rc;
+            need_upload = true;
+        }
+        if (need_upload) {
+            rc = plan_upload(pl);
+            if (rc) return rc;
+        }
+    }
+    return STZ_OK;
+}
+
+int stz_plan_create(stz_ctx *const *ctxs, int n, StzPlan **out) { return plan_make(ctxs, n, out); }
+void stz_plan_destroy(StzPlan *pl) { plan_free(pl); }
+void stz_plan_config(StzPlan *pl, int batch, int use_graph)
+{
+    if (batch > 0) pl->batch = batch;
+    if (use_graph >= 0 && use_graph != pl->use_graph) {
+        pl->use_graph = use_graph;
+    }
+}
+StzPlan *stz_self_plan(stz_ctx *ctx)
+{
+    if (ensure_resident(ctx)) return nullptr;
+    if (!ctx->rs->self) {
+        StzPlan *pl = nullptr;
+        if (plan_make(&ctx, 1, &pl)) return nullptr;
+        ctx->rs->self = pl;
+    }
+    return ctx->rs->self;
+}
+
+// ==================================================================================
+// C-ABI
+// ==================================================================================
+struct stz_ensemble {
+    StzPlan *pl = nullptr;
+};
+
+namespace {
+
+int set_batch(stz_ctx *ctx, int batch)
+{
+    int rc = ensure_resident(ctx);
+    if (rc) return rc;
+    StzResident *rs = ctx->rs;
+    if (batch > rs->rec_cap) {
+        if (rs->rec) (void)hipFree(rs->rec);
+        rs->rec = nullptr;
+        rs->rec_cap = 0;
+        rc = rs_alloc(ctx, (void **)&rs->rec, sizeof(stz_step_result) * (size_t)batch);
+        if (rc) return rc;
+        rs->rec_cap = batch;
+        rs->hrec.resize((size_t)batch);
+    }
+    rs->batch = batch;
+    return STZ_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int stz_run_config(stz_ctx *ctx, int batch, int use_graph, long long window_words)
+{
+    if (!ctx || batch < 0 || batch > 65536 || window_words < 0) return STZ_E_ARG;
+    StzPlan *pl = stz_self_plan(ctx);
+    if (!pl) return STZ_E_HIP;
+    if (batch > 0) {
+        int rc = set_batch(ctx, batch);
+        if (rc) return rc;
+        pl->batch = batch;
+    }
+    if (use_graph >= 0) pl->use_graph = use_graph != 0;
+    if (window_words > 0) {
+        if (window_words < 4096 || window_words > (1ll << 28)) return STZ_E_ARG;
+        ctx->rs->wsize = window_words;
+        ctx->rs->force_rebuild = true;
+    }
+    return STZ_OK;
+}
+
+int stz_run_steps(stz_ctx *ctx, int nsteps, stz_step_result *results, int *done)
+{
+    if (!ctx || nsteps < 0) return STZ_E_ARG;
+    int d = 0, rc = 0;
+    if (nsteps > 0) {
+        StzPlan *pl = stz_self_plan(ctx);
+        if (!pl) return STZ_E_HIP;
+        stz_step_result *const rr[1] = {results};
+        int prc = stz_plan_run(pl, nsteps, rr, &d, &rc, ctx->rs->stop_at_print);
+        if (prc) rc = prc;
+    }
+    if (done) *done = d;
+    return rc;
+}
+
+int stz_run_stop_at_print(stz_ctx *ctx, int on)
+{
+    if (!ctx) return STZ_E_ARG;
+    int rc = ensure_resident(ctx);
+    if (rc) return rc;
+    ctx->rs->stop_at_print = on != 0;
+    return STZ_OK;
+}
+
+int stz_ensemble_create(stz_ctx *const *members, int n, stz_ensemble **out)
+{
+    if (!members || n < 1 || !out) return STZ_E_ARG;
+    for (int i = 0; i < n; ++i)
+        if (!members[i]) return STZ_E_ARG;
+    *out = nullptr;
+    stz_ensemble *e = new stz_ensemble();
+    int rc = stz_plan_create(members, n, &e->pl);
+    if (rc) {
+        delete e;
+        return rc;
+    }
+    for (int i = 0; i < n; ++i) {
+        rc = set_batch(members[i], e->pl->batch);
+        if (rc) {
+            stz_plan_destroy(e->pl);
+            delete e;
+            return rc;
+        }
+    }
+    *out = e;
+    return STZ_OK;
+}
+
+int stz_ensemble_config(stz_ensemble *e, int batch, int use_graph)
+{
+    if (!e || batch < 0 || batch > 65536) return STZ_E_ARG;
+    if (batch > 0) {
+        for (stz_ctx *c : e->pl->all) {
+            int rc = set_batch(c, batch);
+            if (rc) return rc;
+        }
+        e->pl->batch = batch;
+    }
+    if (use_graph >= 0) e->pl->use_graph = use_graph != 0;
+    return STZ_OK;
+}
+
+int stz_ensemble_run(stz_ensemble *e, int nsteps, stz_step_result *const *results, int *done, int *rcs)
+{
+    if (!e || nsteps < 0 || !done || !rcs) return STZ_E_ARG;
+    return stz_plan_run(e->pl, nsteps, results, done, rcs, 0);
+}
+
+void stz_ensemble_destroy(stz_ensemble *e)
+{
+    if (!e) return;
+    if (e->pl) {
+        (void)hipStreamSynchronize(e->pl->stream);
+        stz_plan_destroy(e->pl);
+    }
+    delete e;
+}
+
+}  // extern "C"
+
+extern "C" {
+
+// Host check of the lookup window against the stream drawn in order: seeds a generator,
+// skips `skip` words, builds a window of window_words words there and takes, from the
+// word `offset` into it, either a uniform and then ngauss gaussians (with_uniform = 1)
+// or ngauss gaussians, both from the window (with the kernels' index arithmetic) and
+// by drawing.  Outputs the values and the stream positions after them.
+int stz_mt_window_probe(unsigned long seed, long long skip, long long window_words, long long offset, int with_uniform,
+                        int ngauss, double *win_out, double *seq_out, long long *win_end, long long *seq_end)
+{
+    if (window_words < 16 || offset < 0 || offset >= window_words || ngauss < 0 || !win_out || !seq_out)
+        return STZ_E_ARG;
+    StzRng g;
+    g.seed(seed);
+    g.skip(skip);
+    StzMtWindow W;
+    W.build(g, window_words, 4);
+    MtDev m;
+    m.w = W.words.data();
+    for (int p = 0; p < 2; ++p) {
+        m.g[p] = W.g[p].data();
+        m.rank[p] = W.rank[p].data();
+        m.pair[p] = W.pair[p].data();
+        m.nacc[p] = W.nacc[p];
+    }
+    m.base = W.base;
+    m.len = W.len;
+    // from the window
+    long long pos = W.base + offset;
+    int o = 0;
+    if (with_uniform) {
+        if (pos - m.base >= m.len) return STZ_E_ARG;
+        win_out[o++] = m.w[pos - m.base] / 4294967296.0;
+        pos += 1;
+    }
+    if (ngauss > 0) {
+        int p;
+        const long long k0 = mt_first(m, pos, p);
+        if (k0 + ngauss > m.nacc[p]) return STZ_E_ARG;
+        for (int i = 0; i < ngauss; ++i) win_out[o + i] = m.g[p][k0 + i];
+        pos = mt_after(m, p, k0 + ngauss - 1);
+    }
+    if (win_end) *win_end = pos;
+    // drawn in order
+    StzRng s = g;
+    s.skip(offset);
+    o = 0;
+    if (with_uniform) seq_out[o++] = s.uniform();
+    for (int i = 0; i < ngauss; ++i) seq_out[o + i] = s.gaussian(1.0);
+    if (seq_end) *seq_end = s.count;
+    return STZ_OK;
+}
+
+}  // extern "C"

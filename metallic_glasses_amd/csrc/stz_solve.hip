@@ -37,7 +37,50 @@ struct SolveConsts {
     long long N;
     int rows;           // n1*n2
     int h0, hp;
+    int C;
 };
+
+// Per-replica view of a batched launch: the replica index is a grid coordinate, and
+// its fields, uniform strain and step kind come from reps[r].  With reps == nullptr the
+// launch is the single context's own (pointers from the arguments).
+template <typename R>
+struct View {
+    const double *eps0;
+    cpx<R> *spec;
+    R *sigma;
+    double *spart;
+    int *nanflag;
+    const int *gate;
+    double ea[6];
+};
+
+template <typename R>
+__device__ __forceinline__ View<R> view_of(const SolveRep *reps, int r, const SolveConsts &k, const double *eps0,
+                                           cpx<R> *spec, R *sigma, double *spart, int *nanflag)
+{
+    View<R> v;
+    if (reps) {
+        const SolveRep &q = reps[r];
+        v.eps0 = q.eps0;
+        v.spec = (cpx<R> *)q.spec;
+        v.sigma = (R *)q.sigma;
+        v.spart = q.spart;
+        v.nanflag = reinterpret_cast<int *>(q.res + 15);
+        v.gate = reinterpret_cast<const int *>(q.res + RES_KIND);
+#pragma unroll
+        for (int c = 0; c < 6; ++c) v.ea[c] = q.epsavg[c];
+    } else {
+        v.eps0 = eps0;
+        v.spec = spec;
+        v.sigma = sigma;
+        v.spart = spart;
+        v.nanflag = nanflag;
+        v.gate = k.gate;
+#pragma unroll
+        for (int c = 0; c < 6; ++c) v.ea[c] = k.epsavg[c];
+    }
+    return v;
+}
 
 // --- elementwise elasticity (literal expression order of the reference) ----------
 #pragma clang fp contract(off)
@@ -186,17 +229,19 @@ __device__ __forceinline__ bool gated_off_p(const int *gate)
 {
     return gate && *gate != KIND_THERMAL && *gate != KIND_ATHERMAL;
 }
-__device__ __forceinline__ bool gated_off(const SolveConsts &k) { return gated_off_p(k.gate); }
 
 // ---------------------------------------------------------------------------------
 // k_fwd_x: eps0 -> round trip -> R2C along x for NR rows and all C components
 // ---------------------------------------------------------------------------------
 template <typename R, int L0, int NR, int C>
-__global__ __launch_bounds__(NT_SOLVE) void k_fwd_x(const double *__restrict__ eps0, cpx<R> *__restrict__ spec,
-                                                    const cpx<R> *__restrict__ twg, SolveConsts k)
+__global__ __launch_bounds__(NT_SOLVE) void k_fwd_x(const double *__restrict__ eps0_a, cpx<R> *__restrict__ spec_a,
+                                                    const cpx<R> *__restrict__ twg, SolveConsts k, const SolveRep *reps)
 {
     constexpr int NP = NR / 2;            // packed complex lines per component
-    if (gated_off(k)) return;
+    const View<R> vw = view_of<R>(reps, blockIdx.y, k, eps0_a, spec_a, nullptr, nullptr, nullptr);
+    if (gated_off_p(vw.gate)) return;
+    const double *__restrict__ eps0 = vw.eps0;
+    cpx<R> *__restrict__ spec = vw.spec;
     constexpr int NL = C * NP;
     constexpr int LS = NL + 1;            // odd stride against bank conflicts
     __shared__ cpx<R> buf[L0 * LS];
@@ -211,7 +256,7 @@ __global__ __launch_bounds__(NT_SOLVE) void k_fwd_x(const double *__restrict__ e
         double ev[C], rt[C];
 #pragma unroll
         for (int c = 0; c < C; ++c) ev[c] = eps0[(long long)c * k.N + v];
-        round_trip<C>(ev, k.epsavg, k.lambda, k.mu, rt);
+        round_trip<C>(ev, vw.ea, k.lambda, k.mu, rt);
 #pragma unroll
         for (int c = 0; c < C; ++c) bufr[2 * (i0 * LS + c * NP + (rl >> 1)) + (rl & 1)] = (R)rt[c];
     }
@@ -236,17 +281,19 @@ __global__ __launch_bounds__(NT_SOLVE) void k_fwd_x(const double *__restrict__ e
 // k_ypass: complex FFT along y (stride hp) for a tile of W k0-columns, in place
 // ---------------------------------------------------------------------------------
 template <typename R, int L1, int W>
-__global__ __launch_bounds__(NT_SOLVE) void k_ypass(cpx<R> *__restrict__ spec, const cpx<R> *__restrict__ twg,
-                                                    SolveConsts k, R sgn)
+__global__ __launch_bounds__(NT_SOLVE) void k_ypass(cpx<R> *__restrict__ spec_a, const cpx<R> *__restrict__ twg,
+                                                    SolveConsts k, R sgn, const SolveRep *reps)
 {
-    if (gated_off(k)) return;
+    const View<R> vw = view_of<R>(reps, blockIdx.z / k.C, k, nullptr, spec_a, nullptr, nullptr, nullptr);
+    if (gated_off_p(vw.gate)) return;
+    cpx<R> *__restrict__ spec = vw.spec;
     __shared__ cpx<R> buf[L1 * W];
     __shared__ cpx<R> tw[L1];
     const int tid = threadIdx.x;
     for (int i = tid; i < L1; i += NT_SOLVE) tw[i] = twg[i];
     const int k0base = blockIdx.x * W;
     const int i2 = blockIdx.y;
-    const int c = blockIdx.z;
+    const int c = blockIdx.z % k.C;
     cpx<R> *base = spec + ((long long)c * k.rows + (long long)i2 * L1) * k.hp + k0base;
     for (int e = tid; e < L1 * W; e += NT_SOLVE) {
         const int col = e % W, p = e / W;
@@ -267,11 +314,13 @@ __global__ __launch_bounds__(NT_SOLVE) void k_ypass(cpx<R> *__restrict__ spec, c
 // (3-D: at fixed y index blockIdx.y; 2-D: the y axis itself is the last axis)
 // ---------------------------------------------------------------------------------
 template <typename R, int LZ, int W, int C, int DIM>
-__global__ __launch_bounds__(NT_SOLVE) void k_fused(cpx<R> *__restrict__ spec, const cpx<R> *__restrict__ twg,
-                                                    SolveConsts k)
+__global__ __launch_bounds__(NT_SOLVE) void k_fused(cpx<R> *__restrict__ spec_a, const cpx<R> *__restrict__ twg,
+                                                    SolveConsts k, const SolveRep *reps)
 {
     constexpr int NE = LZ * W;
-    if (gated_off(k)) return;
+    const View<R> vw = view_of<R>(reps, blockIdx.z, k, nullptr, spec_a, nullptr, nullptr, nullptr);
+    if (gated_off_p(vw.gate)) return;
+    cpx<R> *__restrict__ spec = vw.spec;
     constexpr int EPT = (NE + NT_SOLVE - 1) / NT_SOLVE;
     __shared__ cpx<R> buf[NE];
     __shared__ cpx<R> tw[LZ];
@@ -364,16 +413,22 @@ __global__ __launch_bounds__(NT_SOLVE) void k_fused(cpx<R> *__restrict__ spec, c
 // k_inv_x: C2R along x + stress epilogue + per-block partial sums of sigma
 // ---------------------------------------------------------------------------------
 template <typename R, int L0, int NR, int C>
-__global__ __launch_bounds__(NT_SOLVE) void k_inv_x(const cpx<R> *__restrict__ spec, const double *__restrict__ eps0,
-                                                    R *__restrict__ sigma, double *__restrict__ spart,
-                                                    int *__restrict__ nanflag, const cpx<R> *__restrict__ twg,
-                                                    SolveConsts k)
+__global__ __launch_bounds__(NT_SOLVE) void k_inv_x(cpx<R> *__restrict__ spec_a, const double *__restrict__ eps0_a,
+                                                    R *__restrict__ sigma_a, double *__restrict__ spart_a,
+                                                    int *__restrict__ nanflag_a, const cpx<R> *__restrict__ twg,
+                                                    SolveConsts k, const SolveRep *reps)
 {
     constexpr int NP = NR / 2;
     constexpr int NL = C * NP;
     constexpr int LS = NL + 1;
     constexpr int H = L0 / 2;
-    if (gated_off(k)) return;
+    const View<R> vw = view_of<R>(reps, blockIdx.y, k, eps0_a, spec_a, sigma_a, spart_a, nanflag_a);
+    if (gated_off_p(vw.gate)) return;
+    const cpx<R> *__restrict__ spec = vw.spec;
+    const double *__restrict__ eps0 = vw.eps0;
+    R *__restrict__ sigma = vw.sigma;
+    double *__restrict__ spart = vw.spart;
+    int *__restrict__ nanflag = vw.nanflag;
     __shared__ cpx<R> buf[L0 * LS];
     __shared__ cpx<R> tw[L0];
     __shared__ double red[NT_SOLVE];
@@ -411,12 +466,12 @@ __global__ __launch_bounds__(NT_SOLVE) void k_inv_x(const cpx<R> *__restrict__ s
         double ev[C], rt[C], d[C], s[C];
 #pragma unroll
         for (int c = 0; c < C; ++c) ev[c] = eps0[(long long)c * k.N + v];
-        round_trip<C>(ev, k.epsavg, k.lambda, k.mu, rt);
+        round_trip<C>(ev, vw.ea, k.lambda, k.mu, rt);
 #pragma unroll
         for (int c = 0; c < C; ++c) {
             const double z = (double)bufr[2 * (i0 * LS + c * NP + (rl >> 1)) + (rl & 1)];
             bad |= (z != z);
-            const double tot = z / (double)k.N + k.epsavg[c];   // STZ/elastic.c:822
+            const double tot = z / (double)k.N + vw.ea[c];   // STZ/elastic.c:822
             d[c] = tot - rt[c];
         }
         stress_of<C>(d, k.lambda, k.mu, s);
@@ -443,8 +498,14 @@ __global__ __launch_bounds__(NT_SOLVE) void k_inv_x(const cpx<R> *__restrict__ s
 constexpr int NT_MEAN = 1024;
 template <int C>
 __global__ __launch_bounds__(NT_MEAN) void k_mean(const double *__restrict__ spart, int nblocks, double invN,
-                                                  double *__restrict__ out, const int *gate)
+                                                  double *__restrict__ out, const int *gate, const SolveRep *reps)
 {
+    if (reps) {
+        const SolveRep &q = reps[blockIdx.x];
+        spart = q.spart;
+        out = q.res;
+        gate = reinterpret_cast<const int *>(q.res + RES_KIND);
+    }
     if (gated_off_p(gate)) return;
     __shared__ double red[C][NT_MEAN];
     const int tid = threadIdx.x;
@@ -490,6 +551,9 @@ struct Launch {
     stz_ctx *ctx;
     SolveConsts k;
     int *nanflag;
+    const SolveRep *reps = nullptr;   // batched: nrep replicas of ctx's shape
+    int nrep = 1;
+    hipStream_t stream = nullptr;
 
     // largest power-of-two row count <= rows_for(L0) that divides the row total
     int pick_rows(int L0max) const
@@ -503,16 +567,16 @@ struct Launch {
     void go_fwd_x(dim3 grid)
     {
         if constexpr (NRV <= rows_for(L0))
-            hipLaunchKernelGGL((k_fwd_x<R, L0, NRV, C>), grid, dim3(NT_SOLVE), 0, ctx->stream, ctx->eps0,
-                               (cpx<R> *)ctx->spec, (const cpx<R> *)ctx->tw.tw[0], k);
+            hipLaunchKernelGGL((k_fwd_x<R, L0, NRV, C>), grid, dim3(NT_SOLVE), 0, stream, ctx->eps0,
+                               (cpx<R> *)ctx->spec, (const cpx<R> *)ctx->tw.tw[0], k, reps);
     }
 
     template <int L0, int NRV, int C>
     void go_inv_x(dim3 grid, double *spart)
     {
         if constexpr (NRV <= rows_for(L0))
-            hipLaunchKernelGGL((k_inv_x<R, L0, NRV, C>), grid, dim3(NT_SOLVE), 0, ctx->stream, (const cpx<R> *)ctx->spec,
-                               ctx->eps0, (R *)ctx->sigma, spart, nanflag, (const cpx<R> *)ctx->tw.tw[0], k);
+            hipLaunchKernelGGL((k_inv_x<R, L0, NRV, C>), grid, dim3(NT_SOLVE), 0, stream, (cpx<R> *)ctx->spec,
+                               ctx->eps0, (R *)ctx->sigma, spart, nanflag, (const cpx<R> *)ctx->tw.tw[0], k, reps);
     }
 
     template <int L0, int C>
@@ -520,7 +584,7 @@ struct Launch {
     {
         const int NR = pick_rows(rows_for(L0));
         if (!NR) return STZ_E_ARG;
-        const dim3 grid(k.rows / NR);
+        const dim3 grid(k.rows / NR, nrep);
         switch (NR) {
         case 32: go_fwd_x<L0, 32, C>(grid); break;
         case 16: go_fwd_x<L0, 16, C>(grid); break;
@@ -538,7 +602,7 @@ struct Launch {
         const int NR = pick_rows(rows_for(L0));
         if (!NR) return STZ_E_ARG;
         *nblocks = k.rows / NR;
-        const dim3 grid(k.rows / NR);
+        const dim3 grid(k.rows / NR, nrep);
         switch (NR) {
         case 32: go_inv_x<L0, 32, C>(grid, spart); break;
         case 16: go_inv_x<L0, 16, C>(grid, spart); break;
@@ -554,9 +618,9 @@ struct Launch {
     int ypass_n(R sgn)
     {
         constexpr int W = cols_for(L1);
-        const dim3 grid((k.h0 + W - 1) / W, k.n[2], ctx->C);
-        hipLaunchKernelGGL((k_ypass<R, L1, W>), grid, dim3(NT_SOLVE), 0, ctx->stream, (cpx<R> *)ctx->spec,
-                           (const cpx<R> *)ctx->tw.tw[1], k, sgn);
+        const dim3 grid((k.h0 + W - 1) / W, k.n[2], ctx->C * nrep);
+        hipLaunchKernelGGL((k_ypass<R, L1, W>), grid, dim3(NT_SOLVE), 0, stream, (cpx<R> *)ctx->spec,
+                           (const cpx<R> *)ctx->tw.tw[1], k, sgn, reps);
         STZ_CHECK_LAUNCH();
         return STZ_OK;
     }
@@ -565,9 +629,9 @@ struct Launch {
     int fused_n()
     {
         constexpr int W = fused_cols_for(LZ, C);
-        const dim3 grid((k.h0 + W - 1) / W, DIM == 3 ? k.n[1] : 1);
-        hipLaunchKernelGGL((k_fused<R, LZ, W, C, DIM>), grid, dim3(NT_SOLVE), 0, ctx->stream, (cpx<R> *)ctx->spec,
-                           (const cpx<R> *)ctx->tw.tw[DIM - 1], k);
+        const dim3 grid((k.h0 + W - 1) / W, DIM == 3 ? k.n[1] : 1, nrep);
+        hipLaunchKernelGGL((k_fused<R, LZ, W, C, DIM>), grid, dim3(NT_SOLVE), 0, stream, (cpx<R> *)ctx->spec,
+                           (const cpx<R> *)ctx->tw.tw[DIM - 1], k, reps);
         STZ_CHECK_LAUNCH();
         return STZ_OK;
     }
@@ -588,10 +652,13 @@ struct Launch {
     }
 
 template <typename R>
-int solve_impl(stz_ctx *ctx, int gated)
+int solve_impl(stz_ctx *ctx, int gated, int nrep, const SolveRep *reps, hipStream_t stream)
 {
     Launch<R> Lr;
     Lr.ctx = ctx;
+    Lr.reps = reps;
+    Lr.nrep = nrep;
+    Lr.stream = stream;
     SolveConsts &k = Lr.k;
     k.gate = gated ? reinterpret_cast<const int *>(ctx->res + RES_KIND) : nullptr;
     k.lambda = ctx->k.lambda;
@@ -607,6 +674,7 @@ int solve_impl(stz_ctx *ctx, int gated)
     k.rows = ctx->n[1] * ctx->n[2];
     k.h0 = ctx->h0;
     k.hp = ctx->hp;
+    k.C = ctx->C;
     Lr.nanflag = reinterpret_cast<int *>(ctx->res + 15);
     int rc = STZ_OK;
     const int C = ctx->C;
@@ -642,8 +710,8 @@ int solve_impl(stz_ctx *ctx, int gated)
     }
     if (rc) return rc;
     tmark(ctx, 5);
-    if (C == 3) hipLaunchKernelGGL((k_mean<3>), dim3(1), dim3(NT_MEAN), 0, ctx->stream, ctx->spart, nblocks, k.invN, ctx->res, k.gate);
-    else hipLaunchKernelGGL((k_mean<6>), dim3(1), dim3(NT_MEAN), 0, ctx->stream, ctx->spart, nblocks, k.invN, ctx->res, k.gate);
+    if (C == 3) hipLaunchKernelGGL((k_mean<3>), dim3(nrep), dim3(NT_MEAN), 0, stream, ctx->spart, nblocks, k.invN, ctx->res, k.gate, reps);
+    else hipLaunchKernelGGL((k_mean<6>), dim3(nrep), dim3(NT_MEAN), 0, stream, ctx->spart, nblocks, k.invN, ctx->res, k.gate, reps);
     STZ_CHECK_LAUNCH();
     tmark(ctx, 6);
     return STZ_OK;
@@ -694,7 +762,14 @@ void stz_fft_free(stz_ctx *ctx)
 
 int stz_solve_launch(stz_ctx *ctx, int gated)
 {
-    return ctx->prec == STZ_FP64 ? solve_impl<double>(ctx, gated) : solve_impl<float>(ctx, gated);
+    return ctx->prec == STZ_FP64 ? solve_impl<double>(ctx, gated, 1, nullptr, ctx->stream)
+                                 : solve_impl<float>(ctx, gated, 1, nullptr, ctx->stream);
+}
+
+int stz_solve_launch_batched(stz_ctx *proto, int nrep, const SolveRep *reps, hipStream_t stream)
+{
+    return proto->prec == STZ_FP64 ? solve_impl<double>(proto, 1, nrep, reps, stream)
+                                   : solve_impl<float>(proto, 1, nrep, reps, stream);
 }
 
 // rows of the stress-mean partial array: the largest block count of k_inv_x

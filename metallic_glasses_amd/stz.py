@@ -118,6 +118,15 @@ def lib():
         L.stz_timing_enable.argtypes = [vp, C.c_int]
         L.stz_timing_get.argtypes = [vp, dp, P(C.c_longlong)]
         L.stz_timing_solve_get.argtypes = [vp, dp, P(C.c_longlong)]
+        L.stz_run_config.argtypes = [vp, C.c_int, C.c_int, C.c_longlong]
+        L.stz_run_stop_at_print.argtypes = [vp, C.c_int]
+        L.stz_ensemble_create.argtypes = [P(vp), C.c_int, P(vp)]
+        L.stz_ensemble_config.argtypes = [vp, C.c_int, C.c_int]
+        L.stz_ensemble_run.argtypes = [vp, C.c_int, P(P(StepResult)), P(C.c_int), P(C.c_int)]
+        L.stz_ensemble_destroy.argtypes = [vp]
+        L.stz_ensemble_destroy.restype = None
+        L.stz_mt_window_probe.argtypes = [C.c_ulong, C.c_longlong, C.c_longlong, C.c_longlong, C.c_int, C.c_int,
+                                          dp, dp, P(C.c_longlong), P(C.c_longlong)]
         _lib = L
     return _lib
 
@@ -255,11 +264,17 @@ class Stz:
         return rc, r
 
     def run_steps(self, n):
-        """Runs n steps in the library; returns (rc, results[:done])."""
+        """Runs n device-resident steps; returns (rc, results[:done])."""
         arr = (StepResult * max(n, 1))()
         done = C.c_int()
         rc = self._check(lib().stz_run_steps(self._h, int(n), arr, C.byref(done)))
         return rc, arr[:done.value]
+
+    def run_config(self, batch=0, use_graph=-1, window_words=0):
+        self._check(lib().stz_run_config(self._h, int(batch), int(use_graph), int(window_words)))
+
+    def stop_at_print(self, on=True):
+        self._check(lib().stz_run_stop_at_print(self._h, 1 if on else 0))
 
     def step_kmc(self):
         r = StepResult()
@@ -315,3 +330,54 @@ class Stz:
         g = C.c_double()
         self._check(lib().stz_rng_gaussian(self._h, float(sigma), C.byref(g)))
         return g.value
+
+
+class Ensemble:
+    """Replicas advanced together by the batched resident step (stz_ensemble_*)."""
+
+    def __init__(self, members, batch=0, use_graph=-1):
+        self.members = list(members)
+        hs = (C.c_void_p * len(self.members))(*[m._h.value for m in self.members])
+        self._h = C.c_void_p()
+        rc = lib().stz_ensemble_create(hs, len(self.members), C.byref(self._h))
+        if rc:
+            raise StzError(rc, "stz_ensemble_create")
+        if batch or use_graph >= 0:
+            rc = lib().stz_ensemble_config(self._h, int(batch), int(use_graph))
+            if rc:
+                raise StzError(rc, "stz_ensemble_config")
+
+    def run(self, n, keep=True):
+        """Advances every replica by up to n steps; returns (rcs, [results per replica])."""
+        k = len(self.members)
+        bufs = [(StepResult * max(n, 1))() for _ in range(k)] if keep else None
+        ptrs = (C.POINTER(StepResult) * k)(*[C.cast(b, C.POINTER(StepResult)) for b in bufs]) if keep else None
+        done = (C.c_int * k)()
+        rcs = (C.c_int * k)()
+        rc = lib().stz_ensemble_run(self._h, int(n), ptrs, done, rcs)
+        if rc:
+            raise StzError(rc, lib().stz_last_error(self.members[0]._h).decode())
+        out = [bufs[i][:done[i]] for i in range(k)] if keep else [done[i] for i in range(k)]
+        return list(rcs), out
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().stz_ensemble_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        self.close()
+
+
+def mt_window_probe(seed, skip, window_words, offset, with_uniform, ngauss):
+    """Lookup-window draws vs the same draws in order (host only)."""
+    n = ngauss + (1 if with_uniform else 0)
+    w = np.zeros(max(n, 1))
+    s = np.zeros(max(n, 1))
+    we = C.c_longlong()
+    se = C.c_longlong()
+    rc = lib().stz_mt_window_probe(int(seed), int(skip), int(window_words), int(offset), int(with_uniform),
+                                   int(ngauss), _dp(w), _dp(s), C.byref(we), C.byref(se))
+    if rc:
+        raise StzError(rc, "stz_mt_window_probe")
+    return w[:n], s[:n], we.value, se.value
