@@ -40,6 +40,10 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--seed", type=int, default=1234)
     ap.add_argument("--timing-steps", type=int, default=200)
+    ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--ensemble-replicas", type=int, default=64)
+    ap.add_argument("--ensemble-steps", type=int, default=200)
+    ap.add_argument("--ensemble-warmup", type=int, default=200)
     return ap.parse_args()
 
 
@@ -47,6 +51,22 @@ def rate_bytes_per_voxel(C, M, S, b):
     # sigma (C), Q0 (M), transformation strains (S*M), previous strain (S) in the storage
     # precision, plus Perm, Temp, tlast in FP64
     return (C + M + S * M + S) * b + 3 * 8
+
+
+def fft_table(dims, C, N, b, sms, sn):
+    """Algorithmic bytes and achieved GB/s of the five stress-solve kernels (storage
+    precision b bytes; eps0 is always FP64)."""
+    h0 = dims[0] // 2 + 1
+    spec = C * dims[1] * dims[2] * h0 * 2 * b
+    fft_bytes = [C * N * 8 + spec, 2 * spec, 2 * spec, 2 * spec, spec + C * N * 8 + C * N * b]
+    names = ["k_fwd_x", "k_ypass_fwd", "k_fused", "k_ypass_inv", "k_inv_x"]
+    out = []
+    for i in range(5):
+        t = sms[i] / max(sn[i], 1)
+        gbs = fft_bytes[i] / (t * 1e-3) / 1e9 if t > 0 else 0.0
+        out.append({"kernel": names[i], "avg_ms": round(t, 5), "GB/s": round(gbs, 1),
+                    "frac": round(gbs / HBM_PEAK_GBS, 4)})
+    return out
 
 
 def snapshot(g, mg):
@@ -107,7 +127,8 @@ def main():
         nev += r.kind == 1
         if rc:
             break
-    print(f"# warm-up: {nst} steps, {nev} thermal events", file=sys.stderr, flush=True)
+    g.run_steps(20)      # resident path warm (lookup window built) before the timed region
+    print(f"# warm-up: {nst} + 20 steps, {nev} thermal events", file=sys.stderr, flush=True)
 
     snap = None
     if not a.no_cpu or not a.no_mixed:
@@ -168,19 +189,7 @@ def main():
                      "bytes_per_launch": int(k4_bytes), "avg_ms": round(k4_ms, 5)},
         "solve_ms": round(solve_ms, 5),
     }
-    # FFT roofline: algorithmic bytes of the five solve kernels
-    b = 8
-    h0 = dims[0] // 2 + 1
-    spec = C * dims[1] * dims[2] * h0 * 2 * b
-    real = C * N * b
-    fft_bytes = [real + spec, 2 * spec, 2 * spec, 2 * spec, spec + real + C * N * 8]
-    fft = []
-    names = ["k_fwd_x", "k_ypass_fwd", "k_fused", "k_ypass_inv", "k_inv_x"]
-    for i in range(5):
-        t = sms[i] / max(sn[i], 1)
-        gbs = fft_bytes[i] / (t * 1e-3) / 1e9 if t > 0 else 0.0
-        fft.append({"kernel": names[i], "avg_ms": round(t, 5), "GB/s": round(gbs, 1),
-                    "frac": round(gbs / HBM_PEAK_GBS, 4)})
+    fft = fft_table(dims, C, N, 8, sms, sn)
     out["fft_roofline"] = fft
 
     if not a.no_mixed and snap is not None:
@@ -205,6 +214,13 @@ def main():
         msm, nm, _, _ = gm.timing_get()
         gm.timing(False)
         k4m = msm[0] / max(nm[0], 1)
+        gm.timing(True)
+        for _ in range(a.timing_steps // 4):
+            rc_, _r = gm.step()
+            if rc_:
+                break
+        _, _, smsm, snm = gm.timing_get()
+        gm.timing(False)
         kb = rate_bytes_per_voxel(C, M, S, 4) * N
         same = 0
         for x, y in zip(events, events_m):
@@ -228,7 +244,8 @@ def main():
                         "steps_identical": nsame, "ms_per_step": round(dtm * 1e3 / a.steps, 4),
                         "speedup_vs_fp64": round(float(kinds_m[1] / dtm) / ev_fp64, 3),
                         "k_rates_GBs": round(kb / (k4m * 1e-3) / 1e9, 1),
-                        "events_identical_to_fp64": same}
+                        "events_identical_to_fp64": same,
+                        "fft_roofline": fft_table(dims, C, N, 4, smsm, snm)}
         gm.close()
         print(f"# mixed: {out['mixed']}", file=sys.stderr, flush=True)
 
@@ -270,7 +287,102 @@ def main():
                                "events_matching_gpu": cmatch}
         print(f"# cpu: {out['cpu_baseline']}", file=sys.stderr, flush=True)
 
+    if not a.no_ensemble:
+        out["ensemble"] = ensemble_leg(a, mg)
+        print(f"# ensemble: {out['ensemble']}", file=sys.stderr, flush=True)
+
     print(json.dumps(out))
+
+
+ENSEMBLE_GRIDS = [(32, 32, 32), (64, 64, 128), (96, 96, 96), (128, 128, 128)]
+
+
+def ensemble_plan(nrep):
+    """Replica r: grid ENSEMBLE_GRIDS[r % 4], seed r, (T_r, std_r) drawn in pairs from
+    MT19937 seeded 4242 (numpy's legacy generator): T ~ U[250, 450] K, std ~ U[0.1, 0.3] eV."""
+    rs = np.random.RandomState(4242)
+    plan = []
+    for r in range(nrep):
+        T = float(rs.uniform(250.0, 450.0))
+        sd = float(rs.uniform(0.1, 0.3))
+        plan.append((ENSEMBLE_GRIDS[r % 4], r, T, sd))
+    return plan
+
+
+def make_contexts(mg, kws, threads=16):
+    """Creates and initialises contexts on parallel host threads (the MT19937 initial
+    state of each replica is drawn on the host in the reference's order)."""
+    from concurrent.futures import ThreadPoolExecutor
+    with ThreadPoolExecutor(max_workers=threads) as ex:
+        return list(ex.map(lambda kw: mg.Stz(**kw), kws))
+
+
+def count_events(recs):
+    return sum(1 for r in recs if r.kind == 1)
+
+
+def ensemble_leg(a, mg):
+    """Batched mixed-precision ensemble vs the plain FP64 path on the same replicas."""
+    plan = ensemble_plan(a.ensemble_replicas)
+    kws = [mg.config_3d(n=g, seed=seed, q0_std=sd, temperature=T, precision=mg.MIXED) for g, seed, T, sd in plan]
+    t0 = time.perf_counter()
+    members = make_contexts(mg, kws)
+    for m in members:
+        m.set_print(5, 1.0)
+    init_s = time.perf_counter() - t0
+    print(f"# ensemble: {len(members)} replicas initialised in {init_s:.1f}s", file=sys.stderr, flush=True)
+    # FP64 twins of one replica per grid, stepped alone, for the precision comparison
+    twin_ids = list(range(min(4, len(plan))))
+    twins = make_contexts(mg, [dict(kws[i], precision=mg.FP64) for i in twin_ids])
+    for t in twins:
+        t.set_print(5, 1.0)
+    ens = mg.Ensemble(members, batch=64)
+    ens.run(a.ensemble_warmup, keep=False)
+    for t in twins:
+        t.run_steps(a.ensemble_warmup)
+    t0 = time.perf_counter()
+    rcs, outs = ens.run(a.ensemble_steps)
+    dt = time.perf_counter() - t0
+    events = [count_events(o) for o in outs]
+    steps = [len(o) for o in outs]
+    value = sum(events) / dt
+    # plain FP64 path, one replica per grid: seconds per step and per-step agreement
+    t_step = {}
+    cmp = []
+    for j, i in enumerate(twin_ids):
+        t0 = time.perf_counter()
+        _, recs64 = twins[j].run_steps(a.ensemble_steps)
+        t_step[plan[i][0]] = (time.perf_counter() - t0) / max(len(recs64), 1)
+        same = 0
+        for x, y in zip(outs[i], recs64):
+            if (x.kind, x.site, x.mode, x.n_transformed) != (y.kind, y.site, y.mode, y.n_transformed):
+                break
+            same += 1
+        s64 = np.array([r.stress[5] for r in recs64[:same]])
+        smx = np.array([r.stress[5] for r in outs[i][:same]])
+        ss = float(np.max(np.abs(smx - s64)) / max(np.max(np.abs(s64)), 1e-30)) if same else None
+        ferr = None
+        if same == len(recs64) == len(outs[i]):
+            f64 = twins[j].download(mg.F_SIGMA)
+            fmx = members[i].download(mg.F_SIGMA)
+            ferr = float(np.linalg.norm(fmx - f64) / np.linalg.norm(f64))
+        cmp.append({"replica": i, "grid": list(plan[i][0]), "steps_identical": same, "ss_curve_rel_err": ss,
+                    "stress_rel_err": ferr})
+    fp64_time = sum(steps[i] * t_step[plan[i][0]] for i in range(len(plan)))
+    fp64_value = sum(events) / fp64_time
+    for t in twins:
+        t.close()
+    ens.close()
+    for m in members:
+        m.close()
+    return {"value": round(value, 2), "unit": "replica-events/s", "replicas": len(plan),
+            "grids": [list(g) for g in ENSEMBLE_GRIDS], "precision": "mixed", "steps_per_replica": a.ensemble_steps,
+            "events": int(sum(events)), "seconds": round(dt, 3),
+            "fp64_plain_equivalent": round(fp64_value, 2),
+            "speedup_vs_fp64_plain": round(value / fp64_value, 3),
+            "fp64_plain_ms_per_step": {"x".join(map(str, g)): round(v * 1e3, 4) for g, v in t_step.items()},
+            "precision_check": cmp, "init_seconds": round(init_s, 1),
+            "sample": "T and barrier std per replica from MT19937(4242); replica r uses grid r mod 4 and seed r"}
 
 
 if __name__ == "__main__":

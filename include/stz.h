@@ -177,8 +177,9 @@ int stz_step_print(stz_ctx *ctx, stz_step_result *res);
 int stz_run_steps(stz_ctx *ctx, int nsteps, stz_step_result *results, int *done);
 
 /* Resident-run settings: steps per host synchronisation (batch, 0 = keep; default 64),
-   hipGraph replay of the step (1) or direct launches (0; -1 = keep; default 1), words of
-   the MT19937 lookup window (0 = keep; default 2^22).  No counterpart in the reference. */
+   use_graph = g > 0: replay g consecutive steps per hipGraph launch, 0: direct launches
+   (-1 = keep; default 0), words of the MT19937 lookup window (0 = keep; default 2^21).
+   No counterpart in the reference. */
 int stz_run_config(stz_ctx *ctx, int batch, int use_graph, long long window_words);
 /* With on = 1, stz_run_steps returns right after a print step (last res.printed = 1) and
    before its re-solve, like stz_step_kmc: the caller writes the print block's outputs
@@ -187,7 +188,7 @@ int stz_run_stop_at_print(stz_ctx *ctx, int on);
 
 /* Batched ensemble: replicas (contexts with their own grid, seed, temperature and barrier
    spread) advanced together, every kernel of a step launched once per shape class over
-   all its replicas, the step replayed from one hipGraph.  The contexts stay owned by the
+   all its replicas (directly or replayed from a hipGraph, as stz_run_config).  The contexts stay owned by the
    caller and must outlive the ensemble.  stz_ensemble_run advances each replica by up to
    nsteps steps exactly as stz_run_steps would: results[i] (optional, nsteps entries)
    receives replica i's records, done[i] their count and rcs[i] its return code.  No

@@ -19,14 +19,11 @@
 // control block (ctx->res, doubles unless noted), shared by kernels and host:
 //   [0..5] <sigma>  [8] R  [9] #(Q<0)  [10..12] site, mode, Q  [13] #athermal
 //   [15] NaN flag (int)  [16..21] mixed stress offsets  [24] Q<0 counter (u64)
-//   [25] group arrival counter (u32)  [26] step kind (int)  [27] dice*R  [28] dice
-//   [29] athermal arrival counter (u32)
+//   [26] step kind (int)  [27] dice*R  [28] dice
 #define RES_NNEG 24
-#define RES_ARRIVE 25
 #define RES_KIND 26
 #define RES_TAU 27
 #define RES_DICE 28
-#define RES_TICKET 29
 #define KIND_ELASTIC 0
 #define KIND_THERMAL 1
 #define KIND_ATHERMAL 2
@@ -113,6 +110,7 @@ struct RepDev {
     int *acnt, *aoff, *alist;
     RepState *st;
     stz_step_result *rec;
+    int *hflag;              // host-visible copy of the halt state (mapped pinned memory)
     MtDev mt;
     // constants (STZ/init.c:6-75)
     double kT, tTemp, ratepre, lnpre, voxel, thr, dt, permsoft, tempsoft, permcap, half_gamma0;
@@ -404,6 +402,33 @@ __device__ __forceinline__ void fft_lines(cpx<R> *buf, const cpx<R> *tw, R s)
     fft_plan<R, L, NL, LS, NT, 0, 1>(buf, tw, s);
 }
 
+// <sigma> from the stress-solve block partials spart[b][c] (fixed order: per-thread
+// strided runs, then a tree); a workgroup of exactly STZ_NT_MEAN threads; out[c] for c < C
+#define STZ_NT_MEAN 1024
+template <int C>
+__device__ void mean_partials(const double *spart, int nblocks, double invN, double *out)
+{
+    __shared__ double red[C][STZ_NT_MEAN];
+    const int tid = threadIdx.x;
+    double a[C];
+#pragma unroll
+    for (int c = 0; c < C; ++c) a[c] = 0.0;
+    for (int b = tid; b < nblocks; b += STZ_NT_MEAN)
+#pragma unroll
+        for (int c = 0; c < C; ++c) a[c] += spart[(long long)b * C + c];
+#pragma unroll
+    for (int c = 0; c < C; ++c) red[c][tid] = a[c];
+    __syncthreads();
+    for (int w = STZ_NT_MEAN / 2; w > 0; w >>= 1) {
+        if (tid < w)
+#pragma unroll
+            for (int c = 0; c < C; ++c) red[c][tid] = red[c][tid] + red[c][tid + w];
+        __syncthreads();
+    }
+    if (tid < C) out[tid] = red[tid][0] * invN;
+    __syncthreads();
+}
+
 // ---------------------------------------------------------------------------
 // host-side launchers implemented in the .hip files
 // ---------------------------------------------------------------------------
@@ -415,11 +440,12 @@ int stz_select_launch(stz_ctx *ctx, double target, int gated);
 int stz_apply_launch(stz_ctx *ctx, int site, int mode, int gated);
 int stz_athermal_launch(stz_ctx *ctx);
 int stz_athermal_apply_launch(stz_ctx *ctx, int n);
-int stz_athermal_fused_launch(stz_ctx *ctx, int gated);
+int stz_athermal_fused_launch(stz_ctx *ctx, int gated);   // + the |DWork| sum
 int stz_athermal_apply_range(stz_ctx *ctx, long long first, long long count);
 int stz_energy_launch(stz_ctx *ctx);
 bool stz_solve_supported(int dim, const int *n, int prec);
-int stz_solve_launch_batched(stz_ctx *proto, int nrep, const SolveRep *reps, hipStream_t stream);
+// batched solve without the mean kernel (the caller reduces the *nparts partials)
+int stz_solve_launch_batched(stz_ctx *proto, int nrep, const SolveRep *reps, hipStream_t stream, int *nparts);
 
 // resident step (stz_resident.hip)
 struct StzPlan;

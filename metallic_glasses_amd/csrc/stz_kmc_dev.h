@@ -548,38 +548,36 @@ __device__ void athermal_block(const AthArgs<R> &a, long long blk)
     if (tid == 0) a.socp[blk] = red[0];
 }
 
-// Arrival of one workgroup of a grid of `total`; true in the last one to arrive (its
-// threads then see every write the others made before arriving).  Resets the counter.
-__device__ __forceinline__ bool last_arrival(unsigned int *counter, unsigned int total, int *flag_sh)
+// fixed-order sum of the athermal block partials by one workgroup of NTH threads
+template <int NTH>
+__device__ double ordered_sum(const double *p, long long n)
 {
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __threadfence();
-        const bool last = atomicAdd(counter, 1u) == total - 1;
-        if (last) atomicExch(counter, 0u);
-        *flag_sh = last;
-    }
-    __syncthreads();
-    const bool last = *flag_sh != 0;
-    if (last) __threadfence();
-    return last;
-}
-
-// fixed-order sum of the athermal block partials (by the last workgroup)
-__device__ double ordered_sum(const double *p, long long n, double *red)
-{
+    __shared__ double red[NTH];
     const int tid = threadIdx.x;
     double a = 0.0;
-    for (long long i = tid; i < n; i += NT) a = a + p[i];
+    for (long long i = tid; i < n; i += NTH) a = a + p[i];
     red[tid] = a;
     __syncthreads();
-    for (int s2 = NT / 2; s2 > 0; s2 >>= 1) {
+    for (int s2 = NTH / 2; s2 > 0; s2 >>= 1) {
         if (tid < s2) red[tid] = red[tid] + red[tid + s2];
         __syncthreads();
     }
     const double r = red[0];
     __syncthreads();
     return r;
+}
+
+// group sums gsum[g] for the STZ_GRP_PER_BLOCK groups g0.. of one workgroup
+#define STZ_GRP_PER_BLOCK 4
+__device__ __forceinline__ void group_sums(const double *bsum, long long nb, double *gsum, long long ngrp, long long g0,
+                                           double *ws)
+{
+    for (int j = 0; j < STZ_GRP_PER_BLOCK; ++j) {
+        const long long g = g0 + j;
+        if (g >= ngrp) break;
+        const double t = group_sum(bsum, nb, g * STZ_GRP, ws);
+        if (threadIdx.x == 0) gsum[g] = t;
+    }
 }
 
 }  // namespace

@@ -25,27 +25,29 @@ __global__ __launch_bounds__(NT) void k_rates(RateArgs<R> a)
     rates_block<R, DIM>(a, blockIdx.x);
 }
 
-// Fixed-tree sums of STZ_GRP consecutive block sums; the last workgroup to finish
-// computes R, takes the Q<0 count and, when `decide` is set, the step kind
-// (STZ/bmg.c:161, STZ/evolution.c:134-136), the selection target and, for an athermal
-// step, the list offsets of the avalanche (more than `cap` voxels: overflow kind).
+// Fixed-tree sums of STZ_GRP consecutive block sums, STZ_GRP_PER_BLOCK groups per workgroup
 __global__ __launch_bounds__(NT) void k_group(const double *__restrict__ bsum, long long nb, double *__restrict__ gsum,
-                                              long long ng, double *__restrict__ res, int decide, double dt,
-                                              const double *dicep, const int *acnt, int *aoff, long long nblk,
-                                              long long cap)
+                                              long long ng)
 {
     __shared__ double ws[NW];
-    __shared__ int is_last;
+    group_sums(bsum, nb, gsum, ng, (long long)blockIdx.x * STZ_GRP_PER_BLOCK, ws);
+}
+
+// One workgroup: R from the group sums, the Q<0 count and, when `decide` is set, the
+// step kind (STZ/bmg.c:161, STZ/evolution.c:134-136), the selection target and, for an
+// athermal step, the list offsets of the avalanche (more than `cap` voxels: overflow).
+__global__ __launch_bounds__(NT) void k_decide(const double *__restrict__ gsum, long long ng, double *__restrict__ res,
+                                               int decide, double dt, const double *dicep, const int *acnt, int *aoff,
+                                               long long nblk, long long cap)
+{
+    __shared__ double ws[NW];
     __shared__ unsigned long long s_nn;
     const int tid = threadIdx.x;
-    const double t = group_sum(bsum, nb, (long long)blockIdx.x * STZ_GRP, ws);
-    if (tid == 0) gsum[blockIdx.x] = t;
-    if (!last_arrival(reinterpret_cast<unsigned int *>(res + RES_ARRIVE), (unsigned int)ng, &is_last)) return;
     const double R = group_total(gsum, ng, ws);
     if (tid == 0) {
         unsigned long long *cnt = reinterpret_cast<unsigned long long *>(res + RES_NNEG);
-        s_nn = atomicAdd(cnt, 0ull);
-        atomicExch(cnt, 0ull);
+        s_nn = *cnt;
+        *cnt = 0ull;
         res[8] = R;
         res[9] = (double)s_nn;
     }
@@ -84,21 +86,23 @@ __global__ __launch_bounds__(NT) void k_select(RateArgs<R> a, const double *__re
     if (!ok && threadIdx.x == 0) reinterpret_cast<int *>(res + RES_KIND)[0] = KIND_FAIL;
 }
 
-// athermal step of the fused sequence: list offsets from k_group, host-drawn redraws
-// for the first `cap` voxels; the last workgroup sums the |DWork| partials in order
+// athermal step of the fused sequence: list offsets from k_decide, host-drawn redraws
+// for the first `cap` voxels, |DWork| partials per workgroup
 template <typename R, int DIM>
-__global__ __launch_bounds__(NT) void k_athermal(AthArgs<R> a, double *res, long long nblk)
+__global__ __launch_bounds__(NT) void k_athermal(AthArgs<R> a, const double *res)
 {
-    __shared__ int is_last;
-    __shared__ double red[NT];
     const int kind = reinterpret_cast<const int *>(res + RES_KIND)[0];
     if (kind != KIND_ATHERMAL && kind != KIND_ATH_OVERFLOW) return;
     athermal_block<R, DIM>(a, blockIdx.x);
-    if (!last_arrival(reinterpret_cast<unsigned int *>(res + RES_TICKET), (unsigned int)nblk, &is_last)) return;
-    const long long n = (long long)res[13];
-    const double s = ordered_sum(a.socp, nblk, red);
+}
+
+// fixed-order sum of the |DWork| partials (STZ/evolution.c:66-68)
+__global__ __launch_bounds__(STZ_NT_MEAN) void k_socsum(const double *__restrict__ socp, long long nblk, double *res)
+{
+    const int kind = reinterpret_cast<const int *>(res + RES_KIND)[0];
+    if (kind != KIND_ATHERMAL && kind != KIND_ATH_OVERFLOW) return;
+    const double s = ordered_sum<STZ_NT_MEAN>(socp, nblk);
     if (threadIdx.x == 0) res[14] = s;
-    (void)n;
 }
 
 template <typename R, int DIM>
@@ -301,8 +305,10 @@ int rates_impl(stz_ctx *ctx, int decide)
     STZ_CHECK_LAUNCH();
     tmark(ctx, 9);
     const long long nb = (long long)ctx->M * ctx->nblk;
-    hipLaunchKernelGGL(k_group, dim3((unsigned)ctx->ngrp), dim3(NT), 0, ctx->stream, ctx->bsum, nb, ctx->gsum, ctx->ngrp,
-                       ctx->res, decide, ctx->k.dt, ctx->dspec_dev, ctx->acnt, ctx->aoff, ctx->nblk, ctx->acap);
+    const unsigned gb = (unsigned)((ctx->ngrp + STZ_GRP_PER_BLOCK - 1) / STZ_GRP_PER_BLOCK);
+    hipLaunchKernelGGL(k_group, dim3(gb), dim3(NT), 0, ctx->stream, ctx->bsum, nb, ctx->gsum, ctx->ngrp);
+    hipLaunchKernelGGL(k_decide, dim3(1), dim3(NT), 0, ctx->stream, ctx->gsum, ctx->ngrp, ctx->res, decide, ctx->k.dt,
+                       ctx->dspec_dev, ctx->acnt, ctx->aoff, ctx->nblk, ctx->acap);
     STZ_CHECK_LAUNCH();
     tmark(ctx, 10);
     return STZ_OK;
@@ -390,7 +396,8 @@ int athermal_fused_impl(stz_ctx *ctx, int gated)
     a.socp = ctx->socp;
     (void)gated;
     tmark(ctx, 15);
-    hipLaunchKernelGGL((k_athermal<R, DIM>), dim3((unsigned)ctx->nblk), dim3(NT), 0, ctx->stream, a, ctx->res, ctx->nblk);
+    hipLaunchKernelGGL((k_athermal<R, DIM>), dim3((unsigned)ctx->nblk), dim3(NT), 0, ctx->stream, a, ctx->res);
+    hipLaunchKernelGGL(k_socsum, dim3(1), dim3(STZ_NT_MEAN), 0, ctx->stream, ctx->socp, ctx->nblk, ctx->res);
     STZ_CHECK_LAUNCH();
     tmark(ctx, 16);
     return STZ_OK;

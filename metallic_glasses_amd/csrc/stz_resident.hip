@@ -6,20 +6,23 @@
 // Per step and shape class the stream holds ten launches, every one batched over the
 // replicas of the class (grid y/z = replica) and gated on the device:
 //   kb_rates      rate pass (shared code with k_rates)
-//   kb_group      group sums; in the last workgroup of each replica: R, the Q<0 count,
-//                 the step kind, the residence-time selection and the transform
-//                 (thermal), or the avalanche list offsets (athermal)
+//   kb_group      group sums of the block rate sums
+//   kb_decide     one workgroup per replica: R, the Q<0 count, the step kind, the
+//                 residence-time selection and the transform (thermal), or the
+//                 avalanche list offsets (athermal)
 //   kb_athermal   athermal compaction + transforms
-//   5 solve kernels + k_mean (stz_solve.hip, batched)
-//   kb_finish     clock, loading (STZ/evolution.c:262-358), athermal correction
-//                 (STZ/evolution.c:91-97), print-step and termination tests
-//                 (STZ/bmg.c:181-201, 382-410), the step record
+//   5 solve kernels (stz_solve.hip, batched)
+//   kb_finish     <sigma>, the |DWork| sum, clock, loading (STZ/evolution.c:262-358),
+//                 athermal correction (STZ/evolution.c:91-97), print-step and
+//                 termination tests (STZ/bmg.c:181-201, 382-410), the step record
+// No kernel waits on another workgroup: every cross-workgroup step is a kernel boundary.
 // The random numbers are the reference's own MT19937 stream, taken by index from a
 // lookup window built on the host (StzMtWindow), so a resident run draws exactly what
 // the per-call path and the reference draw.  A replica halts (every later kernel of
 // it exits at once) on termination, on a print step (the host re-solves and writes
 // output), when the window runs short, or on error; the host services it between
-// batches.  The step is captured once into a hipGraph and replayed.
+// batches.  The step sequence is launched directly or replayed from a hipGraph that
+// holds a fixed number of consecutive steps.
 #include "stz_kmc_dev.h"
 
 #include <algorithm>
@@ -100,27 +103,33 @@ __global__ __launch_bounds__(NT) void kb_rates(const RepDev *__restrict__ reps, 
     rates_block<R, DIM>(a, blockIdx.x);
 }
 
+__global__ __launch_bounds__(NT) void kb_group(const RepDev *__restrict__ reps, long long nb, long long ngrp)
+{
+    __shared__ double ws[NW];
+    const RepDev &d = reps[blockIdx.y];
+    if (d.st->halt) return;
+    group_sums(d.bsum, nb, d.gsum, ngrp, (long long)blockIdx.x * STZ_GRP_PER_BLOCK, ws);
+}
+
+// One workgroup per replica: R, the Q<0 count, the step kind, and the stream positions
+// of the step's random numbers; then the residence-time selection and transform of a
+// thermal event, or the list offsets of an athermal avalanche.  A step whose draws are
+// not all inside the lookup window is marked KIND_STALL (nothing changes).
 template <typename R, int DIM>
-__global__ __launch_bounds__(NT) void kb_group(const RepDev *__restrict__ reps, long long N, long long nblk, long long ngrp,
-                                               int M)
+__global__ __launch_bounds__(NT) void kb_decide(const RepDev *__restrict__ reps, long long N, long long nblk,
+                                                long long ngrp, int M)
 {
     constexpr int S = Comp<DIM>::S;
     __shared__ double ws[NW];
-    __shared__ int is_last;
     __shared__ int s_kind;
-    __shared__ unsigned long long s_nn;
     __shared__ double s_tau;
     __shared__ long long s_k0;
     __shared__ int s_par;
-    const RepDev &d = reps[blockIdx.y];
+    const RepDev &d = reps[blockIdx.x];
     RepState *st = d.st;
     if (st->halt) return;
     double *res = d.res;
     const int tid = threadIdx.x;
-    const long long nb = (long long)M * nblk;
-    const double t = group_sum(d.bsum, nb, (long long)blockIdx.x * STZ_GRP, ws);
-    if (tid == 0) d.gsum[blockIdx.x] = t;
-    if (!last_arrival(reinterpret_cast<unsigned int *>(res + RES_ARRIVE), (unsigned int)ngrp, &is_last)) return;
     const double Rtot = group_total(d.gsum, ngrp, ws);
     // rate args for the selection: the pending stress increment is in the field now
     RateArgs<R> a = rate_args_of<R>(d, st, N, nblk, M);
@@ -128,12 +137,13 @@ __global__ __launch_bounds__(NT) void kb_group(const RepDev *__restrict__ reps, 
     a.pend_delta = 0.0;
     const MtDev &mt = d.mt;
     const long long MS = (long long)M * S;
+    __syncthreads();
     if (tid == 0) {
         unsigned long long *cnt = reinterpret_cast<unsigned long long *>(res + RES_NNEG);
-        s_nn = atomicAdd(cnt, 0ull);
-        atomicExch(cnt, 0ull);
+        const unsigned long long nn = *cnt;
+        *cnt = 0ull;
         res[8] = Rtot;
-        res[9] = (double)s_nn;
+        res[9] = (double)nn;
         st->pend_comp = -1;          // consumed by the rate pass
         st->pend_delta = 0.0;
         reinterpret_cast<int *>(res + 15)[0] = 0;
@@ -141,7 +151,7 @@ __global__ __launch_bounds__(NT) void kb_group(const RepDev *__restrict__ reps, 
         res[13] = 0.0;
         const long long rpos = st->rpos;
         int kind;
-        if (s_nn > 0) {
+        if (nn > 0) {
             kind = KIND_ATHERMAL;                        // STZ/bmg.c:161
         } else if (1 / Rtot < d.dt) {                    // STZ/evolution.c:134-136
             kind = KIND_THERMAL;
@@ -199,14 +209,11 @@ __global__ __launch_bounds__(NT) void kb_group(const RepDev *__restrict__ reps, 
 }
 
 template <typename R, int DIM>
-__global__ __launch_bounds__(NT) void kb_athermal(const RepDev *__restrict__ reps, long long N, long long nblk, int M)
+__global__ __launch_bounds__(NT) void kb_athermal(const RepDev *__restrict__ reps, long long N, int M)
 {
-    constexpr int S = Comp<DIM>::S;
-    __shared__ int is_last;
-    __shared__ double red[NT];
     const RepDev &d = reps[blockIdx.y];
     const RepState *st = d.st;
-    double *res = d.res;
+    const double *res = d.res;
     if (st->halt || reinterpret_cast<const int *>(res + RES_KIND)[0] != KIND_ATHERMAL) return;
     AthArgs<R> a;
     a.ap = apply_args_of(d, st, N, M);
@@ -225,10 +232,6 @@ __global__ __launch_bounds__(NT) void kb_athermal(const RepDev *__restrict__ rep
     a.N = N;
     a.socp = d.socp;
     athermal_block<R, DIM>(a, blockIdx.x);
-    (void)S;
-    if (!last_arrival(reinterpret_cast<unsigned int *>(res + RES_TICKET), (unsigned int)nblk, &is_last)) return;
-    const double s = ordered_sum(d.socp, nblk, red);
-    if (threadIdx.x == 0) res[14] = s;
 }
 
 // --- host-loop arithmetic of the reference, on the device, with the host's operation order
@@ -337,107 +340,122 @@ __device__ bool terminated_d(const RepDev &d, const RepState *s)
     return false;
 }
 
-// the host loop's bookkeeping for one replica, after the device decided and ran its step
+// tells the host (polling mapped memory while it launches) that the replica halted
+__device__ __forceinline__ void publish_halt(const RepDev &d, int h)
+{
+    __hip_atomic_store(d.hflag, h, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+// the host loop's bookkeeping for one replica, after the device decided and ran its step:
+// <sigma> from the solve partials and the |DWork| sum (all threads, the per-call path's
+// reductions), then the scalar updates on a private copy of the run state (one thread)
 template <int DIM>
-__global__ void kb_finish(const RepDev *__restrict__ reps, int fp64)
+__global__ __launch_bounds__(STZ_NT_MEAN) void kb_finish(const RepDev *__restrict__ reps, int fp64, int nparts,
+                                                         double invN, long long nblk)
 {
     constexpr int C = Comp<DIM>::C;
-    if (threadIdx.x != 0) return;
+    __shared__ double mean[C];
     const RepDev &d = reps[blockIdx.x];
-    RepState *s = d.st;
-    if (s->halt) return;
+    RepState *sp = d.st;
+    if (sp->halt) return;
     double *res = d.res;
     int *kindp = reinterpret_cast<int *>(res + RES_KIND);
     const int kind = kindp[0];
+    const bool solved = kind == KIND_THERMAL || kind == KIND_ATHERMAL;
+    if (solved) mean_partials<C>(d.spart, nparts, invN, mean);
+    double soc = 0.0;
+    if (kind == KIND_ATHERMAL) soc = ordered_sum<STZ_NT_MEAN>(d.socp, nblk);
+    if (threadIdx.x != 0) return;
+    RepState s = *sp;
     if (kind == KIND_STALL) {
-        s->halt = HALT_RNG;
+        sp->halt = HALT_RNG;
+        publish_halt(d, HALT_RNG);
         kindp[0] = KIND_NONE;
         return;
     }
-    if (kind == KIND_FAIL) {
-        s->halt = HALT_ERR;
-        s->err = STZ_E_SELECT;
+    if (kind == KIND_FAIL || (kind != KIND_THERMAL && kind != KIND_ATHERMAL && kind != KIND_ELASTIC)) {
+        sp->halt = HALT_ERR;
+        sp->err = kind == KIND_FAIL ? STZ_E_SELECT : STZ_E_HIP;
+        publish_halt(d, HALT_ERR);
         kindp[0] = KIND_NONE;
         return;
     }
-    if (kind != KIND_THERMAL && kind != KIND_ATHERMAL && kind != KIND_ELASTIC) {
-        s->halt = HALT_ERR;
-        s->err = STZ_E_HIP;
-        return;
-    }
-    if ((kind == KIND_THERMAL || kind == KIND_ATHERMAL) && reinterpret_cast<const int *>(res + 15)[0]) {
-        s->halt = HALT_ERR;
-        s->err = STZ_E_NAN;
+    if (solved && reinterpret_cast<const int *>(res + 15)[0]) {
+        sp->halt = HALT_ERR;
+        sp->err = STZ_E_NAN;
+        publish_halt(d, HALT_ERR);
         kindp[0] = KIND_NONE;
         return;
     }
-    stz_step_result *r = d.rec + s->nrec;
-    r->kind = STZ_STEP_ELASTIC;
-    r->site = r->mode = -1;
-    r->n_transformed = 0;
-    r->q_chosen = 0.0;
-    r->dt_kmc = 0.0;
-    r->printed = 0;
-    r->soc_elast = 0.0;
-    s->step++;
     const double R = res[8];
-    r->n_negative = (long long)res[9];
-    r->total_rate = R;
-    if (kind == KIND_THERMAL || kind == KIND_ATHERMAL) {
-        s->nsolve++;
-        for (int c = 0; c < C; ++c) s->stress[c] = res[c];
-        for (int c = 0; c < 6; ++c) s->soff[c] = 0.0;        // the solve rewrote the stress field
-        s->rpos = reinterpret_cast<const long long *>(res + RES_RPOS)[0];
+    stz_step_result r;
+    r.kind = STZ_STEP_ELASTIC;
+    r.site = r.mode = -1;
+    r.n_transformed = 0;
+    r.n_negative = (long long)res[9];
+    r.q_chosen = 0.0;
+    r.total_rate = R;
+    r.dt_kmc = 0.0;
+    r.printed = 0;
+    r.soc_elast = 0.0;
+    s.step++;
+    if (solved) {
+        s.nsolve++;
+        for (int c = 0; c < C; ++c) s.stress[c] = mean[c];
+        for (int c = 0; c < 6; ++c) s.soff[c] = 0.0;          // the solve rewrote the stress field
+        s.rpos = reinterpret_cast<const long long *>(res + RES_RPOS)[0];
     }
     if (kind == KIND_THERMAL) {
-        s->avlan = 0;                         // UpdateSOC (STZ/evolution.c:5-45)
-        s->soc_elast = 0.0;
+        s.avlan = 0;                          // UpdateSOC (STZ/evolution.c:5-45)
+        s.soc_elast = 0.0;
         const double t = 1 / R;               // STZ/evolution.c:134
-        r->dt_kmc = t;
-        s->kstep++;
-        s->ntrans++;
-        r->kind = STZ_STEP_THERMAL;
-        r->site = (int)res[10];
-        r->mode = (int)res[11];
-        r->q_chosen = res[12];
-        load_thermal_d(d, s, C, fp64 != 0, t);
-        s->totime += t;                       // STZ/evolution.c:317
+        r.dt_kmc = t;
+        s.kstep++;
+        s.ntrans++;
+        r.kind = STZ_STEP_THERMAL;
+        r.site = (int)res[10];
+        r.mode = (int)res[11];
+        r.q_chosen = res[12];
+        load_thermal_d(d, &s, C, fp64 != 0, t);
+        s.totime += t;                        // STZ/evolution.c:317
     } else if (kind == KIND_ATHERMAL) {
-        s->astep++;                           // athermal_plasticity (STZ/evolution.c:47-100)
+        s.astep++;                            // athermal_plasticity (STZ/evolution.c:47-100)
         const long long n = (long long)res[13];
-        const double soc = res[14];
-        s->ntrans += n;
-        s->avlan += n;
-        s->soc_elast += soc;
-        athermal_correction_d(d, s, C);
-        r->kind = STZ_STEP_ATHERMAL;
-        r->n_transformed = (int)n;
-        r->soc_elast = soc;
+        s.ntrans += n;
+        s.avlan += n;
+        s.soc_elast += soc;
+        athermal_correction_d(d, &s, C);
+        r.kind = STZ_STEP_ATHERMAL;
+        r.n_transformed = (int)n;
+        r.soc_elast = soc;
     } else {
-        s->avlan = 0;
-        s->soc_elast = 0.0;
-        r->dt_kmc = 1 / R;
-        s->estep++;
-        s->totime += d.dt;                    // STZ/evolution.c:324
-        load_elastic_d(d, s, C, fp64 != 0);
+        s.avlan = 0;
+        s.soc_elast = 0.0;
+        r.dt_kmc = 1 / R;
+        s.estep++;
+        s.totime += d.dt;                     // STZ/evolution.c:324
+        load_elastic_d(d, &s, C, fp64 != 0);
     }
-    if (s->printperiod > 0) {
-        const double cur = s->epsavg[d.lc];
-        if (fabs(cur - s->strainhist_last) >= (s->printperiod * d.strainincrement * 0.99)) {
-            s->strainhist_last = cur;         // strainhist[N_print-1], STZ/bmg.c:193-201
-            r->printed = 1;
-            s->halt = HALT_PRINT;
-        } else if (terminated_d(d, s)) {
-            s->halt = HALT_DONE;
+    if (s.printperiod > 0) {
+        const double cur = s.epsavg[d.lc];
+        if (fabs(cur - s.strainhist_last) >= (s.printperiod * d.strainincrement * 0.99)) {
+            s.strainhist_last = cur;          // strainhist[N_print-1], STZ/bmg.c:193-201
+            r.printed = 1;
+            s.halt = HALT_PRINT;
+        } else if (terminated_d(d, &s)) {
+            s.halt = HALT_DONE;
         }
     }
-    r->totime = s->totime;
+    r.totime = s.totime;
     for (int c = 0; c < 6; ++c) {
-        r->epsavg[c] = s->epsavg[c];
-        r->stress[c] = s->stress[c];
+        r.epsavg[c] = s.epsavg[c];
+        r.stress[c] = s.stress[c];
     }
-    s->nrec++;
-    if (s->halt == HALT_NONE && s->nrec >= s->budget) s->halt = HALT_BUDGET;
+    d.rec[s.nrec] = r;
+    s.nrec++;
+    if (s.halt == HALT_NONE && s.nrec >= s.budget) s.halt = HALT_BUDGET;
+    *sp = s;
+    if (s.halt != HALT_NONE) publish_halt(d, s.halt);
     kindp[0] = KIND_NONE;
 }
 #pragma clang fp contract(on)
@@ -457,7 +475,7 @@ struct StzResident {
     int *rank[2] = {nullptr, nullptr};
     int *pair[2] = {nullptr, nullptr};
     long long wcap = 0;               // words the device arrays hold
-    long long wsize = 1ll << 22;      // words per window
+    long long wsize = 1ll << 21;      // words per window
     StzMtWindow win;
     bool win_ok = false;
     long long win_epoch = -1;
@@ -468,9 +486,10 @@ struct StzResident {
     RepState hst;                     // host copy of the run state
     std::vector<stz_step_result> hrec;
     int batch = 64;
-    int use_graph = 1;
     int stop_at_print = 0;
     int threads = 8;
+    int *hflag = nullptr;             // mapped pinned halt flag (host address)
+    int *hflag_dev = nullptr;         // its device address
     struct StzPlan *self = nullptr;   // single-context plan
 };
 
@@ -492,9 +511,11 @@ struct StzPlan {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     int batch = 64;
-    int use_graph = 1;
+    int use_graph = 0;                // steps per captured graph; 0: direct launches
+    int graph_steps = 0;              // steps in the instantiated graph
     // rate-pass timing of direct launches (single context with timing on)
     std::vector<hipEvent_t> eva, evb;
+    std::vector<hipEvent_t> evc;      // progress markers of the chunked launches
     int timed = 0;
 };
 
@@ -535,6 +556,9 @@ int ensure_resident(stz_ctx *ctx)
     rs->threads = (int)std::max(1u, std::min(hw, 16u));
     int rc = rs_alloc(ctx, (void **)&rs->st, sizeof(RepState));
     if (rc) return rc;
+    RS_HIP(ctx, hipHostMalloc((void **)&rs->hflag, sizeof(int), hipHostMallocCoherent | hipHostMallocMapped));
+    RS_HIP(ctx, hipHostGetDevicePointer((void **)&rs->hflag_dev, rs->hflag, 0));
+    *rs->hflag = 0;
     rs->rec_cap = rs->batch;
     rc = rs_alloc(ctx, (void **)&rs->rec, sizeof(stz_step_result) * (size_t)rs->rec_cap);
     if (rc) return rc;
@@ -611,6 +635,7 @@ void fill_desc(stz_ctx *ctx)
     d.alist = ctx->alist;
     d.st = rs->st;
     d.rec = rs->rec;
+    d.hflag = rs->hflag_dev;
     const StzConsts &k = ctx->k;
     d.kT = k.kT;
     d.tTemp = k.tTemp;
@@ -687,6 +712,7 @@ int push_state(stz_ctx *ctx, long long budget)
     s.halt = budget > 0 ? HALT_NONE : HALT_BUDGET;
     s.nrec = 0;
     RS_HIP(ctx, hipMemcpyAsync(rs->st, &s, sizeof(s), hipMemcpyHostToDevice, ctx->stream));
+    __atomic_store_n(rs->hflag, s.halt, __ATOMIC_RELAXED);
     const int none = KIND_NONE;
     RS_HIP(ctx, hipMemcpyAsync(ctx->res + RES_KIND, &none, sizeof(int), hipMemcpyHostToDevice, ctx->stream));
     fill_desc(ctx);
@@ -734,16 +760,19 @@ int launch_class(StzPlan *pl, StzClass &k, hipEvent_t ea, hipEvent_t eb)
     if (ea) (void)hipEventRecord(ea, s);
     hipLaunchKernelGGL((kb_rates<R, DIM>), dim3((unsigned)p->nblk, nrep), dim3(NT), 0, s, k.d_reps, p->N, p->nblk, p->M);
     if (eb) (void)hipEventRecord(eb, s);
-    hipLaunchKernelGGL((kb_group<R, DIM>), dim3((unsigned)p->ngrp, nrep), dim3(NT), 0, s, k.d_reps, p->N, p->nblk,
-                       p->ngrp, p->M);
-    hipLaunchKernelGGL((kb_athermal<R, DIM>), dim3((unsigned)p->nblk, nrep), dim3(NT), 0, s, k.d_reps, p->N, p->nblk, p->M);
+    const unsigned gb = (unsigned)((p->ngrp + STZ_GRP_PER_BLOCK - 1) / STZ_GRP_PER_BLOCK);
+    hipLaunchKernelGGL(kb_group, dim3(gb, nrep), dim3(NT), 0, s, k.d_reps, (long long)p->M * p->nblk, p->ngrp);
+    hipLaunchKernelGGL((kb_decide<R, DIM>), dim3(nrep), dim3(NT), 0, s, k.d_reps, p->N, p->nblk, p->ngrp, p->M);
+    hipLaunchKernelGGL((kb_athermal<R, DIM>), dim3((unsigned)p->nblk, nrep), dim3(NT), 0, s, k.d_reps, p->N, p->M);
     if (hipGetLastError() != hipSuccess) return STZ_E_HIP;
     const int saved = p->timing;
     p->timing = 0;
-    int rc = stz_solve_launch_batched(p, nrep, k.d_solve, s);
+    int nparts = 0;
+    int rc = stz_solve_launch_batched(p, nrep, k.d_solve, s, &nparts);
     p->timing = saved;
     if (rc) return rc;
-    hipLaunchKernelGGL((kb_finish<DIM>), dim3(nrep), dim3(64), 0, s, k.d_reps, (int)(p->prec == STZ_FP64));
+    hipLaunchKernelGGL((kb_finish<DIM>), dim3(nrep), dim3(STZ_NT_MEAN), 0, s, k.d_reps, (int)(p->prec == STZ_FP64), nparts,
+                       1.0 / (double)p->N, p->nblk);
     return hipGetLastError() == hipSuccess ? STZ_OK : STZ_E_HIP;
 }
 
@@ -765,6 +794,7 @@ void plan_free(StzPlan *pl)
 {
     if (!pl) return;
     for (hipEvent_t e : pl->eva) (void)hipEventDestroy(e);
+    for (hipEvent_t e : pl->evc) (void)hipEventDestroy(e);
     for (hipEvent_t e : pl->evb) (void)hipEventDestroy(e);
     if (pl->exec) (void)hipGraphExecDestroy(pl->exec);
     if (pl->graph) (void)hipGraphDestroy(pl->graph);
@@ -835,12 +865,18 @@ int plan_upload(StzPlan *pl)
     return STZ_OK;
 }
 
+// captures use_graph consecutive steps into one graph
 int plan_graph(StzPlan *pl)
 {
-    if (pl->exec) return STZ_OK;
+    if (pl->exec && pl->graph_steps == pl->use_graph) return STZ_OK;
+    if (pl->exec) (void)hipGraphExecDestroy(pl->exec);
+    if (pl->graph) (void)hipGraphDestroy(pl->graph);
+    pl->exec = nullptr;
+    pl->graph = nullptr;
     stz_ctx *o = pl->owner;
     RS_HIP(o, hipStreamBeginCapture(pl->stream, hipStreamCaptureModeThreadLocal));
-    int rc = launch_step(pl);
+    int rc = STZ_OK;
+    for (int i = 0; i < pl->use_graph && !rc; ++i) rc = launch_step(pl);
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(pl->stream, &g);
     if (rc) {
@@ -853,20 +889,33 @@ int plan_graph(StzPlan *pl)
     }
     pl->graph = g;
     RS_HIP(o, hipGraphInstantiate(&pl->exec, g, nullptr, nullptr, 0));
+    pl->graph_steps = pl->use_graph;
     return STZ_OK;
 }
 
+// true when every replica of the plan has halted (the device publishes halts to mapped
+// host memory, so the host stops enqueueing steps that would only run as no-ops)
+bool all_halted(const StzPlan *pl)
+{
+    for (const stz_ctx *c : pl->all)
+        if (__atomic_load_n(c->rs->hflag, __ATOMIC_RELAXED) == HALT_NONE) return false;
+    return true;
+}
+
+// Enqueues up to k steps in chunks, keeping at most `depth` steps ahead of the GPU,
+// and stops once every replica has halted.  Steps beyond a replica's budget or after
+// its halt run as no-ops.
 int plan_launch_steps(StzPlan *pl, int k)
 {
     stz_ctx *o = pl->owner;
-    if (pl->use_graph) {
+    const int chunk = pl->use_graph > 0 ? pl->use_graph : 2;
+    const int depth = std::max(3 * chunk, 6);
+    if (pl->use_graph > 0) {
         int rc = plan_graph(pl);
         if (rc) return rc;
-        for (int i = 0; i < k; ++i) RS_HIP(o, hipGraphLaunch(pl->exec, pl->stream));
-        return STZ_OK;
     }
     stz_ctx *p = pl->all[0];
-    const bool timed = pl->all.size() == 1 && p->timing;
+    const bool timed = pl->use_graph == 0 && pl->all.size() == 1 && p->timing;
     if (timed && pl->eva.size() < (size_t)k) {
         while (pl->eva.size() < (size_t)k) {
             hipEvent_t a, b;
@@ -876,11 +925,33 @@ int plan_launch_steps(StzPlan *pl, int k)
             pl->evb.push_back(b);
         }
     }
-    for (int i = 0; i < k; ++i) {
-        int rc = launch_step(pl, timed ? pl->eva[(size_t)i] : nullptr, timed ? pl->evb[(size_t)i] : nullptr);
-        if (rc) return rc;
+    const int nchunks = (k + chunk - 1) / chunk;
+    while ((int)pl->evc.size() < nchunks) {
+        hipEvent_t e;
+        RS_HIP(o, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        pl->evc.push_back(e);
     }
-    pl->timed = timed ? k : 0;
+    int launched = 0, waited = 0, nc = 0;
+    while (launched < k) {
+        const int c = std::min(chunk, k - launched);
+        if (pl->use_graph > 0) {
+            RS_HIP(o, hipGraphLaunch(pl->exec, pl->stream));
+        } else {
+            for (int i = 0; i < c; ++i) {
+                const int j = launched + i;
+                int rc = launch_step(pl, timed ? pl->eva[(size_t)j] : nullptr, timed ? pl->evb[(size_t)j] : nullptr);
+                if (rc) return rc;
+            }
+        }
+        launched += c;
+        RS_HIP(o, hipEventRecord(pl->evc[(size_t)nc++], pl->stream));
+        while ((launched - waited) > depth) {
+            RS_HIP(o, hipEventSynchronize(pl->evc[(size_t)(waited / chunk)]));
+            waited += chunk;
+        }
+        if (all_halted(pl)) break;
+    }
+    pl->timed = timed ? launched : 0;
     return STZ_OK;
 }
 
@@ -911,6 +982,7 @@ void stz_resident_free(stz_ctx *ctx)
     free_window(rs);
     if (rs->st) (void)hipFree(rs->st);
     if (rs->rec) (void)hipFree(rs->rec);
+    if (rs->hflag) (void)hipHostFree(rs->hflag);
     delete rs;
     ctx->rs = nullptr;
 }
@@ -1102,7 +1174,7 @@ extern "C" {
 
 int stz_run_config(stz_ctx *ctx, int batch, int use_graph, long long window_words)
 {
-    if (!ctx || batch < 0 || batch > 65536 || window_words < 0) return STZ_E_ARG;
+    if (!ctx || batch < 0 || batch > 65536 || window_words < 0 || use_graph > 1024) return STZ_E_ARG;
     StzPlan *pl = stz_self_plan(ctx);
     if (!pl) return STZ_E_HIP;
     if (batch > 0) {
@@ -1110,7 +1182,7 @@ int stz_run_config(stz_ctx *ctx, int batch, int use_graph, long long window_word
         if (rc) return rc;
         pl->batch = batch;
     }
-    if (use_graph >= 0) pl->use_graph = use_graph != 0;
+    if (use_graph >= 0) pl->use_graph = use_graph;
     if (window_words > 0) {
         if (window_words < 4096 || window_words > (1ll << 28)) return STZ_E_ARG;
         ctx->rs->wsize = window_words;
@@ -1169,7 +1241,7 @@ int stz_ensemble_create(stz_ctx *const *members, int n, stz_ensemble **out)
 
 int stz_ensemble_config(stz_ensemble *e, int batch, int use_graph)
 {
-    if (!e || batch < 0 || batch > 65536) return STZ_E_ARG;
+    if (!e || batch < 0 || batch > 65536 || use_graph > 1024) return STZ_E_ARG;
     if (batch > 0) {
         for (stz_ctx *c : e->pl->all) {
             int rc = set_batch(c, batch);
@@ -1177,7 +1249,7 @@ int stz_ensemble_config(stz_ensemble *e, int batch, int use_graph)
         }
         e->pl->batch = batch;
     }
-    if (use_graph >= 0) e->pl->use_graph = use_graph != 0;
+    if (use_graph >= 0) e->pl->use_graph = use_graph;
     return STZ_OK;
 }
 

@@ -14,7 +14,8 @@
 //   k_ypass   inverse along y (3-D only)
 //   k_inv_x   C2R along x + epilogue sigma = C0:(eps/N + epsavg - eps0_rt), block
 //             partial sums of sigma (FP64)
-//   k_mean    fixed-order reduction of the partials -> <sigma>
+//   k_mean    fixed-order reduction of the partials -> <sigma> (the resident step does
+//             it in its bookkeeping kernel with the same code)
 //
 // Half-spectrum equivalence: the reference transforms the full complex spectrum and
 // keeps Re(IFFT) (STZ/elastic.c:811-824).  Its k vectors use round() (STZ/init.c:
@@ -311,50 +312,41 @@ __global__ __launch_bounds__(NT_SOLVE) void k_ypass(cpx<R> *__restrict__ spec_a,
 
 // ---------------------------------------------------------------------------------
 // k_fused: last axis forward FFT, Gamma-hat, inverse FFT for one tile of W k0-columns
-// (3-D: at fixed y index blockIdx.y; 2-D: the y axis itself is the last axis)
+// (3-D: at fixed y index blockIdx.y; 2-D: the y axis itself is the last axis).  The
+// tile of all C components (LZ x C*W complex values) stays in LDS from load to store:
+// the C*W lines are transformed together, Gamma-hat mixes the C components of each of
+// the LZ*W wavevectors in place, and the inverse transform follows.
 // ---------------------------------------------------------------------------------
-template <typename R, int LZ, int W, int C, int DIM>
-__global__ __launch_bounds__(NT_SOLVE) void k_fused(cpx<R> *__restrict__ spec_a, const cpx<R> *__restrict__ twg,
-                                                    SolveConsts k, const SolveRep *reps)
+template <typename R, int LZ, int W, int C, int DIM, int NTF>
+__global__ __launch_bounds__(NTF) void k_fused(cpx<R> *__restrict__ spec_a, const cpx<R> *__restrict__ twg,
+                                               SolveConsts k, const SolveRep *reps)
 {
-    constexpr int NE = LZ * W;
+    constexpr int NL = C * W;             // lines, component-major: line = c*W + col
+    constexpr int NE = LZ * NL;
     const View<R> vw = view_of<R>(reps, blockIdx.z, k, nullptr, spec_a, nullptr, nullptr, nullptr);
     if (gated_off_p(vw.gate)) return;
     cpx<R> *__restrict__ spec = vw.spec;
-    constexpr int EPT = (NE + NT_SOLVE - 1) / NT_SOLVE;
     __shared__ cpx<R> buf[NE];
     __shared__ cpx<R> tw[LZ];
     const int tid = threadIdx.x;
-    for (int i = tid; i < LZ; i += NT_SOLVE) tw[i] = twg[i];
+    for (int i = tid; i < LZ; i += NTF) tw[i] = twg[i];
     const int k0base = blockIdx.x * W;
     const int i1 = (DIM == 3) ? blockIdx.y : 0;
     // element (p, col) of component c lives at spec[(c*rows + p*pstride + i1) * hp + k0]
     const long long pstride = (DIM == 3) ? k.n[1] : 1;
-    cpx<R> reg[C][EPT];
-    for (int c = 0; c < C; ++c) {
-        cpx<R> *base = spec + ((long long)c * k.rows + i1) * k.hp + k0base;
-        for (int e = tid; e < NE; e += NT_SOLVE) {
-            const int col = e % W, p = e / W;
-            cpx<R> val = {0, 0};
-            if (k0base + col < k.h0) val = base[(long long)p * pstride * k.hp + col];
-            buf[e] = val;
-        }
-        __syncthreads();
-        fft_lines<R, LZ, W, W, NT_SOLVE>(buf, tw, (R)-1);
-#pragma unroll
-        for (int j = 0; j < EPT; ++j) {
-            const int e = tid + j * NT_SOLVE;
-            if (NE % NT_SOLVE == 0 || e < NE) reg[c][j] = buf[e];
-        }
-        __syncthreads();
+    for (int e = tid; e < NE; e += NTF) {
+        const int line = e % NL, p = e / NL;
+        const int c = line / W, col = line % W;
+        cpx<R> val = {0, 0};
+        if (k0base + col < k.h0) val = spec[((long long)c * k.rows + (long long)p * pstride + i1) * k.hp + k0base + col];
+        buf[e] = val;
     }
-    // Gamma-hat on the elements this thread owns
+    __syncthreads();
+    fft_lines<R, LZ, NL, NL, NTF>(buf, tw, (R)-1);
+    // Gamma-hat on each wavevector of the tile
     const R lam = (R)k.lambda, mu = (R)k.mu, s0 = (R)k.s0, sh = (R)k.sh, sa = (R)k.sa;
-#pragma unroll
-    for (int j = 0; j < EPT; ++j) {
-        const int e = tid + j * NT_SOLVE;
-        if (!(NE % NT_SOLVE == 0 || e < NE)) continue;
-        const int col = e % W, p = e / W;
+    for (int kp = tid; kp < LZ * W; kp += NTF) {
+        const int col = kp % W, p = kp / W;
         int idx[3];
         idx[0] = k0base + col;
         if constexpr (DIM == 3) { idx[1] = i1; idx[2] = p; }
@@ -369,7 +361,11 @@ __global__ __launch_bounds__(NT_SOLVE) void k_fused(cpx<R> *__restrict__ spec_a,
         products<R, DIM>(g, gg);
         R er[C], ei[C], orr[C], oi[C];
 #pragma unroll
-        for (int c = 0; c < C; ++c) { er[c] = reg[c][j].x; ei[c] = reg[c][j].y; }
+        for (int c = 0; c < C; ++c) {
+            const cpx<R> x = buf[p * NL + c * W + col];
+            er[c] = x.x;
+            ei[c] = x.y;
+        }
         gamma_part<R, C>(er, g, gg, lam, mu, s0, sh, sa, orr);
         gamma_part<R, C>(ei, g, gg, lam, mu, s0, sh, sa, oi);
         if (nyq) {
@@ -390,22 +386,14 @@ __global__ __launch_bounds__(NT_SOLVE) void k_fused(cpx<R> *__restrict__ spec_a,
             }
         }
 #pragma unroll
-        for (int c = 0; c < C; ++c) reg[c][j] = {orr[c], oi[c]};
+        for (int c = 0; c < C; ++c) buf[p * NL + c * W + col] = {orr[c], oi[c]};
     }
-    for (int c = 0; c < C; ++c) {
-#pragma unroll
-        for (int j = 0; j < EPT; ++j) {
-            const int e = tid + j * NT_SOLVE;
-            if (NE % NT_SOLVE == 0 || e < NE) buf[e] = reg[c][j];
-        }
-        __syncthreads();
-        fft_lines<R, LZ, W, W, NT_SOLVE>(buf, tw, (R)1);
-        cpx<R> *base = spec + ((long long)c * k.rows + i1) * k.hp + k0base;
-        for (int e = tid; e < NE; e += NT_SOLVE) {
-            const int col = e % W, p = e / W;
-            if (k0base + col < k.h0) base[(long long)p * pstride * k.hp + col] = buf[e];
-        }
-        __syncthreads();
+    __syncthreads();
+    fft_lines<R, LZ, NL, NL, NTF>(buf, tw, (R)1);
+    for (int e = tid; e < NE; e += NTF) {
+        const int line = e % NL, p = e / NL;
+        const int c = line / W, col = line % W;
+        if (k0base + col < k.h0) spec[((long long)c * k.rows + (long long)p * pstride + i1) * k.hp + k0base + col] = buf[e];
     }
 }
 
@@ -495,36 +483,13 @@ __global__ __launch_bounds__(NT_SOLVE) void k_inv_x(cpx<R> *__restrict__ spec_a,
     }
 }
 
-constexpr int NT_MEAN = 1024;
+constexpr int NT_MEAN = STZ_NT_MEAN;
 template <int C>
 __global__ __launch_bounds__(NT_MEAN) void k_mean(const double *__restrict__ spart, int nblocks, double invN,
-                                                  double *__restrict__ out, const int *gate, const SolveRep *reps)
+                                                  double *__restrict__ out, const int *gate)
 {
-    if (reps) {
-        const SolveRep &q = reps[blockIdx.x];
-        spart = q.spart;
-        out = q.res;
-        gate = reinterpret_cast<const int *>(q.res + RES_KIND);
-    }
     if (gated_off_p(gate)) return;
-    __shared__ double red[C][NT_MEAN];
-    const int tid = threadIdx.x;
-    double a[C];
-#pragma unroll
-    for (int c = 0; c < C; ++c) a[c] = 0.0;
-    for (int b = tid; b < nblocks; b += NT_MEAN)
-#pragma unroll
-        for (int c = 0; c < C; ++c) a[c] += spart[(long long)b * C + c];
-#pragma unroll
-    for (int c = 0; c < C; ++c) red[c][tid] = a[c];
-    __syncthreads();
-    for (int w = NT_MEAN / 2; w > 0; w >>= 1) {
-        if (tid < w)
-#pragma unroll
-            for (int c = 0; c < C; ++c) red[c][tid] = red[c][tid] + red[c][tid + w];
-        __syncthreads();
-    }
-    if (tid < C) out[tid] = red[tid][0] * invN;
+    mean_partials<C>(spart, nblocks, invN, out);
 }
 
 // ---------------------------------------------------------------------------------
@@ -541,10 +506,27 @@ bool supported_len(int L)
 }
 
 // rows per workgroup for the x passes: about 1024 complex values per component group
-constexpr int rows_for(int L0) { return L0 >= 512 ? 2 : (L0 >= 256 ? 4 : (L0 >= 128 ? 8 : (L0 >= 64 ? 16 : 32))); }
+// rows per workgroup of the x passes: the largest power of two (2..32) whose LDS tile,
+// L0 x (C*NR/2 + 1) complex values, stays within 32 KB
+template <typename R, int C>
+constexpr int rows_for(int L0)
+{
+    int nr = 32;
+    while (nr > 2 && (long long)L0 * (C * nr / 2 + 1) * (long long)sizeof(cpx<R>) > 32768) nr /= 2;
+    return nr;
+}
 // k0 columns per tile for the strided passes
 constexpr int cols_for(int L) { return L >= 512 ? 4 : (L >= 256 ? 8 : 16); }
-constexpr int fused_cols_for(int L, int C) { return C == 6 ? (L >= 256 ? 4 : 8) : (L >= 512 ? 4 : (L >= 256 ? 8 : 16)); }
+// k0 columns of the fused tile: the largest power of two (2..16) keeping the
+// LZ x C*W tile within 96 KB of LDS
+template <typename R, int C>
+constexpr int fused_cols_for(int L)
+{
+    int w = 16;
+    while (w > 2 && (long long)L * C * w * (long long)sizeof(cpx<R>) > 98304) w /= 2;
+    return w;
+}
+constexpr int fused_threads(int elems) { return elems >= 4096 ? 1024 : (elems >= 2048 ? 512 : 256); }
 
 template <typename R>
 struct Launch {
@@ -566,7 +548,7 @@ struct Launch {
     template <int L0, int NRV, int C>
     void go_fwd_x(dim3 grid)
     {
-        if constexpr (NRV <= rows_for(L0))
+        if constexpr (NRV <= rows_for<R, C>(L0))
             hipLaunchKernelGGL((k_fwd_x<R, L0, NRV, C>), grid, dim3(NT_SOLVE), 0, stream, ctx->eps0,
                                (cpx<R> *)ctx->spec, (const cpx<R> *)ctx->tw.tw[0], k, reps);
     }
@@ -574,7 +556,7 @@ struct Launch {
     template <int L0, int NRV, int C>
     void go_inv_x(dim3 grid, double *spart)
     {
-        if constexpr (NRV <= rows_for(L0))
+        if constexpr (NRV <= rows_for<R, C>(L0))
             hipLaunchKernelGGL((k_inv_x<R, L0, NRV, C>), grid, dim3(NT_SOLVE), 0, stream, (cpx<R> *)ctx->spec,
                                ctx->eps0, (R *)ctx->sigma, spart, nanflag, (const cpx<R> *)ctx->tw.tw[0], k, reps);
     }
@@ -582,7 +564,7 @@ struct Launch {
     template <int L0, int C>
     int fwd_x_n()
     {
-        const int NR = pick_rows(rows_for(L0));
+        const int NR = pick_rows(rows_for<R, C>(L0));
         if (!NR) return STZ_E_ARG;
         const dim3 grid(k.rows / NR, nrep);
         switch (NR) {
@@ -599,7 +581,7 @@ struct Launch {
     template <int L0, int C>
     int inv_x_n(double *spart, int *nblocks)
     {
-        const int NR = pick_rows(rows_for(L0));
+        const int NR = pick_rows(rows_for<R, C>(L0));
         if (!NR) return STZ_E_ARG;
         *nblocks = k.rows / NR;
         const dim3 grid(k.rows / NR, nrep);
@@ -628,9 +610,10 @@ struct Launch {
     template <int LZ, int C, int DIM>
     int fused_n()
     {
-        constexpr int W = fused_cols_for(LZ, C);
+        constexpr int W = fused_cols_for<R, C>(LZ);
+        constexpr int NTF = fused_threads(LZ * C * W);
         const dim3 grid((k.h0 + W - 1) / W, DIM == 3 ? k.n[1] : 1, nrep);
-        hipLaunchKernelGGL((k_fused<R, LZ, W, C, DIM>), grid, dim3(NT_SOLVE), 0, stream, (cpx<R> *)ctx->spec,
+        hipLaunchKernelGGL((k_fused<R, LZ, W, C, DIM, NTF>), grid, dim3(NTF), 0, stream, (cpx<R> *)ctx->spec,
                            (const cpx<R> *)ctx->tw.tw[DIM - 1], k, reps);
         STZ_CHECK_LAUNCH();
         return STZ_OK;
@@ -652,7 +635,7 @@ struct Launch {
     }
 
 template <typename R>
-int solve_impl(stz_ctx *ctx, int gated, int nrep, const SolveRep *reps, hipStream_t stream)
+int solve_impl(stz_ctx *ctx, int gated, int nrep, const SolveRep *reps, hipStream_t stream, int *nparts)
 {
     Launch<R> Lr;
     Lr.ctx = ctx;
@@ -710,8 +693,12 @@ int solve_impl(stz_ctx *ctx, int gated, int nrep, const SolveRep *reps, hipStrea
     }
     if (rc) return rc;
     tmark(ctx, 5);
-    if (C == 3) hipLaunchKernelGGL((k_mean<3>), dim3(nrep), dim3(NT_MEAN), 0, stream, ctx->spart, nblocks, k.invN, ctx->res, k.gate, reps);
-    else hipLaunchKernelGGL((k_mean<6>), dim3(nrep), dim3(NT_MEAN), 0, stream, ctx->spart, nblocks, k.invN, ctx->res, k.gate, reps);
+    if (nparts) {           // batched: the caller reduces the partials
+        *nparts = nblocks;
+        return STZ_OK;
+    }
+    if (C == 3) hipLaunchKernelGGL((k_mean<3>), dim3(1), dim3(NT_MEAN), 0, stream, ctx->spart, nblocks, k.invN, ctx->res, k.gate);
+    else hipLaunchKernelGGL((k_mean<6>), dim3(1), dim3(NT_MEAN), 0, stream, ctx->spart, nblocks, k.invN, ctx->res, k.gate);
     STZ_CHECK_LAUNCH();
     tmark(ctx, 6);
     return STZ_OK;
@@ -762,14 +749,14 @@ void stz_fft_free(stz_ctx *ctx)
 
 int stz_solve_launch(stz_ctx *ctx, int gated)
 {
-    return ctx->prec == STZ_FP64 ? solve_impl<double>(ctx, gated, 1, nullptr, ctx->stream)
-                                 : solve_impl<float>(ctx, gated, 1, nullptr, ctx->stream);
+    return ctx->prec == STZ_FP64 ? solve_impl<double>(ctx, gated, 1, nullptr, ctx->stream, nullptr)
+                                 : solve_impl<float>(ctx, gated, 1, nullptr, ctx->stream, nullptr);
 }
 
-int stz_solve_launch_batched(stz_ctx *proto, int nrep, const SolveRep *reps, hipStream_t stream)
+int stz_solve_launch_batched(stz_ctx *proto, int nrep, const SolveRep *reps, hipStream_t stream, int *nparts)
 {
-    return proto->prec == STZ_FP64 ? solve_impl<double>(proto, 1, nrep, reps, stream)
-                                   : solve_impl<float>(proto, 1, nrep, reps, stream);
+    return proto->prec == STZ_FP64 ? solve_impl<double>(proto, 1, nrep, reps, stream, nparts)
+                                   : solve_impl<float>(proto, 1, nrep, reps, stream, nparts);
 }
 
 // rows of the stress-mean partial array: the largest block count of k_inv_x

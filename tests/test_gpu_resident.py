@@ -60,10 +60,11 @@ def _compare_fields(a, b):
 CASES = [
     # name, params, steps, batch, graph, window words
     ("3d16_graph", _kw3((16, 16, 16)), 600, 64, 1, 0),
+    ("3d16_graph8", _kw3((16, 16, 16)), 600, 60, 8, 0),
     ("3d16_direct_small_batches", _kw3((16, 16, 16)), 600, 7, 0, 0),
-    ("3d16_small_window", _kw3((16, 16, 16)), 600, 64, 1, 8192),
-    ("3d_16x8x32_mixed", _kw3((16, 8, 32), precision=mg.MIXED), 500, 32, 1, 0),
-    ("2d32_reference", dict(dim=2, n=(32, 32, 1), seed=123, loading=1), 1500, 64, 1, 0),
+    ("3d16_small_window", _kw3((16, 16, 16)), 600, 64, 0, 8192),
+    ("3d_16x8x32_mixed", _kw3((16, 8, 32), precision=mg.MIXED), 500, 32, 0, 0),
+    ("2d32_reference", dict(dim=2, n=(32, 32, 1), seed=123, loading=1), 1500, 64, 2, 0),
 ]
 
 
@@ -183,7 +184,7 @@ def test_ensemble_replicas_equal_single_runs(precision):
         m.set_print(5, 1.0)
         members.append(m)
         refs.append(_stepwise(kw, nsteps))
-    ens = mg.Ensemble(members, batch=32)
+    ens = mg.Ensemble(members, batch=32, use_graph=4 if precision == mg.MIXED else 0)
     rcs, outs = ens.run(nsteps)
     for i, (ref, rc_ref, recs_ref) in enumerate(refs):
         print("replica", i, ENSEMBLE[i], flush=True)
