@@ -116,6 +116,9 @@ def main():
     g = mg.Stz(**kw)
     g.set_print(5, 1.0)
     init_s = time.perf_counter() - t_init
+    tune64 = g.autotune(reps=10)
+    print(f"# autotuned solve tiles (x fwd rows, y cols, fused cols, x inv rows): {tune64[0]} "
+          f"ms {np.round(tune64[1], 4).tolist()}", file=sys.stderr, flush=True)
     print(f"# init {dims} fp64: {init_s:.1f}s", file=sys.stderr, flush=True)
 
     # warm-up: elastic loading until the first thermal events, then W events
@@ -188,6 +191,7 @@ def main():
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "bytes_per_launch": int(k4_bytes), "avg_ms": round(k4_ms, 5)},
         "solve_ms": round(solve_ms, 5),
+        "solve_tiles": tune64[0],
     }
     fft = fft_table(dims, C, N, 8, sms, sn)
     out["fft_roofline"] = fft
@@ -201,6 +205,7 @@ def main():
         ref.close()
         gm = mg.Stz(init_state=False, **kw_m)
         restore(gm, snap, mg)
+        tune_m = gm.autotune(reps=10)
         gm.stress_solve()
         smix = gm.download(mg.F_SIGMA)
         err = float(np.linalg.norm(smix - sref) / np.linalg.norm(sref))
@@ -244,7 +249,7 @@ def main():
                         "steps_identical": nsame, "ms_per_step": round(dtm * 1e3 / a.steps, 4),
                         "speedup_vs_fp64": round(float(kinds_m[1] / dtm) / ev_fp64, 3),
                         "k_rates_GBs": round(kb / (k4m * 1e-3) / 1e9, 1),
-                        "events_identical_to_fp64": same,
+                        "events_identical_to_fp64": same, "solve_tiles": tune_m[0],
                         "fft_roofline": fft_table(dims, C, N, 4, smsm, snm)}
         gm.close()
         print(f"# mixed: {out['mixed']}", file=sys.stderr, flush=True)
@@ -329,6 +334,7 @@ def ensemble_leg(a, mg):
     members = make_contexts(mg, kws)
     for m in members:
         m.set_print(5, 1.0)
+        m.autotune(reps=5)          # measured once per grid shape, cached for the rest
     init_s = time.perf_counter() - t0
     print(f"# ensemble: {len(members)} replicas initialised in {init_s:.1f}s", file=sys.stderr, flush=True)
     # FP64 twins of one replica per grid, stepped alone, for the precision comparison
@@ -336,6 +342,7 @@ def ensemble_leg(a, mg):
     twins = make_contexts(mg, [dict(kws[i], precision=mg.FP64) for i in twin_ids])
     for t in twins:
         t.set_print(5, 1.0)
+        t.autotune(reps=5)
     ens = mg.Ensemble(members, batch=64)
     ens.run(a.ensemble_warmup, keep=False)
     for t in twins:

@@ -114,6 +114,17 @@ const char *stz_last_error(const stz_ctx *ctx);
    and runs the initial stress solve (STZ/bmg.c:117-127). */
 int stz_init_state(stz_ctx *ctx);
 
+/* Initial state drawn on the GPU for large grids, where the reference's sequential
+   host stream (stz_init_state) would take minutes: the same distributions from the
+   counter-based Philox4x32-10 generator keyed by seed, with every value a function of
+   (seed, voxel, mode, component), so the FP64 and mixed runs of one seed start from
+   the same state.  Voxels within one of nsph spheres (spheres[4*i..]: centre x, y, z in
+   voxels and radius, periodic) draw Q0 ~ N(inc_mean, inc_std) -- a soft inclusion;
+   *inside (optional) receives their count.  Then epsavg = 0 and the initial solve.
+   No counterpart in the reference. */
+int stz_init_state_device(stz_ctx *ctx, unsigned long long seed, const double *spheres, int nsph, double inc_mean,
+                          double inc_std, long long *inside);
+
 int stz_upload(stz_ctx *ctx, int field, const double *host);
 int stz_download(stz_ctx *ctx, int field, double *host);
 int stz_set_scalars(stz_ctx *ctx, const double epsavg[6], const double stress[6], double totime);
@@ -198,6 +209,24 @@ int stz_ensemble_create(stz_ctx *const *members, int n, stz_ensemble **out);
 int stz_ensemble_config(stz_ensemble *e, int batch, int use_graph);
 int stz_ensemble_run(stz_ensemble *e, int nsteps, stz_step_result *const *results, int *done, int *rcs);
 void stz_ensemble_destroy(stz_ensemble *e);
+
+/* Autotuner of the stress-solve schedule (no counterpart in the reference): times every
+   compiled tile variant of the four solve kernel slots -- x-pass rows (forward, inverse),
+   y-pass columns, fused z-pass columns -- with HIP events on the context's own fields
+   (the stress field is saved and restored), keeps the fastest and caches the choice per
+   grid shape and precision (flags & 1: measure again).  best_ms (optional) receives the
+   chosen variants' mean times (-1 when taken from the cache).  get/set read or pin the
+   choices (0 = built-in default). */
+int stz_autotune(stz_ctx *ctx, int reps, int flags, double best_ms[4]);
+int stz_get_tuning(stz_ctx *ctx, int out[4]);
+int stz_set_tuning(stz_ctx *ctx, const int in[4]);
+/* compiled variants of one slot (0 x forward rows, 1 y columns, 2 fused columns, 3 x
+   inverse rows) for this context's shape; returns their count */
+int stz_tune_candidates(stz_ctx *ctx, int slot, int out[8]);
+
+/* Host evaluation of the Philox4x32-10 rounds used by stz_init_state_device (known-answer
+   checks; no GPU needed). */
+int stz_philox_probe(const unsigned int ctr[4], const unsigned int key[2], unsigned int out[4]);
 
 /* Host-only check of the MT19937 lookup window used by the resident step: window values
    and stream positions against the same draws taken in order (no GPU needed). */

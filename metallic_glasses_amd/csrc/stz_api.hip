@@ -2,9 +2,14 @@
 // orchestration of one main-loop iteration of the reference (STZ/bmg.c:138-411).
 #include "stz_internal.h"
 
+#include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <new>
+#include <tuple>
 #include <vector>
 
 namespace {
@@ -546,6 +551,18 @@ int stz_init_state(stz_ctx *ctx)
     return solve(ctx);   // STZ/bmg.c:117-127
 }
 
+int stz_init_state_device(stz_ctx *ctx, unsigned long long seed, const double *spheres, int nsph, double inc_mean,
+                          double inc_std, long long *inside)
+{
+    if (!ctx || nsph < 0 || (nsph > 0 && !spheres)) return STZ_E_ARG;
+    int rc = stz_init_device_launch(ctx, seed, spheres, nsph, inc_mean, inc_std, inside);
+    if (rc) return rc;
+    rc = invalidate_cache(ctx);
+    if (rc) return rc;
+    for (int c = 0; c < 6; ++c) ctx->epsavg[c] = 0.0;
+    return solve(ctx);   // STZ/bmg.c:117-127
+}
+
 int stz_upload(stz_ctx *ctx, int field, const double *host)
 {
     if (!ctx || !host) return STZ_E_ARG;
@@ -969,6 +986,106 @@ int stz_timing_solve_get(stz_ctx *ctx, double ms[8], long long n[8])
         ms[i] = ctx->sms[i];
         n[i] = ctx->sn[i];
     }
+    return STZ_OK;
+}
+
+// Autotuner of the stress-solve tile schedule: every compiled variant of each of the
+// four kernel slots is timed with HIP events on this context's own fields (the stress
+// field is saved and restored), and the fastest is kept.  Results are cached per grid
+// shape and precision for the process.
+int stz_autotune(stz_ctx *ctx, int reps, int flags, double best_ms[4])
+{
+    if (!ctx || reps < 1) return STZ_E_ARG;
+    static std::map<std::tuple<int, int, int, int, int>, std::array<int, 4>> cache;
+    static std::mutex mu;
+    const auto key = std::make_tuple(ctx->dim, ctx->n[0], ctx->n[1], ctx->n[2], ctx->prec);
+    if (!(flags & 1)) {
+        std::lock_guard<std::mutex> lk(mu);
+        auto it = cache.find(key);
+        if (it != cache.end()) {
+            for (int s = 0; s < 4; ++s) ctx->tune[s] = it->second[(size_t)s];
+            ctx->tune_epoch++;
+            if (best_ms)
+                for (int s = 0; s < 4; ++s) best_ms[s] = -1.0;
+            return STZ_OK;
+        }
+    }
+    int cand[4][8], nc[4], maxc = 0;
+    for (int s = 0; s < 4; ++s) {
+        nc[s] = tune_candidates(ctx, s, cand[s]);
+        maxc = std::max(maxc, nc[s]);
+    }
+    if (!ctx->tev[0])
+        for (int i = 0; i < 24; ++i) STZ_HIP(hipEventCreate(&ctx->tev[i]));
+    const size_t sbytes = (size_t)ctx->C * (size_t)ctx->N * ctx->rsz;
+    void *save = nullptr;
+    STZ_HIP(hipMalloc(&save, sbytes));
+    STZ_HIP(hipMemcpyAsync(save, ctx->sigma, sbytes, hipMemcpyDeviceToDevice, ctx->stream));
+    const int saved_timing = ctx->timing;
+    const int saved_tune[4] = {ctx->tune[0], ctx->tune[1], ctx->tune[2], ctx->tune[3]};
+    double t[4][8];
+    int rc = STZ_OK;
+    for (int i = 0; i < maxc && rc == STZ_OK; ++i) {
+        for (int s = 0; s < 4; ++s) ctx->tune[s] = nc[s] ? cand[s][std::min(i, nc[s] - 1)] : 0;
+        ctx->timing = 1;
+        rc = stz_solve_launch(ctx, 0);
+        if (rc || hipStreamSynchronize(ctx->stream) != hipSuccess) break;
+        for (int j = 0; j < 8; ++j) {
+            ctx->sms[j] = 0.0;
+            ctx->sn[j] = 0;
+        }
+        for (int r = 0; r < reps && rc == STZ_OK; ++r) {
+            rc = stz_solve_launch(ctx, 0);
+            if (hipStreamSynchronize(ctx->stream) != hipSuccess) rc = STZ_E_HIP;
+            acc_solve(ctx);
+        }
+        const double n = (double)std::max<long long>(ctx->sn[0], 1);
+        t[0][i] = ctx->sms[0] / n;
+        t[1][i] = (ctx->sms[1] + ctx->sms[3]) / n;
+        t[2][i] = ctx->sms[2] / n;
+        t[3][i] = ctx->sms[4] / n;
+    }
+    for (int s = 0; s < 4; ++s) {
+        int best = 0;
+        for (int i = 1; i < nc[s]; ++i)
+            if (t[s][i] < t[s][best]) best = i;
+        ctx->tune[s] = nc[s] ? cand[s][best] : 0;
+        if (best_ms) best_ms[s] = nc[s] ? t[s][best] : 0.0;
+    }
+    if (rc) for (int s = 0; s < 4; ++s) ctx->tune[s] = saved_tune[s];
+    ctx->tune_epoch++;
+    (void)hipMemcpyAsync(ctx->sigma, save, sbytes, hipMemcpyDeviceToDevice, ctx->stream);
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipFree(save);
+    ctx->timing = saved_timing;
+    for (int j = 0; j < 8; ++j) {
+        ctx->tms[j] = ctx->sms[j] = 0.0;
+        ctx->tn[j] = ctx->sn[j] = 0;
+    }
+    if (rc) return rc;
+    std::lock_guard<std::mutex> lk(mu);
+    cache[key] = {ctx->tune[0], ctx->tune[1], ctx->tune[2], ctx->tune[3]};
+    return STZ_OK;
+}
+
+int stz_tune_candidates(stz_ctx *ctx, int slot, int out[8])
+{
+    if (!ctx || !out || slot < 0 || slot > 3) return STZ_E_ARG;
+    return tune_candidates(ctx, slot, out);
+}
+
+int stz_get_tuning(stz_ctx *ctx, int out[4])
+{
+    if (!ctx || !out) return STZ_E_ARG;
+    for (int s = 0; s < 4; ++s) out[s] = ctx->tune[s];
+    return STZ_OK;
+}
+
+int stz_set_tuning(stz_ctx *ctx, const int in[4])
+{
+    if (!ctx || !in) return STZ_E_ARG;
+    for (int s = 0; s < 4; ++s) ctx->tune[s] = in[s] < 0 ? 0 : in[s];
+    ctx->tune_epoch++;
     return STZ_OK;
 }
 

@@ -194,6 +194,8 @@ struct stz_ctx {
     long long avlan = 0;
     double soc_elast = 0.0;
 
+    int tune[4] = {0, 0, 0, 0};     // solve tile choices (x fwd rows, y width, fused width, x inv rows); 0: default
+    long long tune_epoch = 0;       // bumped when the choices change (captured graphs are rebuilt)
     StzRng rng;
     long long rng_epoch = 0;       // bumped when the stream is reseeded or replaced
     StzResident *rs = nullptr;     // resident-step resources (created on first use)
@@ -444,6 +446,9 @@ int stz_athermal_fused_launch(stz_ctx *ctx, int gated);   // + the |DWork| sum
 int stz_athermal_apply_range(stz_ctx *ctx, long long first, long long count);
 int stz_energy_launch(stz_ctx *ctx);
 bool stz_solve_supported(int dim, const int *n, int prec);
+int tune_candidates(const stz_ctx *ctx, int slot, int out[8]);
+int stz_init_device_launch(stz_ctx *ctx, unsigned long long seed, const double *spheres_host, int nsph, double inc_mean,
+                           double inc_std, long long *inside);
 // batched solve without the mean kernel (the caller reduces the *nparts partials)
 int stz_solve_launch_batched(stz_ctx *proto, int nrep, const SolveRep *reps, hipStream_t stream, int *nparts);
 

@@ -50,7 +50,7 @@ __host__ __device__ inline double philox_u53(uint32_t a, uint32_t b)
 }
 
 // two independent N(0,1) draws (Box-Muller on two 53-bit uniforms) for one counter
-__host__ __device__ inline void philox_normal2(uint32_t key0, uint32_t key1, uint32_t c0, uint32_t c1, uint32_t c2,
+__device__ inline void philox_normal2(uint32_t key0, uint32_t key1, uint32_t c0, uint32_t c1, uint32_t c2,
                                                uint32_t c3, double &g0, double &g1)
 {
     const Philox4 r = philox4x32_10(Philox4{{c0, c1, c2, c3}}, key0, key1);

@@ -513,6 +513,7 @@ struct StzPlan {
     int batch = 64;
     int use_graph = 0;                // steps per captured graph; 0: direct launches
     int graph_steps = 0;              // steps in the instantiated graph
+    long long graph_epoch = -1;       // tile choices the graph was captured with
     // rate-pass timing of direct launches (single context with timing on)
     std::vector<hipEvent_t> eva, evb;
     std::vector<hipEvent_t> evc;      // progress markers of the chunked launches
@@ -868,7 +869,10 @@ int plan_upload(StzPlan *pl)
 // captures use_graph consecutive steps into one graph
 int plan_graph(StzPlan *pl)
 {
-    if (pl->exec && pl->graph_steps == pl->use_graph) return STZ_OK;
+    long long ep = 0;
+    for (const stz_ctx *c : pl->all) ep += c->tune_epoch;
+    if (pl->exec && pl->graph_steps == pl->use_graph && pl->graph_epoch == ep) return STZ_OK;
+    pl->graph_epoch = ep;
     if (pl->exec) (void)hipGraphExecDestroy(pl->exec);
     if (pl->graph) (void)hipGraphDestroy(pl->graph);
     pl->exec = nullptr;

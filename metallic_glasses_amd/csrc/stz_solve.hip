@@ -528,6 +528,28 @@ constexpr int fused_cols_for(int L)
 }
 constexpr int fused_threads(int elems) { return elems >= 4096 ? 1024 : (elems >= 2048 ? 512 : 256); }
 
+// Tile variants the autotuner may pick (compiled for every length): row counts
+// NR in [2, 2*rows_for] for the x passes with a tile of at most 64 KB (the default may
+// be below rows_for when the row total is not a multiple of it), y-pass
+// widths W in [cols_for/2, 2*cols_for] with at most 64 KB, fused widths W in
+// [fused_cols_for/4, fused_cols_for].
+template <typename R, int C>
+constexpr bool x_variant(int L0, int NR)
+{
+    return NR >= 2 && NR <= 2 * rows_for<R, C>(L0) &&
+           (long long)L0 * (C * NR / 2 + 1) * (long long)sizeof(cpx<R>) <= 65536;
+}
+template <typename R>
+constexpr bool y_variant(int L, int W)
+{
+    return W >= 4 && W * 2 >= cols_for(L) && W <= 2 * cols_for(L) && (long long)L * W * (long long)sizeof(cpx<R>) <= 65536;
+}
+template <typename R, int C>
+constexpr bool f_variant(int L, int W)
+{
+    return W >= 2 && W * 4 >= fused_cols_for<R, C>(L) && W <= fused_cols_for<R, C>(L);
+}
+
 template <typename R>
 struct Launch {
     stz_ctx *ctx;
@@ -545,34 +567,57 @@ struct Launch {
         return (k.rows % NR) == 0 ? NR : 0;
     }
 
+    // each go_* launches a compiled variant or reports that none exists (never skips silently)
     template <int L0, int NRV, int C>
-    void go_fwd_x(dim3 grid)
+    int go_fwd_x(dim3 grid)
     {
-        if constexpr (NRV <= rows_for<R, C>(L0))
+        if constexpr (x_variant<R, C>(L0, NRV)) {
             hipLaunchKernelGGL((k_fwd_x<R, L0, NRV, C>), grid, dim3(NT_SOLVE), 0, stream, ctx->eps0,
                                (cpx<R> *)ctx->spec, (const cpx<R> *)ctx->tw.tw[0], k, reps);
+            return STZ_OK;
+        } else {
+            return STZ_E_UNSUPPORTED;
+        }
     }
 
     template <int L0, int NRV, int C>
-    void go_inv_x(dim3 grid, double *spart)
+    int go_inv_x(dim3 grid, double *spart)
     {
-        if constexpr (NRV <= rows_for<R, C>(L0))
+        if constexpr (x_variant<R, C>(L0, NRV)) {
             hipLaunchKernelGGL((k_inv_x<R, L0, NRV, C>), grid, dim3(NT_SOLVE), 0, stream, (cpx<R> *)ctx->spec,
                                ctx->eps0, (R *)ctx->sigma, spart, nanflag, (const cpx<R> *)ctx->tw.tw[0], k, reps);
+            return STZ_OK;
+        } else {
+            return STZ_E_UNSUPPORTED;
+        }
+    }
+
+    // the tuned row count when it is a compiled variant dividing the rows, else the default
+    template <int L0, int C>
+    int x_rows(int tuned) const
+    {
+        if (tuned > 0 && k.rows % tuned == 0 && x_variant<R, C>(L0, tuned)) return tuned;
+        return pick_rows(rows_for<R, C>(L0));
     }
 
     template <int L0, int C>
     int fwd_x_n()
     {
-        const int NR = pick_rows(rows_for<R, C>(L0));
+        const int NR = x_rows<L0, C>(ctx->tune[0]);
         if (!NR) return STZ_E_ARG;
         const dim3 grid(k.rows / NR, nrep);
+        int rc;
         switch (NR) {
-        case 32: go_fwd_x<L0, 32, C>(grid); break;
-        case 16: go_fwd_x<L0, 16, C>(grid); break;
-        case 8: go_fwd_x<L0, 8, C>(grid); break;
-        case 4: go_fwd_x<L0, 4, C>(grid); break;
-        default: go_fwd_x<L0, 2, C>(grid); break;
+        case 32: rc = go_fwd_x<L0, 32, C>(grid); break;
+        case 16: rc = go_fwd_x<L0, 16, C>(grid); break;
+        case 8: rc = go_fwd_x<L0, 8, C>(grid); break;
+        case 4: rc = go_fwd_x<L0, 4, C>(grid); break;
+        case 2: rc = go_fwd_x<L0, 2, C>(grid); break;
+        default: rc = STZ_E_UNSUPPORTED; break;
+        }
+        if (rc) {
+            ctx->err = "no compiled x-pass variant for this grid";
+            return rc;
         }
         STZ_CHECK_LAUNCH();
         return STZ_OK;
@@ -581,40 +626,92 @@ struct Launch {
     template <int L0, int C>
     int inv_x_n(double *spart, int *nblocks)
     {
-        const int NR = pick_rows(rows_for<R, C>(L0));
+        const int NR = x_rows<L0, C>(ctx->tune[3]);
         if (!NR) return STZ_E_ARG;
         *nblocks = k.rows / NR;
         const dim3 grid(k.rows / NR, nrep);
+        int rc;
         switch (NR) {
-        case 32: go_inv_x<L0, 32, C>(grid, spart); break;
-        case 16: go_inv_x<L0, 16, C>(grid, spart); break;
-        case 8: go_inv_x<L0, 8, C>(grid, spart); break;
-        case 4: go_inv_x<L0, 4, C>(grid, spart); break;
-        default: go_inv_x<L0, 2, C>(grid, spart); break;
+        case 32: rc = go_inv_x<L0, 32, C>(grid, spart); break;
+        case 16: rc = go_inv_x<L0, 16, C>(grid, spart); break;
+        case 8: rc = go_inv_x<L0, 8, C>(grid, spart); break;
+        case 4: rc = go_inv_x<L0, 4, C>(grid, spart); break;
+        case 2: rc = go_inv_x<L0, 2, C>(grid, spart); break;
+        default: rc = STZ_E_UNSUPPORTED; break;
+        }
+        if (rc) {
+            ctx->err = "no compiled x-pass variant for this grid";
+            return rc;
         }
         STZ_CHECK_LAUNCH();
         return STZ_OK;
     }
 
+    template <int L1, int W>
+    int go_ypass(R sgn)
+    {
+        if constexpr (y_variant<R>(L1, W)) {
+            const dim3 grid((k.h0 + W - 1) / W, k.n[2], ctx->C * nrep);
+            hipLaunchKernelGGL((k_ypass<R, L1, W>), grid, dim3(NT_SOLVE), 0, stream, (cpx<R> *)ctx->spec,
+                               (const cpx<R> *)ctx->tw.tw[1], k, sgn, reps);
+            return STZ_OK;
+        } else {
+            return STZ_E_UNSUPPORTED;
+        }
+    }
+
     template <int L1>
     int ypass_n(R sgn)
     {
-        constexpr int W = cols_for(L1);
-        const dim3 grid((k.h0 + W - 1) / W, k.n[2], ctx->C * nrep);
-        hipLaunchKernelGGL((k_ypass<R, L1, W>), grid, dim3(NT_SOLVE), 0, stream, (cpx<R> *)ctx->spec,
-                           (const cpx<R> *)ctx->tw.tw[1], k, sgn, reps);
+        int W = ctx->tune[1];
+        if (!(W > 0 && y_variant<R>(L1, W))) W = cols_for(L1);
+        int rc;
+        switch (W) {
+        case 32: rc = go_ypass<L1, 32>(sgn); break;
+        case 16: rc = go_ypass<L1, 16>(sgn); break;
+        case 8: rc = go_ypass<L1, 8>(sgn); break;
+        case 4: rc = go_ypass<L1, 4>(sgn); break;
+        default: rc = STZ_E_UNSUPPORTED; break;
+        }
+        if (rc) {
+            ctx->err = "no compiled y-pass variant for this grid";
+            return rc;
+        }
         STZ_CHECK_LAUNCH();
         return STZ_OK;
+    }
+
+    template <int LZ, int C, int DIM, int W>
+    int go_fused()
+    {
+        if constexpr (f_variant<R, C>(LZ, W)) {
+            constexpr int NTF = fused_threads(LZ * C * W);
+            const dim3 grid((k.h0 + W - 1) / W, DIM == 3 ? k.n[1] : 1, nrep);
+            hipLaunchKernelGGL((k_fused<R, LZ, W, C, DIM, NTF>), grid, dim3(NTF), 0, stream, (cpx<R> *)ctx->spec,
+                               (const cpx<R> *)ctx->tw.tw[DIM - 1], k, reps);
+            return STZ_OK;
+        } else {
+            return STZ_E_UNSUPPORTED;
+        }
     }
 
     template <int LZ, int C, int DIM>
     int fused_n()
     {
-        constexpr int W = fused_cols_for<R, C>(LZ);
-        constexpr int NTF = fused_threads(LZ * C * W);
-        const dim3 grid((k.h0 + W - 1) / W, DIM == 3 ? k.n[1] : 1, nrep);
-        hipLaunchKernelGGL((k_fused<R, LZ, W, C, DIM, NTF>), grid, dim3(NTF), 0, stream, (cpx<R> *)ctx->spec,
-                           (const cpx<R> *)ctx->tw.tw[DIM - 1], k, reps);
+        int W = ctx->tune[2];
+        if (!(W > 0 && f_variant<R, C>(LZ, W))) W = fused_cols_for<R, C>(LZ);
+        int rc;
+        switch (W) {
+        case 16: rc = go_fused<LZ, C, DIM, 16>(); break;
+        case 8: rc = go_fused<LZ, C, DIM, 8>(); break;
+        case 4: rc = go_fused<LZ, C, DIM, 4>(); break;
+        case 2: rc = go_fused<LZ, C, DIM, 2>(); break;
+        default: rc = STZ_E_UNSUPPORTED; break;
+        }
+        if (rc) {
+            ctx->err = "no compiled fused-pass variant for this grid";
+            return rc;
+        }
         STZ_CHECK_LAUNCH();
         return STZ_OK;
     }
@@ -719,6 +816,39 @@ int upload_twiddles(stz_ctx *ctx, int axis, int L)
 }
 
 }  // namespace
+
+namespace {
+
+template <typename R, int C>
+int candidates_t(const stz_ctx *ctx, int slot, int *out)
+{
+    const int rows = ctx->n[1] * ctx->n[2];
+    int n = 0;
+    if (slot == 0 || slot == 3) {
+        const int L0 = ctx->n[0];
+        for (int v = 32; v >= 2; v /= 2)
+            if (x_variant<R, C>(L0, v) && rows % v == 0) out[n++] = v;
+    } else if (slot == 1) {
+        if (ctx->dim != 3) return 0;
+        for (int v = 32; v >= 4; v /= 2)
+            if (y_variant<R>(ctx->n[1], v)) out[n++] = v;
+    } else {
+        const int L = ctx->n[ctx->dim - 1];
+        for (int v = 16; v >= 2; v /= 2)
+            if (f_variant<R, C>(L, v)) out[n++] = v;
+    }
+    return n;
+}
+
+}  // namespace
+
+// compiled tile variants of solve kernel slot (0 x forward, 1 y passes, 2 fused, 3 x inverse)
+int tune_candidates(const stz_ctx *ctx, int slot, int out[8])
+{
+    if (ctx->prec == STZ_FP64)
+        return ctx->C == 3 ? candidates_t<double, 3>(ctx, slot, out) : candidates_t<double, 6>(ctx, slot, out);
+    return ctx->C == 3 ? candidates_t<float, 3>(ctx, slot, out) : candidates_t<float, 6>(ctx, slot, out);
+}
 
 bool stz_solve_supported(int dim, const int *n, int prec)
 {

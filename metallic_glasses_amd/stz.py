@@ -119,6 +119,11 @@ def lib():
         L.stz_timing_get.argtypes = [vp, dp, P(C.c_longlong)]
         L.stz_timing_solve_get.argtypes = [vp, dp, P(C.c_longlong)]
         L.stz_run_config.argtypes = [vp, C.c_int, C.c_int, C.c_longlong]
+        L.stz_autotune.argtypes = [vp, C.c_int, C.c_int, dp]
+        L.stz_init_state_device.argtypes = [vp, C.c_ulonglong, dp, C.c_int, C.c_double, C.c_double, P(C.c_longlong)]
+        L.stz_get_tuning.argtypes = [vp, P(C.c_int)]
+        L.stz_set_tuning.argtypes = [vp, P(C.c_int)]
+        L.stz_tune_candidates.argtypes = [vp, C.c_int, P(C.c_int)]
         L.stz_run_stop_at_print.argtypes = [vp, C.c_int]
         L.stz_ensemble_create.argtypes = [P(vp), C.c_int, P(vp)]
         L.stz_ensemble_config.argtypes = [vp, C.c_int, C.c_int]
@@ -153,7 +158,10 @@ def _dp(a):
 class Stz:
     """One simulation context on the current HIP device."""
 
-    def __init__(self, init_state=True, **kw):
+    def __init__(self, init_state=True, device_init=None, **kw):
+        """init_state: draw the reference's MT19937 initial state on the host;
+        device_init = dict(seed=..., spheres=[(x, y, z, r), ...], inc_mean=..., inc_std=...):
+        draw it on the GPU instead (counter-based, for large grids)."""
         self.params = make_params(**kw)
         self._h = C.c_void_p()
         rc = lib().stz_create(C.byref(self.params), C.byref(self._h))
@@ -166,7 +174,15 @@ class Stz:
         self.C = 3 if p.dim == 2 else 6
         self.S = 2 if p.dim == 2 else 5
         self.M = p.modes
-        if init_state:
+        self.inclusion_voxels = 0
+        if device_init is not None:
+            sph = np.ascontiguousarray(np.array(device_init.get("spheres", []), dtype=np.float64).reshape(-1, 4))
+            cnt = C.c_longlong()
+            self._check(lib().stz_init_state_device(
+                self._h, int(device_init.get("seed", p.seed)), _dp(sph) if len(sph) else None, len(sph),
+                float(device_init.get("inc_mean", 0.0)), float(device_init.get("inc_std", 0.0)), C.byref(cnt)))
+            self.inclusion_voxels = cnt.value
+        elif init_state:
             self._check(lib().stz_init_state(self._h))
 
     def _check(self, rc):
@@ -272,6 +288,27 @@ class Stz:
 
     def run_config(self, batch=0, use_graph=-1, window_words=0):
         self._check(lib().stz_run_config(self._h, int(batch), int(use_graph), int(window_words)))
+
+    def autotune(self, reps=10, force=False):
+        """Times the solve kernels' tile variants and keeps the fastest; returns
+        (choices, mean ms of the chosen variants)."""
+        ms = np.zeros(4)
+        self._check(lib().stz_autotune(self._h, int(reps), 1 if force else 0, _dp(ms)))
+        return self.tuning(), ms
+
+    def tuning(self):
+        o = (C.c_int * 4)()
+        self._check(lib().stz_get_tuning(self._h, o))
+        return list(o)
+
+    def tune_candidates(self, slot):
+        o = (C.c_int * 8)()
+        n = self._check(lib().stz_tune_candidates(self._h, int(slot), o))
+        return list(o)[:n]
+
+    def set_tuning(self, t):
+        a = (C.c_int * 4)(*[int(x) for x in t])
+        self._check(lib().stz_set_tuning(self._h, a))
 
     def stop_at_print(self, on=True):
         self._check(lib().stz_run_stop_at_print(self._h, 1 if on else 0))
@@ -381,3 +418,13 @@ def mt_window_probe(seed, skip, window_words, offset, with_uniform, ngauss):
     if rc:
         raise StzError(rc, "stz_mt_window_probe")
     return w[:n], s[:n], we.value, se.value
+
+
+def soft_inclusions(n, fraction=0.05, radius=16.0, seed=1234):
+    """Sphere centres for a soft-inclusion region: count = fraction of the grid volume
+    over the sphere volume, centres uniform on the periodic grid from MT19937(seed)."""
+    vol = float(n[0]) * n[1] * n[2]
+    k = max(1, int(round(fraction * vol / (4.0 / 3.0 * np.pi * radius ** 3))))
+    rs = np.random.RandomState(seed)
+    c = rs.uniform(0.0, 1.0, size=(k, 3)) * np.array(n[:3], dtype=np.float64)
+    return [(float(x), float(y), float(z), float(radius)) for x, y, z in c]

@@ -72,6 +72,33 @@ def test_stress_solve_fp64(dims):
     assert np.allclose(s_g[:o.C], s_o[:o.C], rtol=1e-10, atol=1e-12 * np.max(np.abs(ref)))
 
 
+@pytest.mark.parametrize("prec", [mg.FP64, mg.MIXED])
+@pytest.mark.parametrize("dims", [(64, 32), (512, 512), (32, 32, 32), (64, 64, 128), (96, 96, 96), (128, 128, 128)])
+def test_stress_solve_every_tile_variant(dims, prec):
+    """Every compiled tile variant the autotuner can pick gives the oracle's stress."""
+    print("tile variants", dims, prec, flush=True)
+    o, g = _pair(**_kw(dims, modes=2, precision=prec))
+    _random_state(o, 9)
+    _copy_state(o, g)
+    o.solve()
+    ref = o.field(po.F_SIGMA)
+    cands = [g.tune_candidates(s) for s in range(4)]
+    assert cands[0] and cands[2] and cands[3] and (cands[1] or len(dims) == 2)
+    tol = 1e-10 if prec == mg.FP64 else 1e-5
+    for i in range(max(len(c) for c in cands)):
+        t = [c[min(i, len(c) - 1)] if c else 0 for c in cands]
+        g.set_tuning(t)
+        g.stress_solve()
+        got = g.download(mg.F_SIGMA)
+        err = np.linalg.norm(got - ref) / np.linalg.norm(ref)
+        assert err < tol, (t, err)
+    choice, ms = g.autotune(reps=3, force=True)
+    assert all(choice[s] in cands[s] for s in range(4) if cands[s])
+    g.stress_solve()
+    got = g.download(mg.F_SIGMA)
+    assert np.linalg.norm(got - ref) / np.linalg.norm(ref) < tol
+
+
 @pytest.mark.parametrize("dims", [(16, 16), (512, 512), (8, 8, 8), (64, 64, 64), (96, 96, 96)])
 def test_stress_solve_mixed(dims):
     print("solve mixed", dims, flush=True)

@@ -81,3 +81,22 @@ def test_window_end_positions_consume_two_words_per_attempt():
 def test_window_too_short_is_an_error():
     with pytest.raises(mg.StzError):
         mg.mt_window_probe(5, 0, 64, 0, 0, 100)
+
+
+# Philox4x32-10 known answers (Random123 kat_vectors: philox4x32_10)
+PHILOX_KAT = [
+    ((0, 0, 0, 0), (0, 0), (0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8)),
+    ((0xffffffff,) * 4, (0xffffffff, 0xffffffff), (0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd)),
+    ((0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344), (0xa4093822, 0x299f31d0),
+     (0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1)),
+]
+
+
+@pytest.mark.parametrize("ctr,key,expect", PHILOX_KAT)
+def test_philox_known_answers(ctr, key, expect):
+    lib = mg.lib()
+    c = (C.c_uint * 4)(*ctr)
+    k = (C.c_uint * 2)(*key)
+    o = (C.c_uint * 4)()
+    assert lib.stz_philox_probe(c, k, o) == 0
+    assert tuple(o) == expect
