@@ -44,7 +44,102 @@ def parse():
     ap.add_argument("--ensemble-replicas", type=int, default=64)
     ap.add_argument("--ensemble-steps", type=int, default=200)
     ap.add_argument("--ensemble-warmup", type=int, default=200)
+    ap.add_argument("--pmc-target", default=None,
+                    help="comma list of rates,copy: run only those kernels (a few launches) for a "
+                         "rocprofv3 --pmc pass, then exit")
+    ap.add_argument("--pmc-summary", default=None,
+                    help="directory of rocprofv3 --pmc output: print per-kernel mean counters, then exit")
+    ap.add_argument("--trace-summary", default=None,
+                    help="directory of rocprofv3 --kernel-trace output: print per-kernel durations, then exit")
     return ap.parse_args()
+
+
+COPY_PROBE_BYTES = 1 << 30
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "kb_rates_128cube_fp64_traffic.txt")
+
+
+def pmc_target(a, mg):
+    """A short run of the kernels whose byte counters are read: `rates` = the bench
+    workload's resident step (every step launches kb_rates once), `copy` = the plain copy
+    of COPY_PROBE_BYTES that calibrates the counters."""
+    for t in a.pmc_target.split(","):
+        if t == "copy":
+            ms, gbs = mg.copy_probe(COPY_PROBE_BYTES, 5)
+            print(f"# copy probe {COPY_PROBE_BYTES} B: {ms:.4f} ms {gbs:.1f} GB/s", file=sys.stderr, flush=True)
+        elif t == "rates":
+            dims = tuple(int(x) for x in a.grid.split(","))
+            g = mg.Stz(**mg.config_3d(n=dims, seed=a.seed, q0_std=0.2))
+            g.set_print(5, 1.0)
+            rc, res = g.run_steps(a.steps)
+            print(f"# rates target: {len(res)} resident steps of {dims}", file=sys.stderr, flush=True)
+            g.close()
+        else:
+            raise SystemExit(f"unknown pmc target {t}")
+
+
+def _active(vals):
+    """Dispatches that did the kernel's work: a replica that halted (or a step of another
+    kind) still launches every kernel of the step, which then exits at once."""
+    vals = np.asarray(vals, dtype=np.float64)
+    return vals[vals > 0.05 * vals.max()] if len(vals) and vals.max() > 0 else vals
+
+
+def pmc_summary(d):
+    """Per kernel and counter in rocprofv3's counter CSV files: mean over all dispatches
+    and over the active ones (value above 5% of the largest)."""
+    import csv
+    import glob
+    acc = {}
+    for f in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
+        with open(f) as fh:
+            for row in csv.DictReader(fh):
+                key = (row["Kernel_Name"], row["Counter_Name"])
+                disp = row.get("Dispatch_Id", row.get("Correlation_Id"))
+                acc.setdefault(key, {})
+                acc[key][disp] = acc[key].get(disp, 0.0) + float(row["Counter_Value"])
+    for (k, c), v in sorted(acc.items()):
+        vals = np.array(list(v.values()))
+        act = _active(vals)
+        print(f"{c:12s} dispatches {len(vals):5d} mean {vals.mean():14.1f} active {len(act):5d} "
+              f"mean {act.mean() if len(act) else 0.0:14.1f} max {vals.max():14.1f}  {k[:100]}")
+
+
+def trace_summary(d):
+    """Per kernel in rocprofv3's kernel-trace CSV files: calls, mean duration over all
+    dispatches and over the active ones (duration above 5% of the longest)."""
+    import csv
+    import glob
+    acc = {}
+    for f in glob.glob(os.path.join(d, "**", "*kernel_trace.csv"), recursive=True):
+        with open(f) as fh:
+            for row in csv.DictReader(fh):
+                acc.setdefault(row["Kernel_Name"], []).append(
+                    (float(row["End_Timestamp"]) - float(row["Start_Timestamp"])) * 1e-3)
+    tot = sum(sum(v) for v in acc.values())
+    print(f"{'kernel':100s} {'calls':>6s} {'mean_us':>10s} {'active':>6s} {'act_mean_us':>11s} {'pct':>6s}")
+    for k, v in sorted(acc.items(), key=lambda kv: -sum(kv[1])):
+        vals = np.array(v)
+        act = _active(vals)
+        print(f"{k[:100]:100s} {len(vals):6d} {vals.mean():10.3f} {len(act):6d} {act.mean():11.3f} "
+              f"{100 * vals.sum() / tot:6.2f}")
+
+
+def traffic_from_profile(dims, precision):
+    """HBM bytes per kb_rates launch from the committed rocprofv3 --pmc summary (FETCH_SIZE
+    and WRITE_SIZE, each scaled by the copy probe's calibration factor), when that summary
+    was measured on this workload; else None."""
+    try:
+        lines = open(TRAFFIC_FILE).read().splitlines()
+    except OSError:
+        return None
+    kv = {}
+    for ln in lines:
+        if ln.startswith("traffic_") and ":" in ln:
+            k, v = ln.split(":", 1)
+            kv[k.strip()] = v.strip()
+    if kv.get("traffic_workload") != f"{dims[0]}x{dims[1]}x{dims[2]} {precision}":
+        return None
+    return float(kv["traffic_bytes_per_launch"])
 
 
 def rate_bytes_per_voxel(C, M, S, b):
@@ -108,7 +203,16 @@ def main():
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(every box has one GPU); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if a.pmc_summary:
+        pmc_summary(a.pmc_summary)
+        return
+    if a.trace_summary:
+        trace_summary(a.trace_summary)
+        return
     import metallic_glasses_amd as mg
+    if a.pmc_target:
+        pmc_target(a, mg)
+        return
 
     dims = tuple(int(x) for x in a.grid.split(","))
     kw = mg.config_3d(n=dims, seed=a.seed, q0_std=0.2)
@@ -144,6 +248,7 @@ def main():
     N = g.N
     C, M, S = g.C, g.M, g.S
     ev_fp64 = kinds[1] / dt
+    _, copy_gbs = mg.copy_probe(COPY_PROBE_BYTES, 10)
     # kernel timing with HIP events on the library's stream, continuing the same run:
     # the resident step in direct launches (events around every rate pass), then the
     # per-call step (events around every solve kernel)
@@ -188,8 +293,10 @@ def main():
         "steps_by_kind": {"elastic": int(kinds[0]), "thermal": int(kinds[1]), "athermal": int(kinds[2])},
         "events_per_s_direct_launches": round(float(tk[1] / dt_direct), 3),
         "roofline": {"bound": "hbm", "kernel": "kb_rates", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": traffic_from_profile(dims, "fp64"),
                      "bytes_per_launch": int(k4_bytes), "avg_ms": round(k4_ms, 5)},
+        "stream_copy_GBs": round(copy_gbs, 1),
         "solve_ms": round(solve_ms, 5),
         "solve_tiles": tune64[0],
     }

@@ -39,6 +39,8 @@ extern "C" {
 #define STZ_MIXED 1
 #define STZ_TENSION 0
 #define STZ_SHEAR 1
+#define STZ_SOLVER_FFT 0     /* full Green's-function solve after every transform */
+#define STZ_SOLVER_LOCAL 1   /* event-local update, full solve every `resync` updates */
 
 /* fields for stz_upload / stz_download */
 #define STZ_F_EPS0 0
@@ -134,6 +136,25 @@ int stz_get_scalars(stz_ctx *ctx, double epsavg[6], double stress[6], double *to
    779-828): sigma field and <sigma> from EPS0 and EpsAvg.  Discards a pending uniform
    stress increment (the reference's solve overwrites Sigma). */
 int stz_stress_solve(stz_ctx *ctx);
+
+/* Event-local form of the same solve, replacing InhomogeneousStressSolver(uflag=0) after
+   transforms at `n` sites (STZ/evolution.c:70-89, 250-259): eps0 has already changed by
+   the transformation strains that the transforms recorded (the PreDEps0 snapshot of
+   STZ/elastic.c:369-374) at those sites and nowhere else.  The solve is linear and
+   translation invariant, so the new stress is the field as last solved, plus C0 times the
+   change of EpsAvg since then, plus the superposed impulse responses of the sites (built
+   once per grid and elastic constants by FP64 FFT solves of unit impulses, kept for one
+   octant of offsets).  Updates the sigma field and <sigma> like stz_stress_solve, to
+   rounding.  No counterpart in the reference.  n <= 64; -3 if the field has not been
+   solved since the last upload or scalar change. */
+int stz_stress_update_local(stz_ctx *ctx, int n, const int *sites);
+
+/* How the resident step (stz_run_steps, stz_ensemble_run) brings the stress up to date
+   after transforms: STZ_SOLVER_FFT, or STZ_SOLVER_LOCAL = the event-local update for a
+   thermal event and for avalanches of at most local_max voxels, with a full solve after
+   every `resync` local updates, for larger avalanches and at print steps.  No
+   counterpart in the reference.  Defaults: FFT, 256, 8. */
+int stz_set_solver(stz_ctx *ctx, int mode, long long resync, int local_max);
 
 /* SofteningFactor + ActEnergy + Q<0 count + KMC rate total
    (STZ/init.c:711-739, 561-690; STZ/bmg.c:147-157; STZ/evolution.c:120-134). */
@@ -240,6 +261,13 @@ int stz_rng_gaussian(stz_ctx *ctx, double sigma, double *g);
 
 /* Counters: step, astep, kstep, estep, transforms, solves. */
 int stz_counters(stz_ctx *ctx, long long out[6]);
+
+/* Plain device copy of `bytes` (rounded down to 16-byte words) on the current device,
+   timed with HIP events over `reps` launches: avg_ms per launch and gbs = 2*bytes/time
+   (read + write).  Measurement hook with no counterpart in the reference: the stream-copy
+   bandwidth reported beside the kernels' rooflines, and the kernel whose byte counters
+   calibrate the hardware-counter traffic figures.  Allocates and frees its buffers. */
+int stz_copy_probe(long long bytes, int reps, double *avg_ms, double *gbs);
 
 /* MT19937 state (624 words + position) of the context stream, for checkpoints and for
    handing a run over to another implementation mid-trajectory.  No counterpart in the

@@ -68,6 +68,13 @@ __global__ void k_add(R *p, long long n, double d)
     if (i < n) p[i] = p[i] + (R)d;
 }
 
+// plain 16-byte-per-lane copy: the bandwidth and byte-counter reference of stz_copy_probe
+__global__ __launch_bounds__(256) void k_copy16(const double2 *__restrict__ src, double2 *__restrict__ dst, long long n)
+{
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) dst[i] = src[i];
+}
+
 // waits for the stream by polling, which returns sooner than a blocking wait
 int spin_sync(stz_ctx *ctx)
 {
@@ -232,6 +239,7 @@ int solve(stz_ctx *ctx)
     }
     for (int c = 0; c < ctx->C; ++c) ctx->stress[c] = ctx->hres[c];
     ctx->nsolve++;
+    stz_local_mark_solved(ctx);
     acc_solve(ctx);
     return STZ_OK;
 }
@@ -245,6 +253,7 @@ int add_uniform(stz_ctx *ctx, int comp, double delta)
         }
         ctx->pend_comp = comp;
         ctx->pend_delta = delta;
+        ctx->usum[comp] += delta;     // in the field once applied
     } else {
         ctx->soff[comp] += delta;
     }
@@ -501,6 +510,7 @@ void stz_destroy(stz_ctx *ctx)
     if (!ctx) return;
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     stz_resident_free(ctx);
+    stz_local_release(ctx);
     void *ptrs[] = {ctx->q0eff, ctx->scache, ctx->eps0, ctx->sigma, ctx->q0, ctx->deps, ctx->deps_last, ctx->perm, ctx->temp, ctx->tlast,
                     ctx->spec, ctx->minq, ctx->argm, ctx->bsum, ctx->gsum, ctx->res, ctx->spart,
                     ctx->acnt, ctx->aoff, ctx->alist, ctx->dnew, ctx->socp, ctx->gneg};
@@ -586,6 +596,7 @@ int stz_upload(stz_ctx *ctx, int field, const double *host)
     default: return STZ_E_ARG;
     }
     if (rc) return rc;
+    if (field == STZ_F_EPS0 || field == STZ_F_SIGMA) ctx->nlocal = 1ll << 60;   // full solve before a local update
     if (field != STZ_F_EPS0 && field != STZ_F_SIGMA) {
         rc = invalidate_cache(ctx);
         if (rc) return rc;
@@ -635,6 +646,7 @@ int stz_set_scalars(stz_ctx *ctx, const double epsavg[6], const double stress[6]
         if (stress) ctx->stress[c] = stress[c];
     }
     ctx->totime = totime;
+    if (epsavg) ctx->nlocal = 1ll << 60;      // the field no longer matches ebase: full solve first
     return STZ_OK;
 }
 
@@ -650,6 +662,63 @@ int stz_get_scalars(stz_ctx *ctx, double epsavg[6], double stress[6], double *to
 }
 
 int stz_stress_solve(stz_ctx *ctx) { return ctx ? solve(ctx) : STZ_E_ARG; }
+
+int stz_stress_update_local(stz_ctx *ctx, int n, const int *sites)
+{
+    if (!ctx || n < 0 || n > STZ_LOCAL_MAXSRC || (n > 0 && !sites)) return STZ_E_ARG;
+    for (int i = 0; i < n; ++i)
+        if (sites[i] < 0 || sites[i] >= ctx->N) return STZ_E_ARG;
+    if (ctx->nlocal >= (1ll << 59)) {
+        ctx->err = "the stress field has not been solved since its inputs were replaced";
+        return STZ_E_ARG;
+    }
+    int rc = stz_local_prepare(ctx);
+    if (rc) return rc;
+    if (ctx->prec == STZ_FP64 && ctx->pend_comp >= 0) {
+        rc = flush_pending(ctx);              // the increment is counted in usum
+        if (rc) return rc;
+    }
+    double uni[6];
+    stz_local_uniform(ctx, uni);
+    STZ_HIP(hipMemsetAsync(ctx->res + 15, 0, sizeof(double), ctx->stream));
+    if (n > 0) STZ_HIP(hipMemcpyAsync(ctx->lsrc, sites, (size_t)n * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+    rc = stz_local_launch(ctx, n, uni);
+    if (rc) return rc;
+    STZ_HIP(hipMemcpyAsync(ctx->hres, ctx->res, 16 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    STZ_HIP(hipStreamSynchronize(ctx->stream));
+    int nan;
+    std::memcpy(&nan, ctx->hres + 15, sizeof(int));
+    if (nan) {
+        ctx->err = "NaN in the stress update";
+        return STZ_E_NAN;
+    }
+    for (int c = 0; c < ctx->C; ++c) ctx->stress[c] = ctx->hres[c];
+    for (int c = 0; c < 6; ++c) {
+        ctx->soff[c] = 0.0;                   // the update rewrote the stress field
+        ctx->ebase[c] = ctx->epsavg[c];
+        ctx->usum[c] = 0.0;
+    }
+    ctx->pend_comp = -1;
+    ctx->pend_delta = 0.0;
+    ctx->nlocal++;
+    ctx->nsolve++;
+    return STZ_OK;
+}
+
+int stz_set_solver(stz_ctx *ctx, int mode, long long resync, int local_max)
+{
+    if (!ctx || (mode != STZ_SOLVER_FFT && mode != STZ_SOLVER_LOCAL) || resync < 1 || local_max < 1 ||
+        local_max > STZ_LOCAL_MAXSRC)
+        return STZ_E_ARG;
+    if (mode == STZ_SOLVER_LOCAL) {
+        const int rc = stz_local_prepare(ctx);
+        if (rc) return rc;
+    }
+    ctx->solver_mode = mode;
+    ctx->local_resync = resync;
+    ctx->local_max = local_max;
+    return STZ_OK;
+}
 
 int stz_rates(stz_ctx *ctx, long long *n_negative, double *total_rate)
 {
@@ -795,6 +864,7 @@ int stz_step_kmc(stz_ctx *ctx, stz_step_result *res)
         ctx->nsolve++;
         for (int c = 0; c < ctx->C; ++c) ctx->stress[c] = ctx->hres[c];
         for (int c = 0; c < 6; ++c) ctx->soff[c] = 0.0;   // the solve rewrote the stress field
+        stz_local_mark_solved(ctx);
     }
     if (kind == KIND_THERMAL) {
         acc_select(ctx);
@@ -1099,6 +1169,39 @@ int stz_counters(stz_ctx *ctx, long long out[6])
     out[4] = ctx->ntrans;
     out[5] = ctx->nsolve;
     return STZ_OK;
+}
+
+int stz_copy_probe(long long bytes, int reps, double *avg_ms, double *gbs)
+{
+    if (bytes < 16 * 256 || reps < 1 || !avg_ms || !gbs) return STZ_E_ARG;
+    const long long n = bytes / 16;
+    void *src = nullptr, *dst = nullptr;
+    hipStream_t s = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = STZ_E_HIP;
+    const unsigned grid = (unsigned)((n + 255) / 256);
+    float ms = 0.f;
+    if (hipMalloc(&src, n * 16) != hipSuccess || hipMalloc(&dst, n * 16) != hipSuccess) goto out;
+    if (hipStreamCreate(&s) != hipSuccess || hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+        goto out;
+    hipLaunchKernelGGL(k_fill<double>, dim3((unsigned)((2 * n + 255) / 256)), dim3(256), 0, s, (double *)src, 2 * n,
+                       1.0);
+    hipLaunchKernelGGL(k_copy16, dim3(grid), dim3(256), 0, s, (const double2 *)src, (double2 *)dst, n);   // warm
+    (void)hipEventRecord(e0, s);
+    for (int r = 0; r < reps; ++r)
+        hipLaunchKernelGGL(k_copy16, dim3(grid), dim3(256), 0, s, (const double2 *)src, (double2 *)dst, n);
+    (void)hipEventRecord(e1, s);
+    if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess) goto out;
+    *avg_ms = ms / reps;
+    *gbs = 2.0 * (double)(n * 16) / (*avg_ms * 1e-3) / 1e9;
+    rc = STZ_OK;
+out:
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (s) (void)hipStreamDestroy(s);
+    if (src) (void)hipFree(src);
+    if (dst) (void)hipFree(dst);
+    return rc;
 }
 
 }  // extern "C"

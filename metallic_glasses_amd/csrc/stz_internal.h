@@ -15,6 +15,7 @@
 #define STZ_MAXM 64
 #define STZ_BLK 256          // voxels per rate block (one per thread)
 #define STZ_GRP 256          // rate blocks per selection group
+#define STZ_LOCAL_MAXSRC 64  // sites one event-local stress update takes (larger avalanches: full solve)
 
 // control block (ctx->res, doubles unless noted), shared by kernels and host:
 //   [0..5] <sigma>  [8] R  [9] #(Q<0)  [10..12] site, mode, Q  [13] #athermal
@@ -35,6 +36,9 @@
 #define RES_K0 30             // first accepted-pair index of the step's gaussians (double)
 #define RES_PAR 31            // their parity (int)
 #define RES_RPOS 32           // stream position after the step (long long)
+// resident step: how the decided step's stress is brought up to date (ints)
+#define RES_FFT 34            // step kind when the full FFT solve runs, else KIND_NONE (the solve's gate)
+#define RES_LOCAL 35          // 1 when the event-local update runs instead
 
 // halt states of a resident replica
 #define HALT_NONE 0
@@ -98,6 +102,10 @@ struct RepState {
     long long rpos;
     long long budget;        // steps this batch may record
     int pend_comp, halt, err, nrec, printperiod, pad;
+    // event-local stress update: mean strain of the last solve, uniform increments put in
+    // the field since (FP64), updates since the last full solve
+    double ebase[6], usum[6];
+    long long nlocal;
 };
 
 // everything the resident kernels need for one replica (device memory)
@@ -116,6 +124,12 @@ struct RepDev {
     double kT, tTemp, ratepre, lnpre, voxel, thr, dt, permsoft, tempsoft, permcap, half_gamma0;
     double E, nu, mu, lambda, strainrate, strainincrement;
     int loading, lc, athermal_correction, softening;
+    // event-local stress update (kresp == nullptr: always the full solve)
+    const void *kresp;
+    long long local_resync;  // full solve after this many local updates
+    int local_max;           // largest avalanche updated locally
+    int kf[3];
+    long long kF;
 };
 
 // per-replica pointers of the batched stress solve
@@ -193,6 +207,19 @@ struct stz_ctx {
     long long step = 0, astep = 0, kstep = 0, estep = 0, ntrans = 0, nsolve = 0;
     long long avlan = 0;
     double soc_elast = 0.0;
+
+    // event-local stress update (stz_local.hip): folded impulse response [S][C][F] in the
+    // storage precision, shared by contexts of equal shape and elasticity
+    int solver_mode = STZ_SOLVER_FFT;
+    long long local_resync = 256;
+    int local_max = 8;
+    void *kresp = nullptr;
+    int *lsrc = nullptr;          // device list of update sites [STZ_LOCAL_MAXSRC]
+    int kf[3] = {1, 1, 1};
+    long long kF = 0;
+    double ebase[6] = {0, 0, 0, 0, 0, 0};   // mean strain the field was last solved with
+    double usum[6] = {0, 0, 0, 0, 0, 0};    // FP64: uniform increments added to the field since
+    long long nlocal = 1ll << 60;           // local updates since the last full solve (huge: none allowed)
 
     int tune[4] = {0, 0, 0, 0};     // solve tile choices (x fwd rows, y width, fused width, x inv rows); 0: default
     long long tune_epoch = 0;       // bumped when the choices change (captured graphs are rebuilt)
@@ -451,6 +478,14 @@ int stz_init_device_launch(stz_ctx *ctx, unsigned long long seed, const double *
                            double inc_std, long long *inside);
 // batched solve without the mean kernel (the caller reduces the *nparts partials)
 int stz_solve_launch_batched(stz_ctx *proto, int nrep, const SolveRep *reps, hipStream_t stream, int *nparts);
+
+// event-local stress update (stz_local.hip)
+int stz_local_prepare(stz_ctx *ctx);    // builds (or takes from the cache) the folded impulse response
+void stz_local_release(stz_ctx *ctx);
+void stz_local_mark_solved(stz_ctx *ctx); // a full solve rewrote the field with ctx->epsavg
+void stz_local_uniform(const stz_ctx *ctx, double *uni);
+int stz_local_launch(stz_ctx *ctx, int ns, const double *uni);   // sites in ctx->lsrc
+long long stz_local_blocks(const stz_ctx *ctx);
 
 // resident step (stz_resident.hip)
 struct StzPlan;

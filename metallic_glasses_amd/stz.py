@@ -132,6 +132,7 @@ def lib():
         L.stz_ensemble_destroy.restype = None
         L.stz_mt_window_probe.argtypes = [C.c_ulong, C.c_longlong, C.c_longlong, C.c_longlong, C.c_int, C.c_int,
                                           dp, dp, P(C.c_longlong), P(C.c_longlong)]
+        L.stz_copy_probe.argtypes = [C.c_longlong, C.c_int, dp, dp]
         _lib = L
     return _lib
 
@@ -418,6 +419,16 @@ def mt_window_probe(seed, skip, window_words, offset, with_uniform, ngauss):
     if rc:
         raise StzError(rc, "stz_mt_window_probe")
     return w[:n], s[:n], we.value, se.value
+
+
+def copy_probe(nbytes, reps=10):
+    """Plain device copy of nbytes: (ms per launch, GB/s counting read + write)."""
+    ms = C.c_double()
+    gbs = C.c_double()
+    rc = lib().stz_copy_probe(int(nbytes), int(reps), C.byref(ms), C.byref(gbs))
+    if rc:
+        raise StzError(rc, "stz_copy_probe")
+    return ms.value, gbs.value
 
 
 def soft_inclusions(n, fraction=0.05, radius=16.0, seed=1234):
