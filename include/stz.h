@@ -138,15 +138,15 @@ int stz_get_scalars(stz_ctx *ctx, double epsavg[6], double stress[6], double *to
 int stz_stress_solve(stz_ctx *ctx);
 
 /* Event-local form of the same solve, replacing InhomogeneousStressSolver(uflag=0) after
-   transforms at `n` sites (STZ/evolution.c:70-89, 250-259): eps0 has already changed by
+   transforms at `n` distinct sites (STZ/evolution.c:70-89, 250-259): eps0 has already changed by
    the transformation strains that the transforms recorded (the PreDEps0 snapshot of
    STZ/elastic.c:369-374) at those sites and nowhere else.  The solve is linear and
    translation invariant, so the new stress is the field as last solved, plus C0 times the
    change of EpsAvg since then, plus the superposed impulse responses of the sites (built
-   once per grid and elastic constants by FP64 FFT solves of unit impulses, kept for one
-   octant of offsets).  Updates the sigma field and <sigma> like stz_stress_solve, to
-   rounding.  No counterpart in the reference.  n <= 64; -3 if the field has not been
-   solved since the last upload or scalar change. */
+   once per grid and elastic constants by FP64 FFT solves of unit impulses; S*C*N values
+   in the storage precision, shared by contexts of one grid and elasticity).  Updates the sigma field and <sigma> like stz_stress_solve, to
+   rounding.  No counterpart in the reference.  n <= 64; -3 if EPS0 or SIGMA was uploaded
+   since the last full solve. */
 int stz_stress_update_local(stz_ctx *ctx, int n, const int *sites);
 
 /* How the resident step (stz_run_steps, stz_ensemble_run) brings the stress up to date

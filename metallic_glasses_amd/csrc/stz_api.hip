@@ -646,7 +646,6 @@ int stz_set_scalars(stz_ctx *ctx, const double epsavg[6], const double stress[6]
         if (stress) ctx->stress[c] = stress[c];
     }
     ctx->totime = totime;
-    if (epsavg) ctx->nlocal = 1ll << 60;      // the field no longer matches ebase: full solve first
     return STZ_OK;
 }
 
@@ -666,8 +665,11 @@ int stz_stress_solve(stz_ctx *ctx) { return ctx ? solve(ctx) : STZ_E_ARG; }
 int stz_stress_update_local(stz_ctx *ctx, int n, const int *sites)
 {
     if (!ctx || n < 0 || n > STZ_LOCAL_MAXSRC || (n > 0 && !sites)) return STZ_E_ARG;
-    for (int i = 0; i < n; ++i)
+    for (int i = 0; i < n; ++i) {
         if (sites[i] < 0 || sites[i] >= ctx->N) return STZ_E_ARG;
+        for (int j = 0; j < i; ++j)
+            if (sites[j] == sites[i]) return STZ_E_ARG;     // each site's recorded strain is added once
+    }
     if (ctx->nlocal >= (1ll << 59)) {
         ctx->err = "the stress field has not been solved since its inputs were replaced";
         return STZ_E_ARG;
@@ -717,6 +719,7 @@ int stz_set_solver(stz_ctx *ctx, int mode, long long resync, int local_max)
     ctx->solver_mode = mode;
     ctx->local_resync = resync;
     ctx->local_max = local_max;
+    ctx->tune_epoch++;                        // captured step graphs hold the launch sequence
     return STZ_OK;
 }
 

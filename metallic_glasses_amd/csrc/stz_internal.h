@@ -125,11 +125,10 @@ struct RepDev {
     double E, nu, mu, lambda, strainrate, strainincrement;
     int loading, lc, athermal_correction, softening;
     // event-local stress update (kresp == nullptr: always the full solve)
+    int n[3];
     const void *kresp;
     long long local_resync;  // full solve after this many local updates
     int local_max;           // largest avalanche updated locally
-    int kf[3];
-    long long kF;
 };
 
 // per-replica pointers of the batched stress solve
@@ -208,15 +207,13 @@ struct stz_ctx {
     long long avlan = 0;
     double soc_elast = 0.0;
 
-    // event-local stress update (stz_local.hip): folded impulse response [S][C][F] in the
+    // event-local stress update (stz_local.hip): impulse response [S][C][N] in the
     // storage precision, shared by contexts of equal shape and elasticity
     int solver_mode = STZ_SOLVER_FFT;
     long long local_resync = 256;
     int local_max = 8;
     void *kresp = nullptr;
     int *lsrc = nullptr;          // device list of update sites [STZ_LOCAL_MAXSRC]
-    int kf[3] = {1, 1, 1};
-    long long kF = 0;
     double ebase[6] = {0, 0, 0, 0, 0, 0};   // mean strain the field was last solved with
     double usum[6] = {0, 0, 0, 0, 0, 0};    // FP64: uniform increments added to the field since
     long long nlocal = 1ll << 60;           // local updates since the last full solve (huge: none allowed)

@@ -1,6 +1,6 @@
-// stz_local.hip -- event-local stress update (see stz_local_dev.h): the folded impulse
-// response, built once per (grid, elastic constants, precision) and shared by contexts,
-// and the per-call update kernel behind stz_stress_update_local.
+// stz_local.hip -- event-local stress update (see stz_local_dev.h): the impulse response,
+// built once per (grid, elastic constants, precision) and shared by contexts, and the
+// per-call update kernel behind stz_stress_update_local.
 #include "stz_local_dev.h"
 
 #include <cstring>
@@ -11,17 +11,12 @@
 
 namespace {
 
-// K[k][c][j] = (R) sigma_k[c][voxel at octant offset j] of the FP64 impulse solve k
+// K[k][c][v] = (R) sigma_k[c][v] of the FP64 impulse solve k
 template <typename R>
-__global__ void k_fold(const double *__restrict__ sig, R *__restrict__ K, LocalGeom g, int C)
+__global__ void k_keep(const double *__restrict__ sig, R *__restrict__ K, long long n)
 {
-    const long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= g.F) return;
-    const int j0 = (int)(j % g.f[0]);
-    const int j1 = (int)((j / g.f[0]) % g.f[1]);
-    const int j2 = (int)(j / ((long long)g.f[0] * g.f[1]));
-    const long long v = ((long long)j2 * g.n[1] + j1) * g.n[0] + j0;
-    for (int c = 0; c < C; ++c) K[(long long)c * g.F + j] = (R)sig[(long long)c * g.N + v];
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) K[i] = (R)sig[i];
 }
 
 struct Uni {
@@ -77,11 +72,7 @@ KKey key_of(const stz_ctx *ctx)
 LocalGeom geom_of(const stz_ctx *ctx)
 {
     LocalGeom g;
-    for (int a = 0; a < 3; ++a) {
-        g.n[a] = ctx->n[a];
-        g.f[a] = ctx->n[a] / 2 + 1;
-    }
-    g.F = (long long)g.f[0] * g.f[1] * g.f[2];
+    for (int a = 0; a < 3; ++a) g.n[a] = ctx->n[a];
     g.N = ctx->N;
     return g;
 }
@@ -97,7 +88,7 @@ LocalGeom geom_of(const stz_ctx *ctx)
     } while (0)
 
 // S FP64 impulse solves on a solver-only shadow context of the same grid and constants
-// (the stress-solve kernels read only the arrays allocated here), folded into K
+// (the stress-solve kernels read only the arrays allocated here), kept in K
 int build_response(stz_ctx *ctx, void *K)
 {
     const LocalGeom g = geom_of(ctx);
@@ -143,13 +134,14 @@ int build_response(stz_ctx *ctx, void *K)
             ctx->err = sh->err;
             goto done;
         }
-        const unsigned grid = (unsigned)((g.F + 255) / 256);
+        const long long cn = (long long)C * g.N;
+        const unsigned grid = (unsigned)((cn + 255) / 256);
         if (ctx->prec == STZ_FP64)
-            hipLaunchKernelGGL(k_fold<double>, dim3(grid), dim3(256), 0, ctx->stream, (const double *)sh->sigma,
-                               (double *)K + (size_t)k * C * g.F, g, C);
+            hipLaunchKernelGGL(k_keep<double>, dim3(grid), dim3(256), 0, ctx->stream, (const double *)sh->sigma,
+                               (double *)K + (size_t)k * cn, cn);
         else
-            hipLaunchKernelGGL(k_fold<float>, dim3(grid), dim3(256), 0, ctx->stream, (const double *)sh->sigma,
-                               (float *)K + (size_t)k * C * g.F, g, C);
+            hipLaunchKernelGGL(k_keep<float>, dim3(grid), dim3(256), 0, ctx->stream, (const double *)sh->sigma,
+                               (float *)K + (size_t)k * cn, cn);
         LHIP(hipGetLastError());
     }
     LHIP(hipStreamSynchronize(ctx->stream));
@@ -186,7 +178,7 @@ int stz_local_prepare(stz_ctx *ctx)
         ctx->kresp = it->second.k;
     } else {
         void *K = nullptr;
-        if (hipMalloc(&K, (size_t)ctx->S * ctx->C * g.F * ctx->rsz) != hipSuccess) {
+        if (hipMalloc(&K, (size_t)ctx->S * ctx->C * g.N * ctx->rsz) != hipSuccess) {
             ctx->err = "hipMalloc of the impulse response failed";
             return STZ_E_HIP;
         }
@@ -205,8 +197,6 @@ int stz_local_prepare(stz_ctx *ctx)
         ctx->err = "hipMalloc failed";
         return STZ_E_HIP;
     }
-    for (int a = 0; a < 3; ++a) ctx->kf[a] = g.f[a];
-    ctx->kF = g.F;
     return STZ_OK;
 }
 

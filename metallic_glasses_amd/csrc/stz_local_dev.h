@@ -7,16 +7,15 @@
 // only, so the re-solved stress is
 //
 //     sigma_new(x) = sigma_solved_before(x) + C0:(epsavg_now - epsavg_before)
-//                    + sum_k d_k K_k(x - s),
+//                    + sum_k d_k K_k((x - s) mod n),
 //
 // with K_k the stress response to the k-th basis eigenstrain placed at voxel 0 (all C
-// components, including the local -C0:eps0 term).  K is built once per grid and elastic
-// constants by FP64 FFT solves of the S unit impulses.  The multiplier the reference
-// applies (with its Nyquist averaging, STZ/init.c:126-129) is even and invariant under
-// the reflection of any one axis, so K is stored for one octant of offsets,
-// j_a in [0, n_a/2], and a reflected offset picks up the sign (-1)^(number of reflected
-// axes that the component pair is odd in).  Tensor axis x is grid axis 0 (the fast
-// index), y axis 1, z axis 2.
+// components, including the local -C0:eps0 term), built once per grid and elastic
+// constants by FP64 FFT solves of the S unit impulses and kept for the whole grid.
+// (The reference's k vectors use round(), STZ/init.c:126-129, which puts every Nyquist
+// component at -pi; on wavevectors with two or more Nyquist components the operator is
+// then not symmetric under the reflection of one axis, so K cannot be folded to one
+// octant of offsets without changing the result.)
 //
 // sigma_solved_before is the field minus the uniform loading increments made since the
 // last solve: in FP64 they are in the field (usum), in mixed precision they are kept as
@@ -28,31 +27,8 @@
 
 namespace {
 
-// parity masks (bit a: odd under the reflection of axis a) of the C stress components
-// and of the S stored eigenstrain components
-template <int DIM>
-__host__ __device__ constexpr int comp_mask(int c)
-{
-    if constexpr (DIM == 2) {
-        return c == 2 ? 3 : 0;                                 // xx yy xy
-    } else {
-        return c == 3 ? 6 : (c == 4 ? 5 : (c == 5 ? 3 : 0));   // xx yy zz yz xz xy
-    }
-}
-template <int DIM>
-__host__ __device__ constexpr int basis_mask(int k)
-{
-    if constexpr (DIM == 2) {
-        return k == 1 ? 3 : 0;                                 // xx(-yy), xy
-    } else {
-        return k == 2 ? 6 : (k == 3 ? 5 : (k == 4 ? 3 : 0));   // xx(-zz), yy(-zz), yz, xz, xy
-    }
-}
-
 struct LocalGeom {
     int n[3];
-    int f[3];          // n_a/2 + 1 (1 for a missing axis)
-    long long F;       // f0*f1*f2
     long long N;
 };
 
@@ -74,7 +50,7 @@ __device__ __forceinline__ void load_src(const LocalGeom &g, const R *deps_last,
     for (int k = 0; k < S; ++k) s.d[k] = deps_last[(long long)k * g.N + v];
 }
 
-// sum over sources of sum_k d_k K_k(x - s) for the C components at voxel (i0, i1, i2)
+// sum over sources of sum_k d_k K_k((x - s) mod n) for the C components at voxel (i0, i1, i2)
 template <typename R, int DIM>
 __device__ __forceinline__ void local_sum(const LocalGeom &g, const R *__restrict__ K, const LocalSrc<R> *src, int ns,
                                           int i0, int i1, int i2, R *out)
@@ -83,26 +59,16 @@ __device__ __forceinline__ void local_sum(const LocalGeom &g, const R *__restric
 #pragma unroll
     for (int c = 0; c < C; ++c) out[c] = (R)0;
     for (int s = 0; s < ns; ++s) {
-        const int ii[3] = {i0, i1, i2};
-        int j[3], rm = 0;
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            int d = ii[a] - src[s].c[a];
-            if (d < 0) d += g.n[a];
-            const bool r = 2 * d > g.n[a];
-            j[a] = r ? g.n[a] - d : d;
-            rm |= (int)r << a;
-        }
-        const long long jj = ((long long)j[2] * g.f[1] + j[1]) * g.f[0] + j[0];
+        int d0 = i0 - src[s].c[0], d1 = i1 - src[s].c[1], d2 = i2 - src[s].c[2];
+        if (d0 < 0) d0 += g.n[0];
+        if (d1 < 0) d1 += g.n[1];
+        if (d2 < 0) d2 += g.n[2];
+        const long long jj = ((long long)d2 * g.n[1] + d1) * g.n[0] + d0;
 #pragma unroll
         for (int k = 0; k < S; ++k) {
             const R dk = src[s].d[k];
 #pragma unroll
-            for (int c = 0; c < C; ++c) {
-                const R kv = K[((long long)k * C + c) * g.F + jj];
-                const bool neg = __popc((comp_mask<DIM>(c) ^ basis_mask<DIM>(k)) & rm) & 1;
-                out[c] = out[c] + (neg ? -dk : dk) * kv;
-            }
+            for (int c = 0; c < C; ++c) out[c] = out[c] + dk * K[((long long)k * C + c) * g.N + jj];
         }
     }
 }

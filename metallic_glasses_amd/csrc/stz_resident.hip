@@ -24,6 +24,7 @@
 // batches.  The step sequence is launched directly or replayed from a hipGraph that
 // holds a fixed number of consecutive steps.
 #include "stz_kmc_dev.h"
+#include "stz_local_dev.h"
 
 #include <algorithm>
 #include <cmath>
@@ -205,7 +206,53 @@ __global__ __launch_bounds__(NT) void kb_decide(const RepDev *__restrict__ reps,
         const bool ok = select_block<R, DIM>(a, d.gsum, ngrp, s_tau, res, ap, src, d.half_gamma0);
         if (!ok) kind = KIND_FAIL;
     }
-    if (tid == 0) reinterpret_cast<int *>(res + RES_KIND)[0] = kind;
+    if (tid == 0) {
+        // stress after the transforms: the event-local update for a thermal event or a small
+        // avalanche while the field is fresh enough, else the full solve
+        const bool solved = kind == KIND_THERMAL || kind == KIND_ATHERMAL;
+        int local = 0;
+        if (solved && d.kresp && st->nlocal < d.local_resync)
+            local = (kind == KIND_THERMAL || (long long)res[13] <= d.local_max) ? 1 : 0;
+        reinterpret_cast<int *>(res + RES_LOCAL)[0] = local;
+        reinterpret_cast<int *>(res + RES_FFT)[0] = (solved && !local) ? kind : KIND_NONE;
+        reinterpret_cast<int *>(res + RES_KIND)[0] = kind;
+    }
+}
+
+// event-local stress update of the replicas whose step decided it (stz_local_dev.h): the
+// chosen site of a thermal event or the avalanche list, each with the strain it recorded
+template <typename R, int DIM>
+__global__ __launch_bounds__(NT_LOCAL) void kb_local(const RepDev *__restrict__ reps, long long span, int fp64)
+{
+    constexpr int C = Comp<DIM>::C;
+    __shared__ LocalSrc<R> src[STZ_LOCAL_MAXSRC];
+    __shared__ double red[NT_LOCAL];
+    __shared__ double uni[6];
+    const RepDev &d = reps[blockIdx.y];
+    const RepState *st = d.st;
+    double *res = d.res;
+    if (st->halt || reinterpret_cast<const int *>(res + RES_LOCAL)[0] != 1) return;
+    LocalGeom g;
+    for (int a = 0; a < 3; ++a) g.n[a] = d.n[a];
+    g.N = (long long)d.n[0] * d.n[1] * d.n[2];
+    const bool thermal = reinterpret_cast<const int *>(res + RES_KIND)[0] == KIND_THERMAL;
+    const int ns = thermal ? 1 : (int)res[13];
+    for (int s = threadIdx.x; s < ns; s += NT_LOCAL) {
+        const long long v = thermal ? (long long)res[10] : (long long)d.alist[s];
+        load_src<R, DIM>(g, (const R *)d.deps_last, v, src[s]);
+    }
+    if (threadIdx.x == 0) {
+        double de[6], u[6];
+        for (int c = 0; c < 6; ++c) {
+            de[c] = st->epsavg[c] - st->ebase[c];
+            u[c] = 0.0;
+        }
+        c0_mul<C>(de, d.lambda, d.mu, u);
+        for (int c = 0; c < 6; ++c) uni[c] = fp64 ? u[c] - st->usum[c] : u[c];
+    }
+    __syncthreads();
+    local_block<R, DIM>(g, (const R *)d.kresp, src, ns, uni, (R *)d.sigma, blockIdx.x, span, d.spart,
+                        reinterpret_cast<int *>(res + 15), red);
 }
 
 template <typename R, int DIM>
@@ -259,6 +306,7 @@ __device__ void add_uniform_d(RepState *s, bool fp64, int comp, double delta)
     if (fp64) {
         s->pend_comp = comp;
         s->pend_delta = delta;
+        s->usum[comp] += delta;      // in the field once the next rate pass applies it
     } else {
         s->soff[comp] += delta;
     }
@@ -351,7 +399,7 @@ __device__ __forceinline__ void publish_halt(const RepDev &d, int h)
 // reductions), then the scalar updates on a private copy of the run state (one thread)
 template <int DIM>
 __global__ __launch_bounds__(STZ_NT_MEAN) void kb_finish(const RepDev *__restrict__ reps, int fp64, int nparts,
-                                                         double invN, long long nblk)
+                                                         int nparts_local, double invN, long long nblk)
 {
     constexpr int C = Comp<DIM>::C;
     __shared__ double mean[C];
@@ -362,10 +410,15 @@ __global__ __launch_bounds__(STZ_NT_MEAN) void kb_finish(const RepDev *__restric
     int *kindp = reinterpret_cast<int *>(res + RES_KIND);
     const int kind = kindp[0];
     const bool solved = kind == KIND_THERMAL || kind == KIND_ATHERMAL;
-    if (solved) mean_partials<C>(d.spart, nparts, invN, mean);
+    const int local = reinterpret_cast<const int *>(res + RES_LOCAL)[0];
+    if (solved) mean_partials<C>(d.spart, local ? nparts_local : nparts, invN, mean);
     double soc = 0.0;
     if (kind == KIND_ATHERMAL) soc = ordered_sum<STZ_NT_MEAN>(d.socp, nblk);
     if (threadIdx.x != 0) return;
+    // the solve gates of this step are spent (every thread has read them: the reductions
+    // above end in barriers, and without a reduction no other thread uses them)
+    reinterpret_cast<int *>(res + RES_FFT)[0] = KIND_NONE;
+    reinterpret_cast<int *>(res + RES_LOCAL)[0] = 0;
     RepState s = *sp;
     if (kind == KIND_STALL) {
         sp->halt = HALT_RNG;
@@ -402,7 +455,12 @@ __global__ __launch_bounds__(STZ_NT_MEAN) void kb_finish(const RepDev *__restric
     if (solved) {
         s.nsolve++;
         for (int c = 0; c < C; ++c) s.stress[c] = mean[c];
-        for (int c = 0; c < 6; ++c) s.soff[c] = 0.0;          // the solve rewrote the stress field
+        for (int c = 0; c < 6; ++c) {
+            s.soff[c] = 0.0;                  // the solve (or update) rewrote the stress field
+            s.ebase[c] = s.epsavg[c];         // with the mean strain of this step
+            s.usum[c] = 0.0;
+        }
+        s.nlocal = local ? s.nlocal + 1 : 0;
         s.rpos = reinterpret_cast<const long long *>(res + RES_RPOS)[0];
     }
     if (kind == KIND_THERMAL) {
@@ -497,6 +555,7 @@ namespace {
 
 struct StzClass {
     std::vector<stz_ctx *> reps;
+    bool local = false;               // some replica updates the stress event-locally
     RepDev *d_reps = nullptr;
     SolveRep *d_solve = nullptr;
 };
@@ -659,6 +718,10 @@ void fill_desc(stz_ctx *ctx)
     d.lc = ctx->lc;
     d.athermal_correction = ctx->p.athermal_correction;
     d.softening = ctx->p.softening == 1;
+    for (int a = 0; a < 3; ++a) d.n[a] = ctx->n[a];
+    d.kresp = ctx->solver_mode == STZ_SOLVER_LOCAL ? ctx->kresp : nullptr;
+    d.local_resync = ctx->local_resync;
+    d.local_max = std::min(ctx->local_max, STZ_LOCAL_MAXSRC);
 }
 
 // host scalars -> device run state; positions the lookup window at the stream position
@@ -708,14 +771,21 @@ int push_state(stz_ctx *ctx, long long budget)
     s.ntrans = ctx->ntrans;
     s.nsolve = ctx->nsolve;
     s.avlan = ctx->avlan;
+    for (int c = 0; c < 6; ++c) {
+        s.ebase[c] = ctx->ebase[c];
+        s.usum[c] = ctx->usum[c];
+    }
+    s.nlocal = ctx->nlocal;
     s.rpos = pos;
     s.budget = std::min<long long>(budget, rs->rec_cap);
     s.halt = budget > 0 ? HALT_NONE : HALT_BUDGET;
     s.nrec = 0;
     RS_HIP(ctx, hipMemcpyAsync(rs->st, &s, sizeof(s), hipMemcpyHostToDevice, ctx->stream));
     __atomic_store_n(rs->hflag, s.halt, __ATOMIC_RELAXED);
-    const int none = KIND_NONE;
-    RS_HIP(ctx, hipMemcpyAsync(ctx->res + RES_KIND, &none, sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+    const int none[2] = {KIND_NONE, 0};
+    RS_HIP(ctx, hipMemcpyAsync(ctx->res + RES_KIND, &none[0], sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+    RS_HIP(ctx, hipMemcpyAsync(ctx->res + RES_FFT, &none[0], sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+    RS_HIP(ctx, hipMemcpyAsync(ctx->res + RES_LOCAL, &none[1], sizeof(int), hipMemcpyHostToDevice, ctx->stream));
     fill_desc(ctx);
     RS_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return STZ_OK;
@@ -743,6 +813,11 @@ void pull_state(stz_ctx *ctx)
     ctx->ntrans = s.ntrans;
     ctx->nsolve = s.nsolve;
     ctx->avlan = s.avlan;
+    for (int c = 0; c < 6; ++c) {
+        ctx->ebase[c] = s.ebase[c];
+        ctx->usum[c] = s.usum[c];
+    }
+    ctx->nlocal = s.nlocal;
     if (!(rs->rng_lazy == false && ctx->rng.count == s.rpos)) {
         rs->rng_lazy = true;
         rs->rng_target = s.rpos;
@@ -772,7 +847,13 @@ int launch_class(StzPlan *pl, StzClass &k, hipEvent_t ea, hipEvent_t eb)
     int rc = stz_solve_launch_batched(p, nrep, k.d_solve, s, &nparts);
     p->timing = saved;
     if (rc) return rc;
-    hipLaunchKernelGGL((kb_finish<DIM>), dim3(nrep), dim3(STZ_NT_MEAN), 0, s, k.d_reps, (int)(p->prec == STZ_FP64), nparts,
+    const int fp64 = p->prec == STZ_FP64;
+    const long long nlb = stz_local_blocks(p);
+    if (k.local) {
+        const long long span = p->n[0] > NT_LOCAL ? p->n[0] : NT_LOCAL;
+        hipLaunchKernelGGL((kb_local<R, DIM>), dim3((unsigned)nlb, nrep), dim3(NT_LOCAL), 0, s, k.d_reps, span, fp64);
+    }
+    hipLaunchKernelGGL((kb_finish<DIM>), dim3(nrep), dim3(STZ_NT_MEAN), 0, s, k.d_reps, fp64, nparts, (int)nlb,
                        1.0 / (double)p->N, p->nblk);
     return hipGetLastError() == hipSuccess ? STZ_OK : STZ_E_HIP;
 }
@@ -848,8 +929,10 @@ int plan_upload(StzPlan *pl)
     for (StzClass &k : pl->cls) {
         std::vector<RepDev> d;
         std::vector<SolveRep> s;
+        k.local = false;
         for (stz_ctx *c : k.reps) {
             d.push_back(c->rs->hdev);
+            k.local = k.local || c->rs->hdev.kresp != nullptr;
             SolveRep q;
             q.eps0 = c->eps0;
             q.spec = c->spec;

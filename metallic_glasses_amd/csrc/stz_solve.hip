@@ -67,7 +67,7 @@ __device__ __forceinline__ View<R> view_of(const SolveRep *reps, int r, const So
         v.sigma = (R *)q.sigma;
         v.spart = q.spart;
         v.nanflag = reinterpret_cast<int *>(q.res + 15);
-        v.gate = reinterpret_cast<const int *>(q.res + RES_KIND);
+        v.gate = reinterpret_cast<const int *>(q.res + RES_FFT);   // KIND_NONE when the step updates locally
 #pragma unroll
         for (int c = 0; c < 6; ++c) v.ea[c] = q.epsavg[c];
     } else {

@@ -16,6 +16,7 @@ LIB_PATH = os.path.join(PKG, "libstz.so")
 
 FP64, MIXED = 0, 1
 TENSION, SHEAR = 0, 1
+SOLVER_FFT, SOLVER_LOCAL = 0, 1
 F_EPS0, F_SIGMA, F_Q0, F_DEPS, F_DEPS_LAST, F_PERM, F_TEMP, F_TLAST, F_MINQ = range(9)
 ELASTIC, THERMAL, ATHERMAL = 0, 1, 2
 ERRORS = {-1: "selection failed", -2: "HIP error", -3: "bad argument", -4: "NaN in stress solve",
@@ -133,6 +134,8 @@ def lib():
         L.stz_mt_window_probe.argtypes = [C.c_ulong, C.c_longlong, C.c_longlong, C.c_longlong, C.c_int, C.c_int,
                                           dp, dp, P(C.c_longlong), P(C.c_longlong)]
         L.stz_copy_probe.argtypes = [C.c_longlong, C.c_int, dp, dp]
+        L.stz_stress_update_local.argtypes = [vp, C.c_int, P(C.c_int)]
+        L.stz_set_solver.argtypes = [vp, C.c_int, C.c_longlong, C.c_int]
         _lib = L
     return _lib
 
@@ -228,6 +231,15 @@ class Stz:
 
     def stress_solve(self):
         self._check(lib().stz_stress_solve(self._h))
+
+    def stress_update_local(self, sites):
+        """Event-local stress update after transforms at `sites` (see include/stz.h)."""
+        a = (C.c_int * max(len(sites), 1))(*[int(x) for x in sites])
+        self._check(lib().stz_stress_update_local(self._h, len(sites), a))
+
+    def set_solver(self, mode, resync=256, local_max=8):
+        """Stress update of the resident step: SOLVER_FFT or SOLVER_LOCAL."""
+        self._check(lib().stz_set_solver(self._h, int(mode), int(resync), int(local_max)))
 
     def rates(self):
         n = C.c_longlong()
