@@ -153,7 +153,9 @@ int stz_stress_update_local(stz_ctx *ctx, int n, const int *sites);
    after transforms: STZ_SOLVER_FFT, or STZ_SOLVER_LOCAL = the event-local update for a
    thermal event and for avalanches of at most local_max voxels, with a full solve after
    every `resync` local updates, for larger avalanches and at print steps.  No
-   counterpart in the reference.  Defaults: FFT, 256, 8. */
+   counterpart in the reference.  Defaults: FFT, 256, 2 (each further site costs the
+   update another full read of the S*C response fields, so beyond a few sites the full
+   solve is cheaper). */
 int stz_set_solver(stz_ctx *ctx, int mode, long long resync, int local_max);
 
 /* SofteningFactor + ActEnergy + Q<0 count + KMC rate total
@@ -205,7 +207,8 @@ int stz_step_print(stz_ctx *ctx, stz_step_result *res);
    the context's MT19937 stream by index, so the results equal those of nsteps calls of
    stz_step.  Print steps are re-solved on the host as in stz_step.  Stops early on
    termination (return 1) or error (< 0).  results (optional) receives one entry per
-   completed step; done receives the number of completed steps. */
+   completed step; done receives the number of completed steps.  nsteps = 0 only
+   prepares the resident resources and the random-number lookup window. */
 int stz_run_steps(stz_ctx *ctx, int nsteps, stz_step_result *results, int *done);
 
 /* Resident-run settings: steps per host synchronisation (batch, 0 = keep; default 64),

@@ -211,7 +211,7 @@ struct stz_ctx {
     // storage precision, shared by contexts of equal shape and elasticity
     int solver_mode = STZ_SOLVER_FFT;
     long long local_resync = 256;
-    int local_max = 8;
+    int local_max = 2;
     void *kresp = nullptr;
     int *lsrc = nullptr;          // device list of update sites [STZ_LOCAL_MAXSRC]
     double ebase[6] = {0, 0, 0, 0, 0, 0};   // mean strain the field was last solved with

@@ -345,8 +345,9 @@ struct ApplyArgs {
 };
 
 #pragma clang fp contract(off)
+// everything but the redraw of the M*S new transformation strains
 template <typename R, int DIM>
-__device__ void transform_voxel(const ApplyArgs &a, long long v, int mm, const double *dnew, double dscale)
+__device__ void transform_voxel_state(const ApplyArgs &a, long long v, int mm)
 {
     constexpr int C = Comp<DIM>::C, S = Comp<DIM>::S;
     R *deps = (R *)a.deps;
@@ -369,6 +370,14 @@ __device__ void transform_voxel(const ApplyArgs &a, long long v, int mm, const d
     a.temp[v] = ds * a.tempsoft;                                                      // :396
     a.tlast[v] = a.totime;                                                            // :397
     ((R *)a.scache)[v] = (R)__builtin_nan("");                                        // Q0*exp(-D*S) must be recomputed
+}
+
+template <typename R, int DIM>
+__device__ void transform_voxel(const ApplyArgs &a, long long v, int mm, const double *dnew, double dscale)
+{
+    constexpr int S = Comp<DIM>::S;
+    R *deps = (R *)a.deps;
+    transform_voxel_state<R, DIM>(a, v, mm);
     for (int m = 0; m < a.M; ++m)
 #pragma unroll
         for (int k = 0; k < S; ++k)
@@ -470,7 +479,19 @@ __device__ bool select_block(const RateArgs<R> &a, const double *gsum, long long
         res[10] = (double)(blk * NT + f0);
         res[11] = (double)m;
         res[12] = (double)q;
-        if (dnew) transform_voxel<R, DIM>(ap, blk * NT + f0, m, dnew, dscale);   // VoxelShearTransform right away
+        if (dnew) transform_voxel_state<R, DIM>(ap, blk * NT + f0, m);   // VoxelShearTransform right away
+    }
+    if (dnew) {
+        // the redraw of all M modes (STZ/elastic.c:508-512), one value per thread, after the
+        // chosen mode's strain has been snapshotted above
+        constexpr int S = Comp<DIM>::S;
+        __syncthreads();
+        R *deps = (R *)ap.deps;
+        const long long vv = blk * NT + f0;
+        for (int i = tid; i < a.M * S; i += NT) {
+#pragma clang fp contract(off)
+            deps[(long long)i * a.N + vv] = (R)(dscale * dnew[i]);
+        }
     }
     return true;
 }

@@ -26,13 +26,13 @@ struct Uni {
 template <typename R, int DIM>
 __global__ __launch_bounds__(NT_LOCAL) void k_local(LocalGeom g, const R *__restrict__ K, const int *__restrict__ sites,
                                                     int ns, const R *__restrict__ deps_last, R *__restrict__ sigma,
-                                                    Uni uni, long long span, double *__restrict__ spart, int *nanflag)
+                                                    Uni uni, double *__restrict__ spart, int *nanflag)
 {
     __shared__ LocalSrc<R> src[STZ_LOCAL_MAXSRC];
-    __shared__ double red[NT_LOCAL];
+    __shared__ double wred[NW_LOCAL * 6];
     for (int s = threadIdx.x; s < ns; s += NT_LOCAL) load_src<R, DIM>(g, deps_last, sites[s], src[s]);
     __syncthreads();
-    local_block<R, DIM>(g, K, src, ns, uni.u, sigma, blockIdx.x, span, spart, nanflag, red);
+    local_block<R, DIM>(g, K, src, ns, uni.u, sigma, blockIdx.x, spart, nanflag, wred);
 }
 
 template <int C>
@@ -73,7 +73,7 @@ LocalGeom geom_of(const stz_ctx *ctx)
 {
     LocalGeom g;
     for (int a = 0; a < 3; ++a) g.n[a] = ctx->n[a];
-    g.N = ctx->N;
+    g.N = (int)ctx->N;
     return g;
 }
 
@@ -169,6 +169,10 @@ done:
 int stz_local_prepare(stz_ctx *ctx)
 {
     if (ctx->kresp) return STZ_OK;
+    if (ctx->N >= (1ll << 31)) {
+        ctx->err = "the event-local update needs fewer than 2^31 voxels";
+        return STZ_E_ARG;
+    }
     const LocalGeom g = geom_of(ctx);
     std::lock_guard<std::mutex> lk(g_kmu);
     const KKey key = key_of(ctx);
@@ -246,8 +250,7 @@ void stz_local_uniform(const stz_ctx *ctx, double *uni)
 int stz_local_launch(stz_ctx *ctx, int ns, const double *uni)
 {
     LocalGeom g = geom_of(ctx);
-    const long long span = ctx->n[0] > NT_LOCAL ? ctx->n[0] : NT_LOCAL;
-    const long long nb = (ctx->N + span - 1) / span;
+    const long long nb = stz_local_blocks(ctx);
     Uni u;
     for (int c = 0; c < 6; ++c) u.u[c] = uni[c];
     int *nanflag = reinterpret_cast<int *>(ctx->res + 15);
@@ -255,20 +258,20 @@ int stz_local_launch(stz_ctx *ctx, int ns, const double *uni)
     if (ctx->prec == STZ_FP64) {
         if (ctx->dim == 2)
             hipLaunchKernelGGL((k_local<double, 2>), grid, dim3(NT_LOCAL), 0, ctx->stream, g, (const double *)ctx->kresp,
-                               ctx->lsrc, ns, (const double *)ctx->deps_last, (double *)ctx->sigma, u, span, ctx->spart,
+                               ctx->lsrc, ns, (const double *)ctx->deps_last, (double *)ctx->sigma, u, ctx->spart,
                                nanflag);
         else
             hipLaunchKernelGGL((k_local<double, 3>), grid, dim3(NT_LOCAL), 0, ctx->stream, g, (const double *)ctx->kresp,
-                               ctx->lsrc, ns, (const double *)ctx->deps_last, (double *)ctx->sigma, u, span, ctx->spart,
+                               ctx->lsrc, ns, (const double *)ctx->deps_last, (double *)ctx->sigma, u, ctx->spart,
                                nanflag);
     } else {
         if (ctx->dim == 2)
             hipLaunchKernelGGL((k_local<float, 2>), grid, dim3(NT_LOCAL), 0, ctx->stream, g, (const float *)ctx->kresp,
-                               ctx->lsrc, ns, (const float *)ctx->deps_last, (float *)ctx->sigma, u, span, ctx->spart,
+                               ctx->lsrc, ns, (const float *)ctx->deps_last, (float *)ctx->sigma, u, ctx->spart,
                                nanflag);
         else
             hipLaunchKernelGGL((k_local<float, 3>), grid, dim3(NT_LOCAL), 0, ctx->stream, g, (const float *)ctx->kresp,
-                               ctx->lsrc, ns, (const float *)ctx->deps_last, (float *)ctx->sigma, u, span, ctx->spart,
+                               ctx->lsrc, ns, (const float *)ctx->deps_last, (float *)ctx->sigma, u, ctx->spart,
                                nanflag);
     }
     STZ_CHECK_LAUNCH();
@@ -282,9 +285,9 @@ int stz_local_launch(stz_ctx *ctx, int ns, const double *uni)
     return STZ_OK;
 }
 
-// workgroups of the update over one grid (partials the mean reduces)
+// workgroups of the update over one grid (partials the mean reduces; at most n1*n2, the
+// rows of the partial array, since a workgroup spans at least one x row)
 long long stz_local_blocks(const stz_ctx *ctx)
 {
-    const long long span = ctx->n[0] > NT_LOCAL ? ctx->n[0] : NT_LOCAL;
-    return (ctx->N + span - 1) / span;
+    return (ctx->N + SPAN_LOCAL - 1) / SPAN_LOCAL;
 }

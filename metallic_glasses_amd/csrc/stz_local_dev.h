@@ -29,7 +29,7 @@ namespace {
 
 struct LocalGeom {
     int n[3];
-    long long N;
+    int N;             // below 2^31 (checked when the response is built)
 };
 
 // one source of an update: its grid coordinates and stored eigenstrain increment
@@ -40,12 +40,12 @@ struct LocalSrc {
 };
 
 template <typename R, int DIM>
-__device__ __forceinline__ void load_src(const LocalGeom &g, const R *deps_last, long long v, LocalSrc<R> &s)
+__device__ __forceinline__ void load_src(const LocalGeom &g, const R *deps_last, int v, LocalSrc<R> &s)
 {
     constexpr int S = DIM == 2 ? 2 : 5;
-    s.c[0] = (int)(v % g.n[0]);
-    s.c[1] = (int)((v / g.n[0]) % g.n[1]);
-    s.c[2] = (int)(v / ((long long)g.n[0] * g.n[1]));
+    s.c[0] = v % g.n[0];
+    s.c[1] = (v / g.n[0]) % g.n[1];
+    s.c[2] = v / (g.n[0] * g.n[1]);
 #pragma unroll
     for (int k = 0; k < S; ++k) s.d[k] = deps_last[(long long)k * g.N + v];
 }
@@ -63,7 +63,7 @@ __device__ __forceinline__ void local_sum(const LocalGeom &g, const R *__restric
         if (d0 < 0) d0 += g.n[0];
         if (d1 < 0) d1 += g.n[1];
         if (d2 < 0) d2 += g.n[2];
-        const long long jj = ((long long)d2 * g.n[1] + d1) * g.n[0] + d0;
+        const int jj = (d2 * g.n[1] + d1) * g.n[0] + d0;
 #pragma unroll
         for (int k = 0; k < S; ++k) {
             const R dk = src[s].d[k];
@@ -90,47 +90,80 @@ __host__ __device__ inline void c0_mul(const double *e, double lam, double mu, d
 }
 
 constexpr int NT_LOCAL = 256;
+constexpr int VPT_LOCAL = 4;                          // voxels per thread
+constexpr int SPAN_LOCAL = NT_LOCAL * VPT_LOCAL;      // voxels per workgroup
+constexpr int NW_LOCAL = NT_LOCAL / 64;
 
-// Updates the voxels [blk*span, min(N, (blk+1)*span)) of one workgroup: sigma += uni +
-// sum over sources, and writes the workgroup's FP64 partial sums of the new stress to
-// spart[blk][c].  Sources must be in `src` (shared memory) already.
-template <typename R, int DIM>
-__device__ void local_block(const LocalGeom &g, const R *__restrict__ K, const LocalSrc<R> *src, int ns,
-                           const double *uni, R *sigma, long long blk, long long span, double *spart, int *nanflag,
-                           double *red)
+__device__ __forceinline__ double wsum_local(double x)
+{
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) x = x + __shfl_xor(x, d, 64);
+    return x;
+}
+
+// one voxel: sigma += uni + the shifted responses of the sources; returns the new value
+// of each component in FP64 (before rounding to the storage precision)
+template <typename R, int DIM, bool ONE>
+__device__ __forceinline__ void local_voxel(const LocalGeom &g, const R *__restrict__ K, const LocalSrc<R> *src,
+                                            const LocalSrc<R> &s0, int ns, const double *uni, R *sigma, int v,
+                                            double *acc, int &bad)
 {
     constexpr int C = DIM == 2 ? 3 : 6;
-    const int tid = threadIdx.x;
+    const int i0 = v % g.n[0];
+    const int r = v / g.n[0];
+    const int i1 = r % g.n[1];
+    const int i2 = r / g.n[1];
+    R ds[C];
+    if constexpr (ONE) local_sum<R, DIM>(g, K, &s0, 1, i0, i1, i2, ds);
+    else local_sum<R, DIM>(g, K, src, ns, i0, i1, i2, ds);
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+        const double s = ((double)sigma[(long long)c * g.N + v] + uni[c]) + (double)ds[c];
+        bad |= (s != s);
+        sigma[(long long)c * g.N + v] = (R)s;
+        acc[c] += s;
+    }
+}
+
+// Updates the voxels [blk*SPAN_LOCAL, ...) of one workgroup (VPT_LOCAL per thread,
+// consecutive lanes on consecutive voxels) and writes the workgroup's FP64 partial sums
+// of the new stress, in a fixed order, to spart[blk][c].  Sources must be in `src`
+// (shared memory) already; wred holds NW_LOCAL*6 doubles.
+template <typename R, int DIM>
+__device__ void local_block(const LocalGeom &g, const R *__restrict__ K, const LocalSrc<R> *src, int ns,
+                           const double *uni, R *sigma, long long blk, double *spart, int *nanflag, double *wred)
+{
+    constexpr int C = DIM == 2 ? 3 : 6;
+    const int tid = threadIdx.x, lane = tid % 64, w = tid / 64;
     double acc[C];
 #pragma unroll
     for (int c = 0; c < C; ++c) acc[c] = 0.0;
     int bad = 0;
-    const long long v0 = blk * span;
-    const long long v1 = v0 + span < g.N ? v0 + span : g.N;
-    for (long long v = v0 + tid; v < v1; v += NT_LOCAL) {
-        const int i0 = (int)(v % g.n[0]);
-        const int i1 = (int)((v / g.n[0]) % g.n[1]);
-        const int i2 = (int)(v / ((long long)g.n[0] * g.n[1]));
-        R ds[C];
-        local_sum<R, DIM>(g, K, src, ns, i0, i1, i2, ds);
+    const int v0 = (int)blk * SPAN_LOCAL;
+    if (ns == 1) {
+        const LocalSrc<R> s0 = src[0];
 #pragma unroll
-        for (int c = 0; c < C; ++c) {
-            const double s = ((double)sigma[(long long)c * g.N + v] + uni[c]) + (double)ds[c];
-            bad |= (s != s);
-            sigma[(long long)c * g.N + v] = (R)s;
-            acc[c] += s;
+        for (int t = 0; t < VPT_LOCAL; ++t) {
+            const int v = v0 + t * NT_LOCAL + tid;
+            if (v < g.N) local_voxel<R, DIM, true>(g, K, src, s0, 1, uni, sigma, v, acc, bad);
+        }
+    } else {
+        for (int t = 0; t < VPT_LOCAL; ++t) {
+            const int v = v0 + t * NT_LOCAL + tid;
+            if (v < g.N) local_voxel<R, DIM, false>(g, K, src, src[0], ns, uni, sigma, v, acc, bad);
         }
     }
     if (bad) atomicOr(nanflag, 1);
+#pragma unroll
     for (int c = 0; c < C; ++c) {
-        red[tid] = acc[c];
-        __syncthreads();
-        for (int w = NT_LOCAL / 2; w > 0; w >>= 1) {
-            if (tid < w) red[tid] = red[tid] + red[tid + w];
-            __syncthreads();
-        }
-        if (tid == 0) spart[blk * C + c] = red[0];
-        __syncthreads();
+        const double x = wsum_local(acc[c]);
+        if (lane == 0) wred[w * 6 + c] = x;
+    }
+    __syncthreads();
+    if (tid < C) {
+        double t = wred[tid];
+        for (int i = 1; i < NW_LOCAL; ++i) t = t + wred[i * 6 + tid];
+        spart[blk * C + tid] = t;
     }
 }
 

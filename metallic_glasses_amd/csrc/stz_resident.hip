@@ -222,11 +222,11 @@ __global__ __launch_bounds__(NT) void kb_decide(const RepDev *__restrict__ reps,
 // event-local stress update of the replicas whose step decided it (stz_local_dev.h): the
 // chosen site of a thermal event or the avalanche list, each with the strain it recorded
 template <typename R, int DIM>
-__global__ __launch_bounds__(NT_LOCAL) void kb_local(const RepDev *__restrict__ reps, long long span, int fp64)
+__global__ __launch_bounds__(NT_LOCAL) void kb_local(const RepDev *__restrict__ reps, int fp64)
 {
     constexpr int C = Comp<DIM>::C;
     __shared__ LocalSrc<R> src[STZ_LOCAL_MAXSRC];
-    __shared__ double red[NT_LOCAL];
+    __shared__ double wred[NW_LOCAL * 6];
     __shared__ double uni[6];
     const RepDev &d = reps[blockIdx.y];
     const RepState *st = d.st;
@@ -234,11 +234,11 @@ __global__ __launch_bounds__(NT_LOCAL) void kb_local(const RepDev *__restrict__ 
     if (st->halt || reinterpret_cast<const int *>(res + RES_LOCAL)[0] != 1) return;
     LocalGeom g;
     for (int a = 0; a < 3; ++a) g.n[a] = d.n[a];
-    g.N = (long long)d.n[0] * d.n[1] * d.n[2];
+    g.N = d.n[0] * d.n[1] * d.n[2];
     const bool thermal = reinterpret_cast<const int *>(res + RES_KIND)[0] == KIND_THERMAL;
     const int ns = thermal ? 1 : (int)res[13];
     for (int s = threadIdx.x; s < ns; s += NT_LOCAL) {
-        const long long v = thermal ? (long long)res[10] : (long long)d.alist[s];
+        const int v = thermal ? (int)res[10] : d.alist[s];
         load_src<R, DIM>(g, (const R *)d.deps_last, v, src[s]);
     }
     if (threadIdx.x == 0) {
@@ -251,8 +251,8 @@ __global__ __launch_bounds__(NT_LOCAL) void kb_local(const RepDev *__restrict__ 
         for (int c = 0; c < 6; ++c) uni[c] = fp64 ? u[c] - st->usum[c] : u[c];
     }
     __syncthreads();
-    local_block<R, DIM>(g, (const R *)d.kresp, src, ns, uni, (R *)d.sigma, blockIdx.x, span, d.spart,
-                        reinterpret_cast<int *>(res + 15), red);
+    local_block<R, DIM>(g, (const R *)d.kresp, src, ns, uni, (R *)d.sigma, blockIdx.x, d.spart,
+                        reinterpret_cast<int *>(res + 15), wred);
 }
 
 template <typename R, int DIM>
@@ -849,10 +849,8 @@ int launch_class(StzPlan *pl, StzClass &k, hipEvent_t ea, hipEvent_t eb)
     if (rc) return rc;
     const int fp64 = p->prec == STZ_FP64;
     const long long nlb = stz_local_blocks(p);
-    if (k.local) {
-        const long long span = p->n[0] > NT_LOCAL ? p->n[0] : NT_LOCAL;
-        hipLaunchKernelGGL((kb_local<R, DIM>), dim3((unsigned)nlb, nrep), dim3(NT_LOCAL), 0, s, k.d_reps, span, fp64);
-    }
+    if (k.local)
+        hipLaunchKernelGGL((kb_local<R, DIM>), dim3((unsigned)nlb, nrep), dim3(NT_LOCAL), 0, s, k.d_reps, fp64);
     hipLaunchKernelGGL((kb_finish<DIM>), dim3(nrep), dim3(STZ_NT_MEAN), 0, s, k.d_reps, fp64, nparts, (int)nlb,
                        1.0 / (double)p->N, p->nblk);
     return hipGetLastError() == hipSuccess ? STZ_OK : STZ_E_HIP;
@@ -1282,7 +1280,10 @@ int stz_run_steps(stz_ctx *ctx, int nsteps, stz_step_result *results, int *done)
 {
     if (!ctx || nsteps < 0) return STZ_E_ARG;
     int d = 0, rc = 0;
-    if (nsteps > 0) {
+    if (nsteps == 0) {                  // prepare: resident resources and the lookup window
+        rc = ensure_resident(ctx);
+        if (!rc) rc = push_state(ctx, 0);
+    } else {
         StzPlan *pl = stz_self_plan(ctx);
         if (!pl) return STZ_E_HIP;
         stz_step_result *const rr[1] = {results};
