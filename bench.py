@@ -40,8 +40,8 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--seed", type=int, default=1234)
     ap.add_argument("--timing-steps", type=int, default=200)
-    ap.add_argument("--resync", type=int, default=256, help="local updates between full FFT re-solves")
-    ap.add_argument("--local-max", type=int, default=2, help="largest avalanche updated locally")
+    ap.add_argument("--resync", type=int, default=1024, help="local updates between full FFT re-solves")
+    ap.add_argument("--local-max", type=int, default=16, help="largest avalanche updated locally")
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--ensemble-replicas", type=int, default=64)
     ap.add_argument("--ensemble-steps", type=int, default=200)

@@ -143,19 +143,21 @@ int stz_stress_solve(stz_ctx *ctx);
    STZ/elastic.c:369-374) at those sites and nowhere else.  The solve is linear and
    translation invariant, so the new stress is the field as last solved, plus C0 times the
    change of EpsAvg since then, plus the superposed impulse responses of the sites (built
-   once per grid and elastic constants by FP64 FFT solves of unit impulses; S*C*N values
-   in the storage precision, shared by contexts of one grid and elasticity).  Updates the sigma field and <sigma> like stz_stress_solve, to
+   once per grid and elastic constants by FP64 FFT solves of unit impulses and kept for
+   the half space of offsets, since the reference's operator is even: about S*C*N/2
+   values in the storage precision, shared by contexts of one grid and elasticity).  Updates the sigma field and <sigma> like stz_stress_solve, to
    rounding.  No counterpart in the reference.  n <= 64; -3 if EPS0 or SIGMA was uploaded
    since the last full solve. */
 int stz_stress_update_local(stz_ctx *ctx, int n, const int *sites);
 
 /* How the resident step (stz_run_steps, stz_ensemble_run) brings the stress up to date
-   after transforms: STZ_SOLVER_FFT, or STZ_SOLVER_LOCAL = the event-local update for a
-   thermal event and for avalanches of at most local_max voxels, with a full solve after
-   every `resync` local updates, for larger avalanches and at print steps.  No
-   counterpart in the reference.  Defaults: FFT, 256, 2 (each further site costs the
-   update another full read of the S*C response fields, so beyond a few sites the full
-   solve is cheaper). */
+   after transforms: STZ_SOLVER_FFT (the full solve on the device after every transform),
+   or STZ_SOLVER_LOCAL: the event-local update for a thermal event and for avalanches of
+   at most local_max voxels; a step that needs the full solve instead (an avalanche
+   larger than local_max, or `resync` local updates since the last full solve) halts
+   before changing anything and runs on the host path (stz_step), as do print steps.
+   The choice depends only on the run's own state, so results do not depend on batching.
+   No counterpart in the reference.  Defaults: FFT, 1024, 16. */
 int stz_set_solver(stz_ctx *ctx, int mode, long long resync, int local_max);
 
 /* SofteningFactor + ActEnergy + Q<0 count + KMC rate total

@@ -31,6 +31,7 @@
 #define KIND_FAIL 3
 #define KIND_ATH_OVERFLOW 4   // athermal avalanche larger than the pre-drawn redraw buffer
 #define KIND_STALL 5          // resident step: the random-number window does not hold the step's draws
+#define KIND_NEEDSOLVE 6      // resident step with the local update: a full solve is due (nothing changed)
 #define KIND_NONE (-1)        // no step decided (resident step halted)
 // resident step: the random-number positions of the decided step
 #define RES_K0 30             // first accepted-pair index of the step's gaussians (double)
@@ -47,6 +48,7 @@
 #define HALT_RNG 3            // the random-number window must be refilled
 #define HALT_ERR 4            // error code in RepState::err
 #define HALT_BUDGET 5         // the batch's step budget is used up
+#define HALT_SOLVE 6          // the next step needs the full solve: the host runs it on its own path
 
 // physical constants of the reference, STZ/bmg.h:68-74
 #define STZ_EV_IN_J 1.60217733e-19
@@ -210,8 +212,8 @@ struct stz_ctx {
     // event-local stress update (stz_local.hip): impulse response [S][C][N] in the
     // storage precision, shared by contexts of equal shape and elasticity
     int solver_mode = STZ_SOLVER_FFT;
-    long long local_resync = 256;
-    int local_max = 2;
+    long long local_resync = 1024;
+    int local_max = 16;
     void *kresp = nullptr;
     int *lsrc = nullptr;          // device list of update sites [STZ_LOCAL_MAXSRC]
     double ebase[6] = {0, 0, 0, 0, 0, 0};   // mean strain the field was last solved with

@@ -11,12 +11,14 @@
 
 namespace {
 
-// K[k][c][v] = (R) sigma_k[c][v] of the FP64 impulse solve k
+// K[k][c][i] = (R) sigma_k[c][i], i < H: the half space of offsets is the first H voxels
 template <typename R>
-__global__ void k_keep(const double *__restrict__ sig, R *__restrict__ K, long long n)
+__global__ void k_keep(const double *__restrict__ sig, R *__restrict__ K, int C, int N, int H)
 {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) K[i] = (R)sig[i];
+    if (i >= (long long)C * H) return;
+    const int c = (int)(i / H), j = (int)(i % H);
+    K[i] = (R)sig[(long long)c * N + j];
 }
 
 struct Uni {
@@ -32,7 +34,7 @@ __global__ __launch_bounds__(NT_LOCAL) void k_local(LocalGeom g, const R *__rest
     __shared__ double wred[NW_LOCAL * 6];
     for (int s = threadIdx.x; s < ns; s += NT_LOCAL) load_src<R, DIM>(g, deps_last, sites[s], src[s]);
     __syncthreads();
-    local_block<R, DIM>(g, K, src, ns, uni.u, sigma, blockIdx.x, spart, nanflag, wred);
+    local_rows<R, DIM>(g, K, src, ns, uni.u, sigma, blockIdx.x, spart, nanflag, wred);
 }
 
 template <int C>
@@ -69,13 +71,7 @@ KKey key_of(const stz_ctx *ctx)
     return KKey(ctx->dim, ctx->n[0], ctx->n[1], ctx->n[2], ctx->prec, ctx->k.lambda, ctx->k.mu);
 }
 
-LocalGeom geom_of(const stz_ctx *ctx)
-{
-    LocalGeom g;
-    for (int a = 0; a < 3; ++a) g.n[a] = ctx->n[a];
-    g.N = (int)ctx->N;
-    return g;
-}
+LocalGeom geom_of(const stz_ctx *ctx) { return local_geom(ctx->dim, ctx->n); }
 
 #define LHIP(call)                                                                 \
     do {                                                                           \
@@ -134,14 +130,14 @@ int build_response(stz_ctx *ctx, void *K)
             ctx->err = sh->err;
             goto done;
         }
-        const long long cn = (long long)C * g.N;
-        const unsigned grid = (unsigned)((cn + 255) / 256);
+        const long long ch = (long long)C * g.H;
+        const unsigned grid = (unsigned)((ch + 255) / 256);
         if (ctx->prec == STZ_FP64)
             hipLaunchKernelGGL(k_keep<double>, dim3(grid), dim3(256), 0, ctx->stream, (const double *)sh->sigma,
-                               (double *)K + (size_t)k * cn, cn);
+                               (double *)K + (size_t)k * ch, C, g.N, g.H);
         else
             hipLaunchKernelGGL(k_keep<float>, dim3(grid), dim3(256), 0, ctx->stream, (const double *)sh->sigma,
-                               (float *)K + (size_t)k * cn, cn);
+                               (float *)K + (size_t)k * ch, C, g.N, g.H);
         LHIP(hipGetLastError());
     }
     LHIP(hipStreamSynchronize(ctx->stream));
@@ -182,7 +178,7 @@ int stz_local_prepare(stz_ctx *ctx)
         ctx->kresp = it->second.k;
     } else {
         void *K = nullptr;
-        if (hipMalloc(&K, (size_t)ctx->S * ctx->C * g.N * ctx->rsz) != hipSuccess) {
+        if (hipMalloc(&K, (size_t)ctx->S * ctx->C * g.H * ctx->rsz) != hipSuccess) {
             ctx->err = "hipMalloc of the impulse response failed";
             return STZ_E_HIP;
         }
@@ -285,9 +281,10 @@ int stz_local_launch(stz_ctx *ctx, int ns, const double *uni)
     return STZ_OK;
 }
 
-// workgroups of the update over one grid (partials the mean reduces; at most n1*n2, the
-// rows of the partial array, since a workgroup spans at least one x row)
+// workgroups of the update over one grid (partials the mean reduces): one per ipw offset
+// rows of the half space, at most n1*(n2/2 + 1) <= n1*n2, the rows of the partial array
 long long stz_local_blocks(const stz_ctx *ctx)
 {
-    return (ctx->N + SPAN_LOCAL - 1) / SPAN_LOCAL;
+    const LocalGeom g = local_geom(ctx->dim, ctx->n);
+    return (g.items + g.ipw - 1) / g.ipw;
 }

@@ -11,11 +11,17 @@
 //
 // with K_k the stress response to the k-th basis eigenstrain placed at voxel 0 (all C
 // components, including the local -C0:eps0 term), built once per grid and elastic
-// constants by FP64 FFT solves of the S unit impulses and kept for the whole grid.
-// (The reference's k vectors use round(), STZ/init.c:126-129, which puts every Nyquist
-// component at -pi; on wavevectors with two or more Nyquist components the operator is
-// then not symmetric under the reflection of one axis, so K cannot be folded to one
-// octant of offsets without changing the result.)
+// constants by FP64 FFT solves of the S unit impulses.
+//
+// The reference keeps Re() of a full complex inverse transform (STZ/elastic.c:811-824),
+// which applies the average of the multiplier at k and -k: the operator is even, and
+// K(-d) = K(d) in every component.  So K is stored for the half space of offsets whose
+// slowest coordinate is in [0, n/2] (the first n0*n1*(n2/2+1) voxels of the grid), and an
+// offset outside it reads K(-d).  (Single-axis reflections are not symmetries: the
+// reference's round() k vectors, STZ/init.c:126-129, put every Nyquist component at -pi.)
+// A workgroup takes one row of offsets (d1, d2) of the half space and updates the two
+// grid rows s + (., d1, d2) and s - (., d1, d2), which need the same response row, so each
+// response value is read once per update.  A 2-D grid is treated as n0 x 1 x n1.
 //
 // sigma_solved_before is the field minus the uniform loading increments made since the
 // last solve: in FP64 they are in the field (usum), in mixed precision they are kept as
@@ -27,12 +33,36 @@
 
 namespace {
 
+// voxels along x per half-space row group of one workgroup (each updates two grid rows)
+#ifndef STZ_LOCAL_ROW_VOXELS
+#define STZ_LOCAL_ROW_VOXELS 512
+#endif
+
 struct LocalGeom {
-    int n[3];
+    int n[3];          // 3-D: the grid; 2-D: (n0, 1, n1)
     int N;             // below 2^31 (checked when the response is built)
+    int h2;            // n[2]/2 + 1: extent of the stored half space along the slowest axis
+    int H;             // n[0]*n[1]*h2 values per response field
+    int ipw;           // offset rows per workgroup (several when a row is short)
+    int items;         // offset rows n[1]*h2
 };
 
-// one source of an update: its grid coordinates and stored eigenstrain increment
+__host__ __device__ inline LocalGeom local_geom(int dim, const int *n)
+{
+    LocalGeom g;
+    g.n[0] = n[0];
+    g.n[1] = dim == 2 ? 1 : n[1];
+    g.n[2] = dim == 2 ? n[1] : n[2];
+    g.N = g.n[0] * g.n[1] * g.n[2];
+    g.h2 = g.n[2] / 2 + 1;
+    g.H = g.n[0] * g.n[1] * g.h2;
+    g.ipw = g.n[0] >= STZ_LOCAL_ROW_VOXELS ? 1 : STZ_LOCAL_ROW_VOXELS / g.n[0];
+    g.items = g.n[1] * g.h2;
+    return g;
+}
+
+// one source of an update: its grid coordinates (kernel geometry) and stored
+// eigenstrain increment
 template <typename R>
 struct LocalSrc {
     int c[3];
@@ -50,27 +80,16 @@ __device__ __forceinline__ void load_src(const LocalGeom &g, const R *deps_last,
     for (int k = 0; k < S; ++k) s.d[k] = deps_last[(long long)k * g.N + v];
 }
 
-// sum over sources of sum_k d_k K_k((x - s) mod n) for the C components at voxel (i0, i1, i2)
-template <typename R, int DIM>
-__device__ __forceinline__ void local_sum(const LocalGeom &g, const R *__restrict__ K, const LocalSrc<R> *src, int ns,
-                                          int i0, int i1, int i2, R *out)
+// index of offset d (each coordinate in [0, n)) in the half-space table: d itself when
+// its slowest coordinate is at most n2/2, else -d
+__device__ __forceinline__ int half_index(const LocalGeom &g, int d0, int d1, int d2)
 {
-    constexpr int C = DIM == 2 ? 3 : 6, S = DIM == 2 ? 2 : 5;
-#pragma unroll
-    for (int c = 0; c < C; ++c) out[c] = (R)0;
-    for (int s = 0; s < ns; ++s) {
-        int d0 = i0 - src[s].c[0], d1 = i1 - src[s].c[1], d2 = i2 - src[s].c[2];
-        if (d0 < 0) d0 += g.n[0];
-        if (d1 < 0) d1 += g.n[1];
-        if (d2 < 0) d2 += g.n[2];
-        const int jj = (d2 * g.n[1] + d1) * g.n[0] + d0;
-#pragma unroll
-        for (int k = 0; k < S; ++k) {
-            const R dk = src[s].d[k];
-#pragma unroll
-            for (int c = 0; c < C; ++c) out[c] = out[c] + dk * K[((long long)k * C + c) * g.N + jj];
-        }
+    if (2 * d2 > g.n[2]) {
+        d0 = d0 ? g.n[0] - d0 : 0;
+        d1 = d1 ? g.n[1] - d1 : 0;
+        d2 = g.n[2] - d2;
     }
+    return (d2 * g.n[1] + d1) * g.n[0] + d0;
 }
 
 // C0:e for the uniform strain change (the form of the solve's epilogue)
@@ -90,8 +109,6 @@ __host__ __device__ inline void c0_mul(const double *e, double lam, double mu, d
 }
 
 constexpr int NT_LOCAL = 256;
-constexpr int VPT_LOCAL = 4;                          // voxels per thread
-constexpr int SPAN_LOCAL = NT_LOCAL * VPT_LOCAL;      // voxels per workgroup
 constexpr int NW_LOCAL = NT_LOCAL / 64;
 
 __device__ __forceinline__ double wsum_local(double x)
@@ -101,21 +118,11 @@ __device__ __forceinline__ double wsum_local(double x)
     return x;
 }
 
-// one voxel: sigma += uni + the shifted responses of the sources; returns the new value
-// of each component in FP64 (before rounding to the storage precision)
-template <typename R, int DIM, bool ONE>
-__device__ __forceinline__ void local_voxel(const LocalGeom &g, const R *__restrict__ K, const LocalSrc<R> *src,
-                                            const LocalSrc<R> &s0, int ns, const double *uni, R *sigma, int v,
+// sigma[v] += uni + ds (FP64 sum, rounded to the storage precision once)
+template <typename R, int C>
+__device__ __forceinline__ void local_store(const LocalGeom &g, R *sigma, int v, const double *uni, const R *ds,
                                             double *acc, int &bad)
 {
-    constexpr int C = DIM == 2 ? 3 : 6;
-    const int i0 = v % g.n[0];
-    const int r = v / g.n[0];
-    const int i1 = r % g.n[1];
-    const int i2 = r / g.n[1];
-    R ds[C];
-    if constexpr (ONE) local_sum<R, DIM>(g, K, &s0, 1, i0, i1, i2, ds);
-    else local_sum<R, DIM>(g, K, src, ns, i0, i1, i2, ds);
 #pragma unroll
     for (int c = 0; c < C; ++c) {
         const double s = ((double)sigma[(long long)c * g.N + v] + uni[c]) + (double)ds[c];
@@ -125,45 +132,87 @@ __device__ __forceinline__ void local_voxel(const LocalGeom &g, const R *__restr
     }
 }
 
-// Updates the voxels [blk*SPAN_LOCAL, ...) of one workgroup (VPT_LOCAL per thread,
-// consecutive lanes on consecutive voxels) and writes the workgroup's FP64 partial sums
-// of the new stress, in a fixed order, to spart[blk][c].  Sources must be in `src`
-// (shared memory) already; wred holds NW_LOCAL*6 doubles.
+// sum over sources of sum_k d_k K_k(x - s) at voxel (x0, x1, x2) (any number of sources)
 template <typename R, int DIM>
-__device__ void local_block(const LocalGeom &g, const R *__restrict__ K, const LocalSrc<R> *src, int ns,
-                           const double *uni, R *sigma, long long blk, double *spart, int *nanflag, double *wred)
+__device__ __forceinline__ void local_sum(const LocalGeom &g, const R *__restrict__ K, const LocalSrc<R> *src, int ns,
+                                          int x0, int x1, int x2, R *out)
 {
-    constexpr int C = DIM == 2 ? 3 : 6;
-    const int tid = threadIdx.x, lane = tid % 64, w = tid / 64;
+    constexpr int C = DIM == 2 ? 3 : 6, S = DIM == 2 ? 2 : 5;
+#pragma unroll
+    for (int c = 0; c < C; ++c) out[c] = (R)0;
+    for (int j = 0; j < ns; ++j) {
+        int d0 = x0 - src[j].c[0], d1 = x1 - src[j].c[1], d2 = x2 - src[j].c[2];
+        if (d0 < 0) d0 += g.n[0];
+        if (d1 < 0) d1 += g.n[1];
+        if (d2 < 0) d2 += g.n[2];
+        const int hj = half_index(g, d0, d1, d2);
+#pragma unroll
+        for (int k = 0; k < S; ++k) {
+            const R dk = src[j].d[k];
+#pragma unroll
+            for (int c = 0; c < C; ++c) out[c] = out[c] + dk * K[((long long)k * C + c) * g.H + hj];
+        }
+    }
+}
+
+// One workgroup: the offset rows w*ipw .. of the half space (relative to source 0) and,
+// for each, the grid rows s0 + (., d1, d2) and (off the boundary planes d2 = 0, n2/2)
+// s0 - (., d1, d2).  With one source a thread reads the response at (d0, d1, d2) once
+// and applies it to x = s0 + d and x = s0 - d.  Writes the workgroup's FP64 partial sums
+// of the new stress, in a fixed order, to spart[w][c]; wred holds NW_LOCAL*6 doubles.
+template <typename R, int DIM>
+__device__ void local_rows(const LocalGeom &g, const R *__restrict__ K, const LocalSrc<R> *src, int ns,
+                           const double *uni, R *sigma, int w, double *spart, int *nanflag, double *wred)
+{
+    constexpr int C = DIM == 2 ? 3 : 6, S = DIM == 2 ? 2 : 5;
+    const int tid = threadIdx.x, lane = tid % 64, wv = tid / 64;
+    const int n0 = g.n[0], n1 = g.n[1], n2 = g.n[2];
     double acc[C];
 #pragma unroll
     for (int c = 0; c < C; ++c) acc[c] = 0.0;
     int bad = 0;
-    const int v0 = (int)blk * SPAN_LOCAL;
-    if (ns == 1) {
-        const LocalSrc<R> s0 = src[0];
+    const LocalSrc<R> s0 = src[0];
+    for (int e = tid; e < g.ipw * n0; e += NT_LOCAL) {
+        const int item = w * g.ipw + e / n0;
+        if (item >= g.items) break;
+        const int d0 = e % n0;
+        const int d1 = item % n1, d2 = item / n1;
+        const bool pair = d2 != 0 && 2 * d2 != n2;
+        const int a1 = (s0.c[1] + d1) % n1, a2 = (s0.c[2] + d2) % n2;
+        const int b1 = (s0.c[1] - d1 + n1) % n1, b2 = (s0.c[2] - d2 + n2) % n2;
+        const int va = (a2 * n1 + a1) * n0 + (s0.c[0] + d0) % n0;
+        const int vb = (b2 * n1 + b1) * n0 + (s0.c[0] - d0 + n0) % n0;
+        R da[C], db[C];
+        if (ns == 1) {
+            const int hj = (d2 * n1 + d1) * n0 + d0;
 #pragma unroll
-        for (int t = 0; t < VPT_LOCAL; ++t) {
-            const int v = v0 + t * NT_LOCAL + tid;
-            if (v < g.N) local_voxel<R, DIM, true>(g, K, src, s0, 1, uni, sigma, v, acc, bad);
+            for (int c = 0; c < C; ++c) da[c] = (R)0;
+#pragma unroll
+            for (int k = 0; k < S; ++k) {
+                const R dk = s0.d[k];
+#pragma unroll
+                for (int c = 0; c < C; ++c) da[c] = da[c] + dk * K[((long long)k * C + c) * g.H + hj];
+            }
+#pragma unroll
+            for (int c = 0; c < C; ++c) db[c] = da[c];
+        } else {
+            local_sum<R, DIM>(g, K, src, ns, va % n0, a1, a2, da);
+            if (pair) local_sum<R, DIM>(g, K, src, ns, vb % n0, b1, b2, db);
         }
-    } else {
-        for (int t = 0; t < VPT_LOCAL; ++t) {
-            const int v = v0 + t * NT_LOCAL + tid;
-            if (v < g.N) local_voxel<R, DIM, false>(g, K, src, src[0], ns, uni, sigma, v, acc, bad);
-        }
+        local_store<R, C>(g, sigma, va, uni, da, acc, bad);
+        if (pair) local_store<R, C>(g, sigma, vb, uni, db, acc, bad);
     }
     if (bad) atomicOr(nanflag, 1);
 #pragma unroll
     for (int c = 0; c < C; ++c) {
         const double x = wsum_local(acc[c]);
-        if (lane == 0) wred[w * 6 + c] = x;
+        if (lane == 0) wred[wv * 6 + c] = x;
     }
     __syncthreads();
     if (tid < C) {
         double t = wred[tid];
         for (int i = 1; i < NW_LOCAL; ++i) t = t + wred[i * 6 + tid];
-        spart[blk * C + tid] = t;
+        spart[(long long)w * C + tid] = t;
     }
 }
 

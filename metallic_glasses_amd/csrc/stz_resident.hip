@@ -154,6 +154,8 @@ __global__ __launch_bounds__(NT) void kb_decide(const RepDev *__restrict__ reps,
         int kind;
         if (nn > 0) {
             kind = KIND_ATHERMAL;                        // STZ/bmg.c:161
+        } else if (1 / Rtot < d.dt && d.kresp && st->nlocal >= d.local_resync) {
+            kind = KIND_NEEDSOLVE;                       // a thermal event, but a full solve is due
         } else if (1 / Rtot < d.dt) {                    // STZ/evolution.c:134-136
             kind = KIND_THERMAL;
             // the dice (STZ/evolution.c:142), then M*S gaussians (STZ/elastic.c:508-512)
@@ -188,7 +190,9 @@ __global__ __launch_bounds__(NT) void kb_decide(const RepDev *__restrict__ reps,
             int p = 0;
             long long k0 = mt.nacc[0] + 1;
             if (rpos - mt.base <= mt.len) k0 = mt_first(mt, rpos, p);
-            if (k0 + n * MS > mt.nacc[p]) {
+            if (d.kresp && (n > d.local_max || st->nlocal >= d.local_resync)) {
+                kind = KIND_NEEDSOLVE;                   // the avalanche goes to the full solve
+            } else if (k0 + n * MS > mt.nacc[p]) {
                 kind = KIND_STALL;
             } else {
                 res[13] = (double)n;
@@ -210,9 +214,7 @@ __global__ __launch_bounds__(NT) void kb_decide(const RepDev *__restrict__ reps,
         // stress after the transforms: the event-local update for a thermal event or a small
         // avalanche while the field is fresh enough, else the full solve
         const bool solved = kind == KIND_THERMAL || kind == KIND_ATHERMAL;
-        int local = 0;
-        if (solved && d.kresp && st->nlocal < d.local_resync)
-            local = (kind == KIND_THERMAL || (long long)res[13] <= d.local_max) ? 1 : 0;
+        const int local = (solved && d.kresp) ? 1 : 0;    // the local replica's full solves run on the host
         reinterpret_cast<int *>(res + RES_LOCAL)[0] = local;
         reinterpret_cast<int *>(res + RES_FFT)[0] = (solved && !local) ? kind : KIND_NONE;
         reinterpret_cast<int *>(res + RES_KIND)[0] = kind;
@@ -232,9 +234,7 @@ __global__ __launch_bounds__(NT_LOCAL) void kb_local(const RepDev *__restrict__ 
     const RepState *st = d.st;
     double *res = d.res;
     if (st->halt || reinterpret_cast<const int *>(res + RES_LOCAL)[0] != 1) return;
-    LocalGeom g;
-    for (int a = 0; a < 3; ++a) g.n[a] = d.n[a];
-    g.N = d.n[0] * d.n[1] * d.n[2];
+    const LocalGeom g = local_geom(DIM, d.n);
     const bool thermal = reinterpret_cast<const int *>(res + RES_KIND)[0] == KIND_THERMAL;
     const int ns = thermal ? 1 : (int)res[13];
     for (int s = threadIdx.x; s < ns; s += NT_LOCAL) {
@@ -251,8 +251,8 @@ __global__ __launch_bounds__(NT_LOCAL) void kb_local(const RepDev *__restrict__ 
         for (int c = 0; c < 6; ++c) uni[c] = fp64 ? u[c] - st->usum[c] : u[c];
     }
     __syncthreads();
-    local_block<R, DIM>(g, (const R *)d.kresp, src, ns, uni, (R *)d.sigma, blockIdx.x, d.spart,
-                        reinterpret_cast<int *>(res + 15), wred);
+    local_rows<R, DIM>(g, (const R *)d.kresp, src, ns, uni, (R *)d.sigma, blockIdx.x, d.spart,
+                       reinterpret_cast<int *>(res + 15), wred);
 }
 
 template <typename R, int DIM>
@@ -420,9 +420,10 @@ __global__ __launch_bounds__(STZ_NT_MEAN) void kb_finish(const RepDev *__restric
     reinterpret_cast<int *>(res + RES_FFT)[0] = KIND_NONE;
     reinterpret_cast<int *>(res + RES_LOCAL)[0] = 0;
     RepState s = *sp;
-    if (kind == KIND_STALL) {
-        sp->halt = HALT_RNG;
-        publish_halt(d, HALT_RNG);
+    if (kind == KIND_STALL || kind == KIND_NEEDSOLVE) {
+        const int h = kind == KIND_STALL ? HALT_RNG : HALT_SOLVE;
+        sp->halt = h;
+        publish_halt(d, h);
         kindp[0] = KIND_NONE;
         return;
     }
@@ -556,6 +557,7 @@ namespace {
 struct StzClass {
     std::vector<stz_ctx *> reps;
     bool local = false;               // some replica updates the stress event-locally
+    bool fft = true;                  // some replica takes the full solve on the device
     RepDev *d_reps = nullptr;
     SolveRep *d_solve = nullptr;
 };
@@ -841,12 +843,14 @@ int launch_class(StzPlan *pl, StzClass &k, hipEvent_t ea, hipEvent_t eb)
     hipLaunchKernelGGL((kb_decide<R, DIM>), dim3(nrep), dim3(NT), 0, s, k.d_reps, p->N, p->nblk, p->ngrp, p->M);
     hipLaunchKernelGGL((kb_athermal<R, DIM>), dim3((unsigned)p->nblk, nrep), dim3(NT), 0, s, k.d_reps, p->N, p->M);
     if (hipGetLastError() != hipSuccess) return STZ_E_HIP;
-    const int saved = p->timing;
-    p->timing = 0;
     int nparts = 0;
-    int rc = stz_solve_launch_batched(p, nrep, k.d_solve, s, &nparts);
-    p->timing = saved;
-    if (rc) return rc;
+    if (k.fft) {            // replicas whose every transform is followed by the full solve
+        const int saved = p->timing;
+        p->timing = 0;
+        const int rc = stz_solve_launch_batched(p, nrep, k.d_solve, s, &nparts);
+        p->timing = saved;
+        if (rc) return rc;
+    }
     const int fp64 = p->prec == STZ_FP64;
     const long long nlb = stz_local_blocks(p);
     if (k.local)
@@ -928,9 +932,11 @@ int plan_upload(StzPlan *pl)
         std::vector<RepDev> d;
         std::vector<SolveRep> s;
         k.local = false;
+        k.fft = false;
         for (stz_ctx *c : k.reps) {
             d.push_back(c->rs->hdev);
             k.local = k.local || c->rs->hdev.kresp != nullptr;
+            k.fft = k.fft || c->rs->hdev.kresp == nullptr;
             SolveRep q;
             q.eps0 = c->eps0;
             q.spec = c->spec;
@@ -1163,6 +1169,29 @@ int stz_plan_run(StzPlan *pl, int nsteps, stz_step_result *const *results, int *
                     for (int cc = 0; cc < 6; ++cc) results[i][done[i] - 1].stress[cc] = c->stress[cc];
                 if (prc == 1) {
                     rcs[i] = 1;
+                    active[i] = 0;
+                    continue;
+                }
+            }
+            if (halt == HALT_SOLVE) {
+                // the step needs the full solve (periodic re-solve of the local update, or an
+                // avalanche larger than it takes): it runs on the host path, whose stress
+                // solve is the full FFT solve, with the same random numbers
+                stz_rng_sync(c);
+                stz_step_result r;
+                int hrc = stz_step_kmc(c, &r);
+                if (hrc >= 0 && r.printed && !stop_at_print) hrc = stz_step_print(c, &r);
+                if (hrc >= 0) {
+                    if (results && results[i]) results[i][done[i]] = r;
+                    done[i] += 1;
+                    next_budget -= 1;
+                }
+                if (hrc != 0) {
+                    rcs[i] = hrc;
+                    active[i] = 0;
+                    continue;
+                }
+                if (r.printed && stop_at_print) {
                     active[i] = 0;
                     continue;
                 }

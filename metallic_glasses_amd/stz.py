@@ -237,7 +237,7 @@ class Stz:
         a = (C.c_int * max(len(sites), 1))(*[int(x) for x in sites])
         self._check(lib().stz_stress_update_local(self._h, len(sites), a))
 
-    def set_solver(self, mode, resync=256, local_max=2):
+    def set_solver(self, mode, resync=1024, local_max=16):
         """Stress update of the resident step: SOLVER_FFT or SOLVER_LOCAL."""
         self._check(lib().stz_set_solver(self._h, int(mode), int(resync), int(local_max)))
 

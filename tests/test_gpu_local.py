@@ -172,7 +172,7 @@ def _kw3(n, **extra):
     return kw
 
 
-def _run(kw, nsteps, mode, resync=64, local_max=8, printperiod=5):
+def _run(kw, nsteps, mode, resync=64, local_max=16, printperiod=5):
     g = mg.Stz(**kw)
     g.set_print(printperiod, 1.0)
     g.set_solver(mode, resync, local_max)
@@ -228,8 +228,10 @@ def test_resident_local_mixed_tracks_fp64():
     assert err <= 1e-4
 
 
+@pytest.mark.parametrize("resync,local_max", [(1024, 16), (100, 1)], ids=["device", "host_fallbacks"])
 @pytest.mark.parametrize("name", ["shear3d_64", "ref2d_512"])
-def test_resident_local_replays_golden_trace(name):
+def test_resident_local_replays_golden_trace(name, resync, local_max):
+    """(100, 1): every 2-voxel avalanche and every 100th update run on the host path."""
     path = os.path.join(GOLD, f"trace_{name}.npz")
     if not os.path.exists(path):
         pytest.skip("fixture absent")
@@ -239,7 +241,7 @@ def test_resident_local_replays_golden_trace(name):
     print("resident local trace", name, n, "steps", flush=True)
     g = mg.Stz(**kw)
     g.set_print(int(d["printperiod"]), float(d["endstrain"]))
-    g.set_solver(mg.SOLVER_LOCAL, 256, 8)
+    g.set_solver(mg.SOLVER_LOCAL, resync, local_max)
     rc, recs = g.run_steps(n)
     assert len(recs) == n
     lc = 0 if kw.get("loading", 0) == 0 else (2 if kw["dim"] == 2 else 5)

@@ -11,10 +11,11 @@ STZ/elastic.c:779-828 behind the wrapper STZ/elastic.c:977-1066):
 - linearity: a multi-site eigenstrain change plus a mean-strain change re-solved in full
   equals the old field + C0:(mean-strain change) + the shifted responses, written here
   in numpy independently of the HIP kernel;
-- why the response is kept for the whole grid: the reference's round() k vectors
-  (STZ/init.c:126-129) put every Nyquist component at -pi, so the operator is not
-  symmetric under the reflection of one axis on wavevectors with two Nyquist components
-  (a (-1)^(x+y) checkerboard in 2-D), and an octant-folded table would be wrong.
+- which symmetry halves the response table: the response is even, K(-x) = K(x) (the
+  reference's Re() averages the multiplier at k and -k), but it is not symmetric under
+  the reflection of one axis: the reference's round() k vectors (STZ/init.c:126-129) put
+  every Nyquist component at -pi, which leaves a (-1)^(x+y) checkerboard on wavevectors
+  with two Nyquist components, so an octant-folded table would be wrong.
 """
 import numpy as np
 import pytest
@@ -84,6 +85,18 @@ def test_impulse_response_is_translation_invariant(dims):
             r = _solve(o, _basis_field(o, k, site), np.zeros(6))
             rolled = np.roll(K[k], shift=sh, axis=(1, 2, 3))
             assert np.max(np.abs(r - rolled)) <= 1e-13 * np.max(np.abs(K[k]))
+
+
+@pytest.mark.parametrize("dims", GRIDS)
+def test_impulse_response_is_even(dims):
+    """K(-x) = K(x) in every component: the reference keeps Re() of a full complex
+    inverse transform (STZ/elastic.c:811-824), which applies the average of the
+    multiplier at k and -k, an even operator -- so half of the response table (offsets
+    with the slowest axis in [0, n/2]) determines the rest."""
+    o = po.Oracle(empty=True, **_kw(dims))
+    for K in _responses(o):
+        inv = np.roll(np.flip(K, axis=(1, 2, 3)), 1, axis=(1, 2, 3))   # x -> (-x) mod n on every axis
+        assert np.max(np.abs(inv - K)) <= 1e-13 * np.max(np.abs(K))
 
 
 def test_reflection_asymmetry_is_the_double_nyquist_checkerboard():
