@@ -242,8 +242,13 @@ def curve_err(ca, cb, n):
 
 
 def same_state_err(gm, kw, mg):
-    """Stress-field error of a mixed-precision context against the FP64 solve of its own
-    eigenstrain field and mean strain (independent of where the trajectories part)."""
+    """Stress-field error of a context at the end of a run against the FP64 full solve of
+    its own eigenstrain field and mean strain (independent of where trajectories part).
+    Between solves the field also holds the loading increments of the steps since (as the
+    reference's Sigma does), so it is first brought to the current mean strain with a
+    zero-site local update; what remains is the accumulated rounding of the run's own
+    solves and updates (FP32 storage in mixed precision)."""
+    gm.stress_update_local([])
     f = mg.Stz(init_state=False, **dict(kw, precision=mg.FP64))
     f.upload(mg.F_EPS0, gm.download(mg.F_EPS0))
     e, st, t = gm.scalars()

@@ -151,6 +151,26 @@ def test_update_local_long_sequence_fp64():
     assert np.max(np.abs(a - b)) <= 1e-12 * np.max(np.abs(a))
 
 
+@pytest.mark.parametrize("precision", [mg.FP64, mg.MIXED])
+def test_zero_site_update_equals_resolve_after_a_run(precision):
+    """After resident steps the field also holds the loading increments since its last
+    solve; a zero-site update brings it to what a full solve of the current state gives."""
+    kw = _kw3((16, 16, 16), precision=precision)
+    g = mg.Stz(**kw)
+    g.set_print(5, 1.0)
+    g.set_solver(mg.SOLVER_LOCAL, 1024, 16)
+    g.run_steps(300)
+    twin = mg.Stz(init_state=False, **dict(kw, precision=mg.FP64))
+    twin.upload(mg.F_EPS0, g.download(mg.F_EPS0))
+    e, st, t = g.scalars()
+    twin.set_scalars(e, st, t)
+    twin.stress_solve()
+    g.stress_update_local([])
+    a, b = twin.download(mg.F_SIGMA), g.download(mg.F_SIGMA)
+    err = np.linalg.norm(b - a) / np.linalg.norm(a)
+    assert err <= (1e-12 if precision == mg.FP64 else 1e-5), err
+
+
 def test_update_local_rejects_a_stale_field():
     kw = _kw((16, 8, 32))
     g = mg.Stz(init_state=False, **kw)
