@@ -150,6 +150,15 @@ int stz_stress_solve(stz_ctx *ctx);
    since the last full solve. */
 int stz_stress_update_local(stz_ctx *ctx, int n, const int *sites);
 
+/* Print-step displacement field (InhomogeneousStressSolver(..., uflag=2) at STZ/bmg.c:289-295:
+   HomogeneousStressSolver's displacement branch, STZ/elastic.c:873-900): from the
+   round-tripped EPS0 and EpsAvg, u_hat = -i (F - alpha G n) / (mu |k|) and
+   u = Re(IFFT)/N + the supercell position (2-D as the reference writes it, its slow
+   index along x; 3-D: x along the fast axis).  host_u receives dim*N doubles
+   [component][voxel] (the X1R/X2R outputs, STZ/bmg.c:300-330).  The stress is not
+   changed. */
+int stz_displacement(stz_ctx *ctx, double *host_u);
+
 /* How the resident step (stz_run_steps, stz_ensemble_run) brings the stress up to date
    after transforms: STZ_SOLVER_FFT (the full solve on the device after every transform),
    or STZ_SOLVER_LOCAL: the event-local update for a thermal event and for avalanches of
@@ -263,6 +272,17 @@ int stz_mt_window_probe(unsigned long seed, long long skip, long long window_wor
 int stz_rng_seed(stz_ctx *ctx, unsigned long seed);
 int stz_rng_uniform(stz_ctx *ctx, double *u);
 int stz_rng_gaussian(stz_ctx *ctx, double sigma, double *g);
+
+/* Checkpoint and restart (no counterpart in the reference, which never saves its RNG
+   state, Q0, transformation strains or softening state).  The file holds the MT19937
+   state and position, every per-voxel array in its storage precision, EpsAvg, <sigma>,
+   the clock, the pending uniform stress, the print/termination state and the counters;
+   loading it into a context created with the same grid, mode count and precision
+   continues the run exactly as the saving context would (-3 on a mismatch or a corrupt
+   file; after a failed load the context's state is undefined until the next successful
+   load or initialisation). */
+int stz_checkpoint_save(stz_ctx *ctx, const char *path);
+int stz_checkpoint_load(stz_ctx *ctx, const char *path);
 
 /* Counters: step, astep, kstep, estep, transforms, solves. */
 int stz_counters(stz_ctx *ctx, long long out[6]);

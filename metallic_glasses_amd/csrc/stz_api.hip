@@ -662,6 +662,18 @@ int stz_get_scalars(stz_ctx *ctx, double epsavg[6], double stress[6], double *to
 
 int stz_stress_solve(stz_ctx *ctx) { return ctx ? solve(ctx) : STZ_E_ARG; }
 
+}  // extern "C"
+
+int stz_invalidate_derived(stz_ctx *ctx)
+{
+    const int rc = invalidate_cache(ctx);
+    if (rc) return rc;
+    STZ_HIP(hipStreamSynchronize(ctx->stream));
+    return STZ_OK;
+}
+
+extern "C" {
+
 int stz_stress_update_local(stz_ctx *ctx, int n, const int *sites)
 {
     if (!ctx || n < 0 || n > STZ_LOCAL_MAXSRC || (n > 0 && !sites)) return STZ_E_ARG;
@@ -705,6 +717,21 @@ int stz_stress_update_local(stz_ctx *ctx, int n, const int *sites)
     ctx->nlocal++;
     ctx->nsolve++;
     return STZ_OK;
+}
+
+int stz_displacement(stz_ctx *ctx, double *host_u)
+{
+    if (!ctx || !host_u) return STZ_E_ARG;
+    const size_t bytes = (size_t)ctx->dim * (size_t)ctx->N * 8;
+    double *u = nullptr;
+    STZ_HIP(hipMalloc((void **)&u, bytes));
+    int rc = stz_displacement_launch(ctx, u);
+    if (!rc && hipMemcpyAsync(host_u, u, bytes, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess) rc = STZ_E_HIP;
+    if (!rc && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = STZ_E_HIP;
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipFree(u);
+    if (rc == STZ_E_HIP && ctx->err.empty()) ctx->err = "displacement: HIP error";
+    return rc;
 }
 
 int stz_set_solver(stz_ctx *ctx, int mode, long long resync, int local_max)

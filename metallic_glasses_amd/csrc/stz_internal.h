@@ -209,7 +209,10 @@ struct stz_ctx {
     long long avlan = 0;
     double soc_elast = 0.0;
 
-    // event-local stress update (stz_local.hip): impulse response [S][C][N] in the
+    // marks the derived per-voxel arrays stale after a restore (the next rate pass rebuilds them)
+int stz_invalidate_derived(stz_ctx *ctx);
+
+// event-local stress update (stz_local.hip): impulse response [S][C][N] in the
     // storage precision, shared by contexts of equal shape and elasticity
     int solver_mode = STZ_SOLVER_FFT;
     long long local_resync = 1024;
@@ -463,6 +466,7 @@ __device__ void mean_partials(const double *spart, int nblocks, double invN, dou
 int stz_fft_setup(stz_ctx *ctx);
 void stz_fft_free(stz_ctx *ctx);
 int stz_solve_launch(stz_ctx *ctx, int gated); // enqueues the full stress solve on ctx->stream
+int stz_displacement_launch(stz_ctx *ctx, double *u);   // print-step displacement into u[dim][N] (device)
 int stz_rates_launch(stz_ctx *ctx, int decide); // enqueues rates + group sums (+ step-kind decision)
 int stz_select_launch(stz_ctx *ctx, double target, int gated);
 int stz_apply_launch(stz_ctx *ctx, int site, int mode, int gated);
@@ -477,6 +481,9 @@ int stz_init_device_launch(stz_ctx *ctx, unsigned long long seed, const double *
                            double inc_std, long long *inside);
 // batched solve without the mean kernel (the caller reduces the *nparts partials)
 int stz_solve_launch_batched(stz_ctx *proto, int nrep, const SolveRep *reps, hipStream_t stream, int *nparts);
+
+// marks the derived per-voxel arrays stale after a restore (the next rate pass rebuilds them)
+int stz_invalidate_derived(stz_ctx *ctx);
 
 // event-local stress update (stz_local.hip)
 int stz_local_prepare(stz_ctx *ctx);    // builds (or takes from the cache) the folded impulse response

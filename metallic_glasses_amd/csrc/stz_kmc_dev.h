@@ -11,6 +11,10 @@
 
 namespace {
 
+#ifndef STZ_RATE_DEPTH
+#define STZ_RATE_DEPTH 1
+#endif
+
 constexpr int NT = 256;
 constexpr int WAVE = 64;
 constexpr int NW = NT / WAVE;
@@ -216,24 +220,27 @@ __device__ __forceinline__ void rates_block(const RateArgs<R> &a, long long blk)
     const long long v = blk * NT + tid;
     const bool valid = v < a.N;
     VoxelState<R> s;
-    ModeData<R> cur, nxt;
+    // modes are loaded STZ_RATE_DEPTH ahead of the one being evaluated
+    ModeData<R> md[STZ_RATE_DEPTH + 1];
     if (valid) {
         load_voxel<R, DIM>(a, v, true, s);
-        load_mode<R, DIM>(a, v, 0, s.fresh, cur);
+#pragma unroll
+        for (int j = 0; j < STZ_RATE_DEPTH; ++j)
+            if (j < a.M) load_mode<R, DIM>(a, v, j, s.fresh, md[j]);
     }
     R minq = (R)0;
     int am = 0, nneg = 0;
     for (int m = 0; m < a.M; ++m) {
         double rate = 0.0;
         if (valid) {
-            // the next mode's loads are in flight while this mode is evaluated
-            if (m + 1 < a.M) load_mode<R, DIM>(a, v, m + 1, s.fresh, nxt);
+            if (m + STZ_RATE_DEPTH < a.M) load_mode<R, DIM>(a, v, m + STZ_RATE_DEPTH, s.fresh, md[STZ_RATE_DEPTH]);
             R q;
-            const R qe = mode_q<R, DIM>(a, cur, s, q, rate);
+            const R qe = mode_q<R, DIM>(a, md[0], s, q, rate);
             if (!s.fresh) a.q0eff[(long long)m * a.N + v] = qe;
             if (m == 0 || minq > q) { minq = q; am = m; }     // first argmin (STZ/evolution.c:56-63)
             nneg += (q < (R)0);
-            cur = nxt;
+#pragma unroll
+            for (int j = 0; j < STZ_RATE_DEPTH; ++j) md[j] = md[j + 1];
         }
         const double t = wave_sum_xor(rate);
         if (lane == 0) wsum[m][w] = t;

@@ -171,7 +171,17 @@ __device__ void local_rows(const LocalGeom &g, const R *__restrict__ K, const Lo
 #pragma unroll
     for (int c = 0; c < C; ++c) acc[c] = 0.0;
     int bad = 0;
-    const LocalSrc<R> s0 = src[0];
+    // rows are laid out around source 0 (any origin partitions the grid; with no source,
+    // voxel 0 and only the uniform part)
+    LocalSrc<R> s0;
+    if (ns > 0) {
+        s0 = src[0];
+    } else {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) s0.c[a] = 0;
+#pragma unroll
+        for (int k2 = 0; k2 < STZ_MAXS; ++k2) s0.d[k2] = (R)0;
+    }
     for (int e = tid; e < g.ipw * n0; e += NT_LOCAL) {
         const int item = w * g.ipw + e / n0;
         if (item >= g.items) break;

@@ -25,6 +25,7 @@
 #include "stz_internal.h"
 
 #include <cmath>
+#include <type_traits>
 
 namespace {
 
@@ -39,6 +40,7 @@ struct SolveConsts {
     int rows;           // n1*n2
     int h0, hp;
     int C;
+    double L[3];        // voxel edges (displacement position term)
 };
 
 // Per-replica view of a batched launch: the replica index is a grid coordinate, and
@@ -188,9 +190,10 @@ __device__ __forceinline__ void gamma_part(const R *e, const R *g, const R *gg, 
     }
 }
 
-// unit vector of the reference k grid (Grad, STZ/init.c:116-137) for integer indices
+// unit vector of the reference k grid (Grad, STZ/init.c:116-137) for integer indices;
+// returns |k| (g2, set to 1 at k = 0)
 template <int DIM>
-__device__ __forceinline__ void unit_k(const int *idx, const int *n, double *g)
+__device__ __forceinline__ double unit_k(const int *idx, const int *n, double *g)
 {
     g[0] = g[1] = g[2] = 0.0;
 #pragma unroll
@@ -204,6 +207,56 @@ __device__ __forceinline__ void unit_k(const int *idx, const int *n, double *g)
     if (idx[0] + idx[1] + (DIM == 3 ? idx[2] : 0) == 0) g2 = 1.0;
 #pragma unroll
     for (int a = 0; a < 3; ++a) g[a] /= g2;
+    return g2;
+}
+
+// Displacement branch of HomogeneousStressSolver (uflag 1/2, STZ/elastic.c:873-887) on one
+// real part of the eigenstrain spectrum: F = (C0:e).n, G = n.F and
+// tmp_a = F_a + (-alpha)*(G*n_a) (HadamardSumnSProd, :765-777), a < DIM
+template <typename R, int C, int DIM>
+__device__ __forceinline__ void disp_part(const R *e, const R *g, R lam, R mu, R nalpha, R *o)
+{
+    R t[C], F[3], G;
+    if constexpr (C == 3) {
+        R tra = lam * (e[0] + e[1]);
+        t[0] = tra + 2 * mu * e[0];
+        t[1] = tra + 2 * mu * e[1];
+        t[2] = 2 * mu * e[2];
+        F[0] = t[0] * g[0] + t[2] * g[1];
+        F[1] = t[2] * g[0] + t[1] * g[1];
+        G = F[0] * g[0] + F[1] * g[1];
+    } else {
+        R tra = lam * ((e[0] + e[1]) + e[2]);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) t[c] = tra + 2 * mu * e[c];
+#pragma unroll
+        for (int c = 3; c < 6; ++c) t[c] = 2 * mu * e[c];
+        F[0] = t[0] * g[0] + t[5] * g[1] + t[4] * g[2];
+        F[1] = t[5] * g[0] + t[1] * g[1] + t[3] * g[2];
+        F[2] = t[4] * g[0] + t[3] * g[1] + t[2] * g[2];
+        G = F[0] * g[0] + F[1] * g[1] + F[2] * g[2];
+    }
+#pragma unroll
+    for (int a = 0; a < DIM; ++a) o[a] = F[a] + nalpha * (G * g[a]);
+}
+
+// supercell position added to the displacement (STZ/elastic.c:893-899).  2-D as the
+// reference writes it: its slow index i along x, the fast index j along y; 3-D: x_b the
+// grid index along axis b (x = the fast axis), u_a += sum_b x_b (delta_ab + e_ab) L_b.
+template <int DIM>
+__device__ __forceinline__ double position(int a, int i0, int i1, int i2, const double *e, const double *L)
+{
+    if constexpr (DIM == 2) {
+        const double i = (double)i1, j = (double)i0;
+        return a == 0 ? i * (1 + e[0]) * L[0] + j * e[2] * L[1] : i * e[2] * L[0] + j * (1 + e[1]) * L[1];
+    } else {
+        constexpr int vo[3][3] = {{0, 5, 4}, {5, 1, 3}, {4, 3, 2}};
+        const double x[3] = {(double)i0, (double)i1, (double)i2};
+        double pos = 0.0;
+#pragma unroll
+        for (int b = 0; b < 3; ++b) pos = pos + x[b] * ((a == b ? 1.0 : 0.0) + e[vo[a][b]]) * L[b];
+        return pos;
+    }
 }
 
 template <typename R, int DIM>
@@ -317,7 +370,7 @@ __global__ __launch_bounds__(NT_SOLVE) void k_ypass(cpx<R> *__restrict__ spec_a,
 // the C*W lines are transformed together, Gamma-hat mixes the C components of each of
 // the LZ*W wavevectors in place, and the inverse transform follows.
 // ---------------------------------------------------------------------------------
-template <typename R, int LZ, int W, int C, int DIM, int NTF>
+template <typename R, int LZ, int W, int C, int DIM, int NTF, int OP = 0>
 __global__ __launch_bounds__(NTF) void k_fused(cpx<R> *__restrict__ spec_a, const cpx<R> *__restrict__ twg,
                                                SolveConsts k, const SolveRep *reps)
 {
@@ -343,8 +396,52 @@ __global__ __launch_bounds__(NTF) void k_fused(cpx<R> *__restrict__ spec_a, cons
     }
     __syncthreads();
     fft_lines<R, LZ, NL, NL, NTF>(buf, tw, (R)-1);
-    // Gamma-hat on each wavevector of the tile
+    // Gamma-hat (OP 0) or the displacement operator (OP 1) on each wavevector of the tile
     const R lam = (R)k.lambda, mu = (R)k.mu, s0 = (R)k.s0, sh = (R)k.sh, sa = (R)k.sa;
+    if constexpr (OP == 1) {
+        // u_hat_a = (tmp_a(im)*(1/mu)/|k|, -1.0*tmp_a(re)*(1/mu)/|k|) (HadamardSiDivd,
+        // STZ/elastic.c:754-763) into component slots 0..DIM-1; on Nyquist bins the
+        // average with the mirror unit vector, as Re() of the full transform applies
+        const R nalpha = (R)(-1.0 * k.alpha);
+        for (int kp = tid; kp < LZ * W; kp += NTF) {
+            const int col = kp % W, p = kp / W;
+            int idx[3];
+            idx[0] = k0base + col;
+            if constexpr (DIM == 3) { idx[1] = i1; idx[2] = p; }
+            else { idx[1] = p; idx[2] = 0; }
+            double gd[3];
+            const R kmag = (R)unit_k<DIM>(idx, k.n, gd);
+            bool nyq = false;
+#pragma unroll
+            for (int a = 0; a < DIM; ++a) nyq |= (2 * idx[a] == k.n[a]);
+            const R g[3] = {(R)gd[0], (R)gd[1], (R)gd[2]};
+            R er[C], ei[C], pr[3], pi[3];
+#pragma unroll
+            for (int c = 0; c < C; ++c) {
+                const cpx<R> x = buf[p * NL + c * W + col];
+                er[c] = x.x;
+                ei[c] = x.y;
+            }
+            disp_part<R, C, DIM>(er, g, lam, mu, nalpha, pr);
+            disp_part<R, C, DIM>(ei, g, lam, mu, nalpha, pi);
+            if (nyq) {
+                R gm[3] = {g[0], g[1], g[2]};
+#pragma unroll
+                for (int a = 0; a < DIM; ++a)
+                    if (2 * idx[a] == k.n[a]) gm[a] = -gm[a];
+                R pr2[3], pi2[3];
+                disp_part<R, C, DIM>(er, gm, lam, mu, nalpha, pr2);
+                disp_part<R, C, DIM>(ei, gm, lam, mu, nalpha, pi2);
+#pragma unroll
+                for (int a = 0; a < DIM; ++a) {
+                    pr[a] = (pr[a] + pr2[a]) * (R)0.5;
+                    pi[a] = (pi[a] + pi2[a]) * (R)0.5;
+                }
+            }
+#pragma unroll
+            for (int a = 0; a < DIM; ++a) buf[p * NL + a * W + col] = {pi[a] * s0 / kmag, (R)-1.0 * pr[a] * s0 / kmag};
+        }
+    } else
     for (int kp = tid; kp < LZ * W; kp += NTF) {
         const int col = kp % W, p = kp / W;
         int idx[3];
@@ -480,6 +577,59 @@ __global__ __launch_bounds__(NT_SOLVE) void k_inv_x(cpx<R> *__restrict__ spec_a,
         }
         if (tid == 0) spart[(long long)blockIdx.x * C + c] = red[0];
         __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// k_inv_x_disp: C2R along x of the DIM displacement components left in the component
+// slots 0..DIM-1 by the OP 1 fused pass, u_a = Re/N + supercell position
+// (STZ/elastic.c:884-899), into u[a][v] (FP64)
+// ---------------------------------------------------------------------------------
+template <typename R, int L0, int NR, int DIM>
+__global__ __launch_bounds__(NT_SOLVE) void k_inv_x_disp(const cpx<R> *__restrict__ spec,
+                                                         double *__restrict__ u, const cpx<R> *__restrict__ twg,
+                                                         SolveConsts k)
+{
+    constexpr int NP = NR / 2;
+    constexpr int NL = DIM * NP;
+    constexpr int LS = NL + 1;
+    constexpr int H = L0 / 2;
+    __shared__ cpx<R> buf[L0 * LS];
+    __shared__ cpx<R> tw[L0];
+    const int tid = threadIdx.x;
+    for (int i = tid; i < L0; i += NT_SOLVE) tw[i] = twg[i];
+    const long long row0 = (long long)blockIdx.x * NR;
+    for (int e = tid; e < DIM * NP * L0; e += NT_SOLVE) {
+        const int kk = e % L0;
+        const int p = (e / L0) % NP;
+        const int c = e / (L0 * NP);
+        const cpx<R> *ra = spec + ((long long)c * k.rows + row0 + 2 * p) * k.hp;
+        const cpx<R> *rb = ra + k.hp;
+        cpx<R> Z;
+        if (kk == 0 || kk == H) {
+            Z = {ra[kk].x, rb[kk].x};
+        } else if (kk < H) {
+            const cpx<R> a = ra[kk], b = rb[kk];
+            Z = {a.x - b.y, a.y + b.x};
+        } else {
+            const cpx<R> a = ra[L0 - kk], b = rb[L0 - kk];
+            Z = {a.x + b.y, -a.y + b.x};
+        }
+        buf[kk * LS + c * NP + p] = Z;
+    }
+    __syncthreads();
+    fft_lines<R, L0, NL, LS, NT_SOLVE>(buf, tw, (R)1);
+    const R *bufr = reinterpret_cast<const R *>(buf);
+    for (int e = tid; e < NR * L0; e += NT_SOLVE) {
+        const int rl = e / L0, i0 = e % L0;
+        const long long r = row0 + rl;
+        const long long v = r * L0 + i0;
+        const int i1 = (int)(r % k.n[1]), i2 = (int)(r / k.n[1]);
+#pragma unroll
+        for (int a = 0; a < DIM; ++a) {
+            const double z = (double)bufr[2 * (i0 * LS + a * NP + (rl >> 1)) + (rl & 1)];
+            u[(long long)a * k.N + v] = z / (double)k.N + position<DIM>(a, i0, i1, i2, k.epsavg, k.L);
+        }
     }
 }
 
@@ -801,6 +951,112 @@ int solve_impl(stz_ctx *ctx, int gated, int nrep, const SolveRep *reps, hipStrea
     return STZ_OK;
 }
 
+// Print-step displacement (HomogeneousStressSolver's displacement branch,
+// STZ/elastic.c:873-900) of the context's eps0 (round-tripped) and EpsAvg into u[DIM][N]
+// on the device: x-forward, y-forward, the fused z pass with the displacement operator,
+// y-inverse, and the C2R x pass with the position term.  Default tiles (print time only).
+template <typename R>
+int disp_impl(stz_ctx *ctx, double *u)
+{
+    Launch<R> Lr;
+    Lr.ctx = ctx;
+    Lr.stream = ctx->stream;
+    SolveConsts &k = Lr.k;
+    k.gate = nullptr;
+    k.lambda = ctx->k.lambda;
+    k.mu = ctx->k.mu;
+    k.alpha = ctx->k.alpha;
+    k.s0 = 1.0 / ctx->k.mu;
+    k.sh = 0.5 / ctx->k.mu;
+    k.sa = -1.0 * ctx->k.alpha / ctx->k.mu;
+    k.invN = 1.0 / (double)ctx->N;
+    for (int c = 0; c < 6; ++c) k.epsavg[c] = ctx->epsavg[c];
+    for (int a = 0; a < 3; ++a) k.n[a] = ctx->n[a];
+    k.N = ctx->N;
+    k.rows = ctx->n[1] * ctx->n[2];
+    k.h0 = ctx->h0;
+    k.hp = ctx->hp;
+    k.C = ctx->C;
+    k.L[0] = ctx->p.lx;
+    k.L[1] = ctx->p.ly;
+    k.L[2] = ctx->p.lz;
+    Lr.nanflag = reinterpret_cast<int *>(ctx->res + 15);
+    const int saved[4] = {ctx->tune[0], ctx->tune[1], ctx->tune[2], ctx->tune[3]};
+    for (int t = 0; t < 4; ++t) ctx->tune[t] = 0;          // default tiles
+    int rc = STZ_OK;
+    if (ctx->C == 3) {
+        STZ_LEN_DISPATCH(ctx->n[0], rc = Lr.template fwd_x_n<LL, 3>());
+    } else {
+        STZ_LEN_DISPATCH(ctx->n[0], rc = Lr.template fwd_x_n<LL, 6>());
+    }
+    for (int t = 0; t < 4; ++t) ctx->tune[t] = saved[t];
+    if (rc) return rc;
+    if (ctx->C == 6) {
+        for (int t = 0; t < 4; ++t) ctx->tune[t] = 0;
+        STZ_LEN_DISPATCH(ctx->n[1], rc = Lr.template ypass_n<LL>((R)-1));
+        for (int t = 0; t < 4; ++t) ctx->tune[t] = saved[t];
+        if (rc) return rc;
+    }
+    if (ctx->C == 3) {
+        STZ_LEN_DISPATCH(ctx->n[1], {
+            constexpr int W = fused_cols_for<R, 3>(LL);
+            constexpr int NTF = fused_threads(LL * 3 * W);
+            hipLaunchKernelGGL((k_fused<R, LL, W, 3, 2, NTF, 1>), dim3((k.h0 + W - 1) / W, 1, 1), dim3(NTF), 0,
+                               ctx->stream, (cpx<R> *)ctx->spec, (const cpx<R> *)ctx->tw.tw[1], k, nullptr);
+        });
+    } else {
+        STZ_LEN_DISPATCH(ctx->n[2], {
+            constexpr int W = fused_cols_for<R, 6>(LL);
+            constexpr int NTF = fused_threads(LL * 6 * W);
+            hipLaunchKernelGGL((k_fused<R, LL, W, 6, 3, NTF, 1>), dim3((k.h0 + W - 1) / W, k.n[1], 1), dim3(NTF), 0,
+                               ctx->stream, (cpx<R> *)ctx->spec, (const cpx<R> *)ctx->tw.tw[2], k, nullptr);
+        });
+    }
+    STZ_CHECK_LAUNCH();
+    if (ctx->C == 6) {
+        for (int t = 0; t < 4; ++t) ctx->tune[t] = 0;
+        STZ_LEN_DISPATCH(ctx->n[1], rc = Lr.template ypass_n<LL>((R)1));
+        for (int t = 0; t < 4; ++t) ctx->tune[t] = saved[t];
+        if (rc) return rc;
+    }
+    const int dim = ctx->dim;
+    auto go = [&](auto L0c) -> int {
+        constexpr int L0 = decltype(L0c)::value;
+        const int NR = ctx->C == 3 ? Lr.pick_rows(rows_for<R, 3>(L0)) : Lr.pick_rows(rows_for<R, 6>(L0));
+        if (!NR) return STZ_E_ARG;
+        const dim3 grid(k.rows / NR);
+        // NR is the default x-pass row count for C components (<= 32 KB of LDS), so the
+        // DIM <= C component tile of each reachable variant fits
+        auto launch = [&](auto NRc) -> int {
+            constexpr int NRV = decltype(NRc)::value;
+            if constexpr ((long long)L0 * (3 * NRV / 2 + 1) * (long long)sizeof(cpx<R>) <= 65536) {
+                if (dim == 2)
+                    hipLaunchKernelGGL((k_inv_x_disp<R, L0, NRV, 2>), grid, dim3(NT_SOLVE), 0, ctx->stream,
+                                       (const cpx<R> *)ctx->spec, u, (const cpx<R> *)ctx->tw.tw[0], k);
+                else
+                    hipLaunchKernelGGL((k_inv_x_disp<R, L0, NRV, 3>), grid, dim3(NT_SOLVE), 0, ctx->stream,
+                                       (const cpx<R> *)ctx->spec, u, (const cpx<R> *)ctx->tw.tw[0], k);
+                return STZ_OK;
+            } else {
+                ctx->err = "no compiled displacement x-pass for this grid";
+                return STZ_E_UNSUPPORTED;
+            }
+        };
+        switch (NR) {
+        case 32: return launch(std::integral_constant<int, 32>());
+        case 16: return launch(std::integral_constant<int, 16>());
+        case 8: return launch(std::integral_constant<int, 8>());
+        case 4: return launch(std::integral_constant<int, 4>());
+        case 2: return launch(std::integral_constant<int, 2>());
+        default: return STZ_E_ARG;
+        }
+    };
+    STZ_LEN_DISPATCH(ctx->n[0], rc = go(std::integral_constant<int, LL>()));
+    if (rc) return rc;
+    STZ_CHECK_LAUNCH();
+    return STZ_OK;
+}
+
 template <typename R>
 int upload_twiddles(stz_ctx *ctx, int axis, int L)
 {
@@ -881,6 +1137,11 @@ int stz_solve_launch(stz_ctx *ctx, int gated)
 {
     return ctx->prec == STZ_FP64 ? solve_impl<double>(ctx, gated, 1, nullptr, ctx->stream, nullptr)
                                  : solve_impl<float>(ctx, gated, 1, nullptr, ctx->stream, nullptr);
+}
+
+int stz_displacement_launch(stz_ctx *ctx, double *u)
+{
+    return ctx->prec == STZ_FP64 ? disp_impl<double>(ctx, u) : disp_impl<float>(ctx, u);
 }
 
 int stz_solve_launch_batched(stz_ctx *proto, int nrep, const SolveRep *reps, hipStream_t stream, int *nparts)

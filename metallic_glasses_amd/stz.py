@@ -136,6 +136,9 @@ def lib():
         L.stz_copy_probe.argtypes = [C.c_longlong, C.c_int, dp, dp]
         L.stz_stress_update_local.argtypes = [vp, C.c_int, P(C.c_int)]
         L.stz_set_solver.argtypes = [vp, C.c_int, C.c_longlong, C.c_int]
+        L.stz_checkpoint_save.argtypes = [vp, C.c_char_p]
+        L.stz_displacement.argtypes = [vp, dp]
+        L.stz_checkpoint_load.argtypes = [vp, C.c_char_p]
         _lib = L
     return _lib
 
@@ -236,6 +239,19 @@ class Stz:
         """Event-local stress update after transforms at `sites` (see include/stz.h)."""
         a = (C.c_int * max(len(sites), 1))(*[int(x) for x in sites])
         self._check(lib().stz_stress_update_local(self._h, len(sites), a))
+
+    def displacement(self):
+        """Print-step displacement field u[dim][N] (see include/stz.h)."""
+        u = np.empty(self.dim * self.N, dtype=np.float64)
+        self._check(lib().stz_displacement(self._h, _dp(u)))
+        return u
+
+    def checkpoint_save(self, path):
+        """Writes everything the next step reads (see include/stz.h)."""
+        self._check(lib().stz_checkpoint_save(self._h, os.fsencode(path)))
+
+    def checkpoint_load(self, path):
+        self._check(lib().stz_checkpoint_load(self._h, os.fsencode(path)))
 
     def set_solver(self, mode, resync=1024, local_max=16):
         """Stress update of the resident step: SOLVER_FFT or SOLVER_LOCAL."""

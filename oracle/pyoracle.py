@@ -92,6 +92,7 @@ def lib():
         L.ora_step.argtypes = [C.c_void_p, P(OraStepInfo)]
         L.ora_set_print.argtypes = [C.c_void_p, C.c_int, C.c_double]
         L.ora_set_strainhist.argtypes = [C.c_void_p, C.c_double]
+        L.ora_displacement.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
         L.ora_set_threads.argtypes = [C.c_int]
         L.ora_set_threads.restype = C.c_int
         _lib = L
@@ -171,6 +172,12 @@ class Oracle:
 
     def solve(self):
         lib().ora_stress_solve(self._h)
+
+    def displacement(self):
+        """Print-step re-solve and displacement field u[dim][N]."""
+        u = np.zeros(self.params.dim * self.N)
+        lib().ora_displacement(self._h, u.ctypes.data_as(C.POINTER(C.c_double)))
+        return u
 
     def softening(self):
         lib().ora_softening(self._h)

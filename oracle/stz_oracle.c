@@ -31,6 +31,11 @@
  *                     compatible eps0 -> zero fluctuation stress; laminate
  *                     closed form; k.sigma_hat = 0 (equilibrium); brute-force
  *                     dense Green's-operator solve on a 4x4 / 4x4x4 grid.
+ *   ora_displacement  print-step InhomogeneousStressSolver(uflag=2), STZ/bmg.c:289-295,
+ *                     STZ/elastic.c:873-900 + helpers :754-777.
+ *                     pins: the symmetric spectral gradient of u equals the solved
+ *                     strain fluctuation on every non-Nyquist k; eps0 = 0 leaves the
+ *                     position term alone (closed form).
  *   ora_softening     SofteningFactor default branch, STZ/init.c:711-739.
  *   ora_act_energy    ActEnergy with SF_SQUARE_FWD, MICRO_MEMORY,
  *                     NO_SF_XTAL_MRO, STZ/init.c:561-690; Q<0 count
@@ -695,6 +700,91 @@ void ora_stress_solve(ora_state *s)
     hom_solve(s);
     /* Tau1o = 0 again -> epso = C0^-1 Tau0 == epsolast, mix 0.5 leaves it, convg = 0 -> break
        (STZ/elastic.c:1027-1058).  The stress field and <sigma> of hom_solve are final. */
+}
+
+/* Print-step displacement: InhomogeneousStressSolver(..., uflag=2) (STZ/bmg.c:289-295)
+   re-solves the stress and then runs HomogeneousStressSolver's displacement branch on the
+   same round-tripped field (STZ/elastic.c:1050-1054, 873-900): per k,
+   tmp_a = F_a + (-alpha)*(G*n_a) (HadamardSumnSProd, :765-777) and
+   u_hat_a = (tmp_a.im*(1/mu)/|k|, -1.0*tmp_a.re*(1/mu)/|k|) (HadamardSiDivd, :754-763),
+   u_a = Re(IFFT)/N + the supercell position (:893-899).  The reference adds the 2-D
+   position with its slow index i along x and the fast index j along y
+   (uo[0] += i*(1+e_xx)*LX + j*e_xy*LY); kept as written.  3-D: u_a += sum_b x_b
+   (delta_ab + e_ab) L_b with x_b the grid index along axis b (x = the fast axis). */
+void ora_displacement(ora_state *s, double *u)
+{
+    ora_stress_solve(s);
+    const long N = s->N;
+    const int C = s->C, D = s->p.dim;
+    const int *n = s->p.n;
+    const double lambda = s->lambda, mu = s->mu;
+    const double alpha = (lambda + mu) / (lambda + 2.0 * mu);
+    double *sp = s->spec;
+    for (int c = 0; c < C; c++) {
+        double *f = sp + 2 * (long)c * N;
+#pragma omp parallel for schedule(static)
+        for (long v = 0; v < N; v++) { f[2 * v] = s->epsolast[(long)c * N + v]; f[2 * v + 1] = 0.; }
+        ora_fft3d(f, n[0], n[1], n[2], -1);
+    }
+    const double sdiv = 1.0 / mu;
+#pragma omp parallel for schedule(static)
+    for (long v = 0; v < N; v++) {
+        const int idx[3] = {(int)(v % n[0]), (int)((v / n[0]) % n[1]), (int)(v / ((long)n[0] * n[1]))};
+        double kk[3] = {0, 0, 0};
+        for (int a = 0; a < D; a++) {
+            double sa = (double)(idx[a]) * (double)(1.0 / n[a]);
+            kk[a] = (sa - round(sa)) * ORA_TWO_PI;
+        }
+        double g2 = D == 2 ? sqrt(kk[0] * kk[0] + kk[1] * kk[1]) : sqrt(kk[0] * kk[0] + kk[1] * kk[1] + kk[2] * kk[2]);
+        if (idx[0] + idx[1] + idx[2] == 0) g2 = 1.0;
+        const double g[3] = {s->g[0][v], s->g[1][v], s->g[2][v]};
+        double tmp[2][3];
+        for (int ri = 0; ri < 2; ri++) {
+            double e[6] = {0}, t[6];
+            for (int c = 0; c < C; c++) e[c] = sp[2 * ((long)c * N + v) + ri];
+            iso_mul(s, lambda, mu, e, t);                 /* IsotropicMul_C :802 */
+            double F[3], G;
+            if (C == 3) {
+                F[0] = t[0] * g[0] + t[2] * g[1];         /* :804 */
+                F[1] = t[2] * g[0] + t[1] * g[1];         /* :805 */
+                G = F[0] * g[0] + F[1] * g[1];            /* :806 */
+            } else {
+                F[0] = t[0] * g[0] + t[5] * g[1] + t[4] * g[2];
+                F[1] = t[5] * g[0] + t[1] * g[1] + t[3] * g[2];
+                F[2] = t[4] * g[0] + t[3] * g[1] + t[2] * g[2];
+                G = F[0] * g[0] + F[1] * g[1] + F[2] * g[2];
+            }
+            for (int a = 0; a < D; a++) tmp[ri][a] = F[a] + -1.0 * alpha * (G * g[a]);
+        }
+        for (int a = 0; a < D; a++) {
+            sp[2 * ((long)a * N + v)] = tmp[1][a] * sdiv / g2;
+            sp[2 * ((long)a * N + v) + 1] = -1.0 * tmp[0][a] * sdiv / g2;
+        }
+    }
+    const double L[3] = {s->p.lx, s->p.ly, s->p.lz};
+    for (int a = 0; a < D; a++) {
+        double *f = sp + 2 * (long)a * N;
+        ora_fft3d(f, n[0], n[1], n[2], +1);
+#pragma omp parallel for schedule(static)
+        for (long v = 0; v < N; v++) {
+            const int idx[3] = {(int)(v % n[0]), (int)((v / n[0]) % n[1]), (int)(v / ((long)n[0] * n[1]))};
+            const double *e = s->epsavg;
+            double x = f[2 * v] / (double)N;
+            if (D == 2) {
+                const double i = (double)idx[1], j = (double)idx[0];
+                if (a == 0) x += i * (1 + e[0]) * L[0] + j * e[2] * L[1];
+                else x += i * e[2] * L[0] + j * (1 + e[1]) * L[1];
+            } else {
+                /* e_ab in Voigt order (xx, yy, zz, yz, xz, xy) */
+                const int vo[3][3] = {{0, 5, 4}, {5, 1, 3}, {4, 3, 2}};
+                double pos = 0.0;
+                for (int b = 0; b < 3; b++)
+                    pos = pos + (double)idx[b] * ((a == b ? 1.0 : 0.0) + e[vo[a][b]]) * L[b];
+                x += pos;
+            }
+            u[(long)a * N + v] = x;
+        }
+    }
 }
 
 /* ------------------------------------------------------------------ */
