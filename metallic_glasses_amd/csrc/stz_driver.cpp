@@ -6,7 +6,8 @@
 // and stz_step_print, and writes the reference's
 // outputs: actQ.out (per thermal event), SS.out, SOC.out, ElastSOC.out,
 // ElasticEnergy.out, EA.out, EpsAvg.out, avg.out, and the Strain%03d_P00.out /
-// Stress%03d_P00.out field dumps at print steps.
+// Stress%03d_P00.out field dumps and the X1R/X2R (X3R in 3-D) displacement records at
+// print steps.
 //
 // usage: stz_run [input-file]      (default "input", as STZ/bmg.c:31)
 #include <cmath>
@@ -224,7 +225,7 @@ int main(int argc, char **argv)
     FILE *fq = std::fopen("actQ.out", "w");
     std::vector<double> strainhist{0.0}, stresshist{0.0}, elhist{0.0};
     Soc soc;
-    std::vector<double> eps0((size_t)(C * N)), sig((size_t)(C * N)), eq((size_t)N);
+    std::vector<double> eps0((size_t)(C * N)), sig((size_t)(C * N)), eq((size_t)N), disp((size_t)(in.dim * N));
     long long step = 0, astep = 0, kstep = 0, estep = 0;
     double prev_time = totime;
     // the main loop runs device-resident in batches; each batch returns after a print
@@ -293,6 +294,19 @@ int main(int argc, char **argv)
                 if (rc < 0) {
                     std::fprintf(stderr, "print re-solve failed (%d): %s\n", rc, stz_last_error(ctx));
                     return 2;
+                }
+                if (in.field_output) {
+                    // displacement record X1R/X2R (X3R in 3-D), STZ/bmg.c:286-330
+                    const int drc = stz_displacement(ctx, disp.data());
+                    if (drc < 0) {
+                        std::fprintf(stderr, "displacement failed (%d): %s\n", drc, stz_last_error(ctx));
+                        return 2;
+                    }
+                    static const char *names[3] = {"X1R", "X2R", "X3R"};
+                    for (int a = 0; a < in.dim; ++a) {
+                        std::vector<double> comp(disp.begin() + (size_t)a * N, disp.begin() + (size_t)(a + 1) * N);
+                        write_field(names[a], np - 1, comp, p.n[0], rows);
+                    }
                 }
                 write_summaries(strainhist, stresshist, elhist, soc, r.epsavg, C);
                 std::printf("step=%lld\nastep=%lld\nkstep=%lld\nestep=%lld\n", step, astep, kstep, estep);

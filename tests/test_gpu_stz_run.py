@@ -80,12 +80,15 @@ def test_driver_reproduces_oracle_outputs(tmp_path):
     qs = []
     stress = [0.0]
     strain = [0.0]
+    u1 = None
     while True:
         rc, info = o.step()
         if info.kind == 1:
             qs.append(info.q_chosen)
         if info.printed:
             strain.append(info.epsavg[2])
+            if u1 is None:
+                u1 = o.displacement()         # the first print block's X1R/X2R (index 001)
         if rc:
             break
     assert len(qs) == len(q) and len(qs) > 20
@@ -97,3 +100,8 @@ def test_driver_reproduces_oracle_outputs(tmp_path):
         assert (tmp_path / name).exists(), name
     field = np.loadtxt(tmp_path / "Strain001_P00.out")
     assert field.shape == (32, 32)
+    # displacement records at the first print step: %lf text, 6 decimals
+    for a, name in enumerate(("X1R001_P00.out", "X2R001_P00.out")):
+        x = np.loadtxt(tmp_path / name)
+        assert x.shape == (32, 32)
+        assert np.max(np.abs(x.ravel() - u1[a * 1024:(a + 1) * 1024])) <= 1.5e-6, name
