@@ -33,7 +33,7 @@ extern "C" {
 #define STZ_E_HIP (-2)         /* HIP runtime error */
 #define STZ_E_ARG (-3)         /* bad argument / unsupported configuration */
 #define STZ_E_NAN (-4)         /* NaN in the stress solve (STZ/elastic.c:212-215) */
-#define STZ_E_UNSUPPORTED (-5) /* crystal-like MRO voxel drawn (out of scope) */
+#define STZ_E_UNSUPPORTED (-5) /* crystal-like MRO voxel drawn, or an unsupported combination */
 
 #define STZ_FP64 0
 #define STZ_MIXED 1
@@ -132,10 +132,27 @@ int stz_download(stz_ctx *ctx, int field, double *host);
 int stz_set_scalars(stz_ctx *ctx, const double epsavg[6], const double stress[6], double totime);
 int stz_get_scalars(stz_ctx *ctx, double epsavg[6], double stress[6], double *totime);
 
-/* InhomogeneousStressSolver(..., uflag=0) with uniform moduli (STZ/elastic.c:977-1066,
-   779-828): sigma field and <sigma> from EPS0 and EpsAvg.  Discards a pending uniform
+/* InhomogeneousStressSolver(..., uflag=0) (STZ/elastic.c:977-1066, 779-828): sigma field
+   and <sigma> from EPS0 and EpsAvg.  With uniform moduli one homogeneous solve; with
+   per-voxel moduli (stz_set_moduli) the reference's fixed-point loop of homogeneous solves
+   (mix 0.5, until the RMS change of the reference-medium eigenstrain is below
+   1e-3 |EpsAvg| or |EpsAvg| < 1e-4, at most 101 solves).  Discards a pending uniform
    stress increment (the reference's solve overwrites Sigma). */
 int stz_stress_solve(stz_ctx *ctx);
+
+/* Per-voxel Lame moduli Lambda(x), Mu(x) (N doubles each, voxel order) of an elastically
+   inhomogeneous glass: the Lambda/Mu arguments of every InhomogeneousStressSolver call
+   (STZ/bmg.c:93, 121, 292; STZ/evolution.c:83, 253), set in the reference by
+   STZ/init.c:457-465 (there uniform: Lambda = lambda, Mu = mu).  lambda = mu = NULL
+   returns to uniform moduli.  The reference medium C0 stays (lambda, mu) of the
+   parameters.  Every later stress solve runs the fixed-point loop; steps run on the host
+   path (stz_step; stz_run_steps falls back to it); the event-local update and ensembles
+   return STZ_E_UNSUPPORTED.  Device memory: (2 + 3C) doubles per voxel. */
+int stz_set_moduli(stz_ctx *ctx, const double *lambda, const double *mu);
+
+/* Homogeneous solves the last stress solve took (1 with uniform moduli).  No counterpart
+   in the reference (its loop counter `step`, STZ/elastic.c:1014-1015). */
+int stz_inhom_iterations(stz_ctx *ctx);
 
 /* Event-local form of the same solve, replacing InhomogeneousStressSolver(uflag=0) after
    transforms at `n` distinct sites (STZ/evolution.c:70-89, 250-259): eps0 has already changed by
@@ -156,7 +173,9 @@ int stz_stress_update_local(stz_ctx *ctx, int n, const int *sites);
    u = Re(IFFT)/N + the supercell position (2-D as the reference writes it, its slow
    index along x; 3-D: x along the fast axis).  host_u receives dim*N doubles
    [component][voxel] (the X1R/X2R outputs, STZ/bmg.c:300-330).  The stress is not
-   changed. */
+   changed, except with per-voxel moduli: then the call first re-solves (the sigma field
+   and <sigma>), as the reference's uflag=2 call does, and u comes from the eigenstrain
+   the fixed point ends on (its epsolast, STZ/elastic.c:1051-1055). */
 int stz_displacement(stz_ctx *ctx, double *host_u);
 
 /* How the resident step (stz_run_steps, stz_ensemble_run) brings the stress up to date

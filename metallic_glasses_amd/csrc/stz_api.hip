@@ -227,7 +227,8 @@ int solve(stz_ctx *ctx)
     ctx->pend_delta = 0.0;
     for (int c = 0; c < 6; ++c) ctx->soff[c] = 0.0;
     STZ_HIP(hipMemsetAsync(ctx->res + 15, 0, sizeof(double), ctx->stream));
-    int rc = stz_solve_launch(ctx, 0);
+    // per-voxel moduli: the fixed-point loop of homogeneous solves, else one solve
+    int rc = ctx->lamx ? stz_inhom_run(ctx) : stz_solve_launch(ctx, 0);
     if (rc) return rc;
     STZ_HIP(hipMemcpyAsync(ctx->hres, ctx->res, 16 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     STZ_HIP(hipStreamSynchronize(ctx->stream));
@@ -511,6 +512,7 @@ void stz_destroy(stz_ctx *ctx)
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     stz_resident_free(ctx);
     stz_local_release(ctx);
+    stz_inhom_free(ctx);
     void *ptrs[] = {ctx->q0eff, ctx->scache, ctx->eps0, ctx->sigma, ctx->q0, ctx->deps, ctx->deps_last, ctx->perm, ctx->temp, ctx->tlast,
                     ctx->spec, ctx->minq, ctx->argm, ctx->bsum, ctx->gsum, ctx->res, ctx->spart,
                     ctx->acnt, ctx->aoff, ctx->alist, ctx->dnew, ctx->socp, ctx->gneg};
@@ -682,6 +684,10 @@ int stz_stress_update_local(stz_ctx *ctx, int n, const int *sites)
         for (int j = 0; j < i; ++j)
             if (sites[j] == sites[i]) return STZ_E_ARG;     // each site's recorded strain is added once
     }
+    if (ctx->lamx) {
+        ctx->err = "the event-local stress update assumes uniform moduli";
+        return STZ_E_UNSUPPORTED;
+    }
     if (ctx->nlocal >= (1ll << 59)) {
         ctx->err = "the stress field has not been solved since its inputs were replaced";
         return STZ_E_ARG;
@@ -722,6 +728,12 @@ int stz_stress_update_local(stz_ctx *ctx, int n, const int *sites)
 int stz_displacement(stz_ctx *ctx, double *host_u)
 {
     if (!ctx || !host_u) return STZ_E_ARG;
+    if (ctx->lamx) {
+        // the displacement comes from the eigenstrain the fixed point ends on: solve first,
+        // as the reference's print-step InhomogeneousStressSolver(uflag=2) call does
+        const int rc = solve(ctx);
+        if (rc) return rc;
+    }
     const size_t bytes = (size_t)ctx->dim * (size_t)ctx->N * 8;
     double *u = nullptr;
     STZ_HIP(hipMalloc((void **)&u, bytes));
@@ -739,6 +751,10 @@ int stz_set_solver(stz_ctx *ctx, int mode, long long resync, int local_max)
     if (!ctx || (mode != STZ_SOLVER_FFT && mode != STZ_SOLVER_LOCAL) || resync < 1 || local_max < 1 ||
         local_max > STZ_LOCAL_MAXSRC)
         return STZ_E_ARG;
+    if (mode == STZ_SOLVER_LOCAL && ctx->lamx) {
+        ctx->err = "the event-local stress update assumes uniform moduli";
+        return STZ_E_UNSUPPORTED;
+    }
     if (mode == STZ_SOLVER_LOCAL) {
         const int rc = stz_local_prepare(ctx);
         if (rc) return rc;
@@ -866,8 +882,10 @@ int stz_step_kmc(stz_ctx *ctx, stz_step_result *res)
     if (rc) return rc;
     rc = stz_athermal_fused_launch(ctx, 1);       // athermal: compaction + transforms
     if (rc) return rc;
-    rc = stz_solve_launch(ctx, 1);                // thermal or athermal: stress solve
-    if (rc) return rc;
+    if (!ctx->lamx) {
+        rc = stz_solve_launch(ctx, 1);            // thermal or athermal: stress solve
+        if (rc) return rc;
+    }
     STZ_HIP(hipMemcpyAsync(ctx->hres, ctx->res, 32 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     rc = spin_sync(ctx);
     if (rc) return rc;
@@ -883,7 +901,10 @@ int stz_step_kmc(stz_ctx *ctx, stz_step_result *res)
         ctx->err = "KMC failed to locate an activation mode";
         return STZ_E_SELECT;
     }
-    if (kind == KIND_THERMAL || kind == KIND_ATHERMAL) {
+    if ((kind == KIND_THERMAL || kind == KIND_ATHERMAL) && ctx->lamx) {
+        rc = solve(ctx);                          // the inhomogeneous loop decides on the host
+        if (rc) return rc;
+    } else if (kind == KIND_THERMAL || kind == KIND_ATHERMAL) {
         int nan;
         std::memcpy(&nan, ctx->hres + 15, sizeof(int));
         if (nan) {

@@ -6,7 +6,8 @@
 // holds everything the next step reads: the generator (624 words + position), every
 // per-voxel array in its storage precision (bit for bit), EpsAvg, <sigma>, the clock,
 // the pending uniform stress increment, the print/termination state, the counters and
-// the event-local update's bookkeeping.  Derived arrays (the Q0*exp(-D*S) cache, min Q,
+// the event-local update's bookkeeping, and per-voxel moduli when they are set (the
+// reference keeps them fixed for a run, STZ/init.c:457-465).  Derived arrays (the Q0*exp(-D*S) cache, min Q,
 // block sums) are rebuilt by the next rate pass.  A run restarted from a checkpoint
 // continues exactly as the saved run does.
 #include "stz_internal.h"
@@ -23,7 +24,7 @@ constexpr char kMagic[8] = {'S', 'T', 'Z', 'C', 'K', 'P', 'T', '1'};
 struct CkptHeader {
     char magic[8];
     int32_t version;
-    int32_t dim, n0, n1, n2, modes, precision, C, S, rsz;
+    int32_t dim, n0, n1, n2, modes, precision, C, S, rsz, inhom;
     int64_t N;
 };
 
@@ -45,7 +46,7 @@ struct Field {
 std::vector<Field> fields_of(stz_ctx *c)
 {
     const size_t N = (size_t)c->N, R = c->rsz;
-    return {
+    std::vector<Field> f = {
         {c->eps0, (size_t)c->C * N * 8},
         {c->sigma, (size_t)c->C * N * R},
         {c->q0, (size_t)c->M * N * R},
@@ -55,6 +56,11 @@ std::vector<Field> fields_of(stz_ctx *c)
         {c->temp, N * 8},
         {c->tlast, N * 8},
     };
+    if (c->lamx) {
+        f.push_back({c->lamx, N * 8});
+        f.push_back({c->mux, N * 8});
+    }
+    return f;
 }
 
 CkptHeader header_of(const stz_ctx *c)
@@ -62,7 +68,7 @@ CkptHeader header_of(const stz_ctx *c)
     CkptHeader h;
     std::memset(&h, 0, sizeof(h));
     std::memcpy(h.magic, kMagic, 8);
-    h.version = 1;
+    h.version = 2;
     h.dim = c->dim;
     h.n0 = c->n[0];
     h.n1 = c->n[1];
@@ -72,6 +78,7 @@ CkptHeader header_of(const stz_ctx *c)
     h.C = c->C;
     h.S = c->S;
     h.rsz = (int32_t)c->rsz;
+    h.inhom = c->lamx ? 1 : 0;
     h.N = c->N;
     return h;
 }
@@ -144,11 +151,13 @@ int stz_checkpoint_load(stz_ctx *ctx, const char *path)
     if (!f) return fail(ctx, nullptr, "cannot open the checkpoint file", STZ_E_ARG);
     CkptHeader h;
     CkptScalars s;
-    if (std::fread(&h, sizeof(h), 1, f) != 1 || std::memcmp(h.magic, kMagic, 8) != 0 || h.version != 1)
+    if (std::fread(&h, sizeof(h), 1, f) != 1 || std::memcmp(h.magic, kMagic, 8) != 0 || h.version != 2)
         return fail(ctx, f, "not a checkpoint file of this library", STZ_E_ARG);
     const CkptHeader mine = header_of(ctx);
     if (std::memcmp(&h, &mine, sizeof(h)) != 0)
-        return fail(ctx, f, "the checkpoint was written by a context of another grid, mode count or precision",
+        return fail(ctx, f,
+                    "the checkpoint was written by a context of another grid, mode count, precision or "
+                    "moduli kind (uniform or per voxel: call stz_set_moduli first)",
                     STZ_E_ARG);
     if (std::fread(&s, sizeof(s), 1, f) != 1 || s.rng_pos < 0 || s.rng_pos > 624 || s.pend_comp < -1 ||
         s.pend_comp >= ctx->C)

@@ -40,6 +40,8 @@
 // resident step: how the decided step's stress is brought up to date (ints)
 #define RES_FFT 34            // step kind when the full FFT solve runs, else KIND_NONE (the solve's gate)
 #define RES_LOCAL 35          // 1 when the event-local update runs instead
+// inhomogeneous solve: sum of the squared eigenstrain change of the last iteration (double)
+#define RES_CONV 40
 
 // halt states of a resident replica
 #define HALT_NONE 0
@@ -209,10 +211,7 @@ struct stz_ctx {
     long long avlan = 0;
     double soc_elast = 0.0;
 
-    // marks the derived per-voxel arrays stale after a restore (the next rate pass rebuilds them)
-int stz_invalidate_derived(stz_ctx *ctx);
-
-// event-local stress update (stz_local.hip): impulse response [S][C][N] in the
+    // event-local stress update (stz_local.hip): impulse response [S][C][N] in the
     // storage precision, shared by contexts of equal shape and elasticity
     int solver_mode = STZ_SOLVER_FFT;
     long long local_resync = 1024;
@@ -222,6 +221,14 @@ int stz_invalidate_derived(stz_ctx *ctx);
     double ebase[6] = {0, 0, 0, 0, 0, 0};   // mean strain the field was last solved with
     double usum[6] = {0, 0, 0, 0, 0, 0};    // FP64: uniform increments added to the field since
     long long nlocal = 1ll << 60;           // local updates since the last full solve (huge: none allowed)
+
+    // elastic inhomogeneity (stz_inhom.hip): Lambda(x), Mu(x) in FP64; nullptr = uniform
+    double *lamx = nullptr, *mux = nullptr;
+    double *tau0 = nullptr;                 // C(x):eps0 [C][N]
+    double *estar[2] = {nullptr, nullptr};  // reference-medium eigenstrain, in/out of an iteration [C][N]
+    double *cpart = nullptr;                // per-block sums of the squared eigenstrain change [rows]
+    int inhom_cur = 0;                      // estar slot solved last (the reference's epsolast)
+    int inhom_iters = 1;                    // homogeneous solves of the last stress solve
 
     int tune[4] = {0, 0, 0, 0};     // solve tile choices (x fwd rows, y width, fused width, x inv rows); 0: default
     long long tune_epoch = 0;       // bumped when the choices change (captured graphs are rebuilt)
@@ -467,6 +474,11 @@ int stz_fft_setup(stz_ctx *ctx);
 void stz_fft_free(stz_ctx *ctx);
 int stz_solve_launch(stz_ctx *ctx, int gated); // enqueues the full stress solve on ctx->stream
 int stz_displacement_launch(stz_ctx *ctx, double *u);   // print-step displacement into u[dim][N] (device)
+int stz_solve_spectral(stz_ctx *ctx);   // y passes + fused Gamma-hat pass of ctx->spec (ungated)
+// elastic inhomogeneity (stz_inhom.hip)
+int stz_inhom_fwd_x(stz_ctx *ctx, const double *src);   // R2C x pass of an eigenstrain [C][N], no round trip
+int stz_inhom_run(stz_ctx *ctx);        // the fixed-point loop; <sigma> in res[0..C), sigma written
+int stz_inhom_free(stz_ctx *ctx);
 int stz_rates_launch(stz_ctx *ctx, int decide); // enqueues rates + group sums (+ step-kind decision)
 int stz_select_launch(stz_ctx *ctx, double target, int gated);
 int stz_apply_launch(stz_ctx *ctx, int site, int mode, int gated);

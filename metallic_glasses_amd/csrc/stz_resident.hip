@@ -1309,6 +1309,27 @@ int stz_run_steps(stz_ctx *ctx, int nsteps, stz_step_result *results, int *done)
 {
     if (!ctx || nsteps < 0) return STZ_E_ARG;
     int d = 0, rc = 0;
+    if (ctx->lamx) {
+        // per-voxel moduli: the fixed-point loop ends on a host decision each iteration,
+        // so every step runs on the host path (same random numbers, same results)
+        const bool stop_at_print = ctx->rs && ctx->rs->stop_at_print;
+        for (int i = 0; i < nsteps; ++i) {
+            stz_step_result r;
+            int hrc = stz_step_kmc(ctx, &r);
+            if (hrc >= 0 && r.printed && !stop_at_print) hrc = stz_step_print(ctx, &r);
+            if (hrc >= 0) {
+                if (results) results[d] = r;
+                ++d;
+            }
+            if (hrc != 0) {
+                rc = hrc;
+                break;
+            }
+            if (r.printed && stop_at_print) break;
+        }
+        if (done) *done = d;
+        return rc;
+    }
     if (nsteps == 0) {                  // prepare: resident resources and the lookup window
         rc = ensure_resident(ctx);
         if (!rc) rc = push_state(ctx, 0);
@@ -1337,6 +1358,11 @@ int stz_ensemble_create(stz_ctx *const *members, int n, stz_ensemble **out)
     if (!members || n < 1 || !out) return STZ_E_ARG;
     for (int i = 0; i < n; ++i)
         if (!members[i]) return STZ_E_ARG;
+    for (int i = 0; i < n; ++i)
+        if (members[i]->lamx) {
+            members[i]->err = "an ensemble steps uniform-moduli replicas only";
+            return STZ_E_UNSUPPORTED;
+        }
     *out = nullptr;
     stz_ensemble *e = new stz_ensemble();
     int rc = stz_plan_create(members, n, &e->pl);
@@ -1373,6 +1399,11 @@ int stz_ensemble_config(stz_ensemble *e, int batch, int use_graph)
 int stz_ensemble_run(stz_ensemble *e, int nsteps, stz_step_result *const *results, int *done, int *rcs)
 {
     if (!e || nsteps < 0 || !done || !rcs) return STZ_E_ARG;
+    for (stz_ctx *c : e->pl->all)
+        if (c->lamx) {
+            c->err = "an ensemble steps uniform-moduli replicas only";
+            return STZ_E_UNSUPPORTED;
+        }
     return stz_plan_run(e->pl, nsteps, results, done, rcs, 0);
 }
 

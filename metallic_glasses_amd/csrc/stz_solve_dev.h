@@ -1,0 +1,544 @@
+// stz_solve_dev.h -- device pieces shared by the stress-solve translation units: the
+// launch constants, the per-replica view of a batched launch, the elementwise elasticity
+// of the reference (literal expression order), and the two x-axis passes (R2C + round
+// trip, C2R + stress epilogue) in their uniform (MODE 0) and inhomogeneous (MODE 1) forms.
+#pragma once
+#include "stz_internal.h"
+
+// compile-time length of a runtime axis length (the supported FFT lengths)
+#define STZ_LEN_DISPATCH(L, ...)                                \
+    switch (L) {                                                \
+    case 4: { constexpr int LL = 4; __VA_ARGS__; } break;       \
+    case 8: { constexpr int LL = 8; __VA_ARGS__; } break;       \
+    case 16: { constexpr int LL = 16; __VA_ARGS__; } break;     \
+    case 32: { constexpr int LL = 32; __VA_ARGS__; } break;     \
+    case 64: { constexpr int LL = 64; __VA_ARGS__; } break;     \
+    case 96: { constexpr int LL = 96; __VA_ARGS__; } break;     \
+    case 128: { constexpr int LL = 128; __VA_ARGS__; } break;   \
+    case 256: { constexpr int LL = 256; __VA_ARGS__; } break;   \
+    case 512: { constexpr int LL = 512; __VA_ARGS__; } break;   \
+    default: return STZ_E_ARG;                                  \
+    }
+
+namespace {
+
+constexpr int NT_SOLVE = 256;
+
+struct SolveConsts {
+    const int *gate;    // fused step: run only when the step kind is thermal
+    double lambda, mu, alpha, s0, sh, sa, invN;
+    double epsavg[6];
+    int n[3];
+    long long N;
+    int rows;           // n1*n2
+    int h0, hp;
+    int C;
+    double L[3];        // voxel edges (displacement position term)
+    // fixed-point iteration of the inhomogeneous solve (x passes of MODE 1 only)
+    const double *tau0; // C(x):eps0 [C][N]
+    const double *lamx, *mux;   // Lambda(x), Mu(x) [N]
+    double *eout;       // next reference-medium eigenstrain [C][N]
+    double *cpart;      // per-block sums of its squared change
+};
+
+// Per-replica view of a batched launch: the replica index is a grid coordinate, and
+// its fields, uniform strain and step kind come from reps[r].  With reps == nullptr the
+// launch is the single context's own (pointers from the arguments).
+template <typename R>
+struct View {
+    const double *eps0;
+    cpx<R> *spec;
+    R *sigma;
+    double *spart;
+    int *nanflag;
+    const int *gate;
+    double ea[6];
+};
+
+template <typename R>
+__device__ __forceinline__ View<R> view_of(const SolveRep *reps, int r, const SolveConsts &k, const double *eps0,
+                                           cpx<R> *spec, R *sigma, double *spart, int *nanflag)
+{
+    View<R> v;
+    if (reps) {
+        const SolveRep &q = reps[r];
+        v.eps0 = q.eps0;
+        v.spec = (cpx<R> *)q.spec;
+        v.sigma = (R *)q.sigma;
+        v.spart = q.spart;
+        v.nanflag = reinterpret_cast<int *>(q.res + 15);
+        v.gate = reinterpret_cast<const int *>(q.res + RES_FFT);   // KIND_NONE when the step updates locally
+#pragma unroll
+        for (int c = 0; c < 6; ++c) v.ea[c] = q.epsavg[c];
+    } else {
+        v.eps0 = eps0;
+        v.spec = spec;
+        v.sigma = sigma;
+        v.spart = spart;
+        v.nanflag = nanflag;
+        v.gate = k.gate;
+#pragma unroll
+        for (int c = 0; c < 6; ++c) v.ea[c] = k.epsavg[c];
+    }
+    return v;
+}
+
+// --- elementwise elasticity (literal expression order of the reference) ----------
+#pragma clang fp contract(off)
+
+// Tau0 = C0:e (IsotropicMul_RF with uniform moduli, STZ/elastic.c:905-920), then
+// C0^-1(Tau0 + Tau1) with Tau1 = (C0 - C0):(e + epsavg) (IsotropicMul_RF_D :922-938,
+// IsotropicInv :940-955).
+template <int C>
+__device__ __forceinline__ void round_trip(const double *e, const double *ea, double lam, double mu, double *r)
+{
+    double t0[C], t1[C], tau[C];
+    double z[C];
+#pragma unroll
+    for (int c = 0; c < C; ++c) z[c] = e[c] + ea[c];
+    if constexpr (C == 3) {
+        double tra = lam * (e[0] + e[1]);
+        t0[0] = tra + 2 * mu * e[0];
+        t0[1] = tra + 2 * mu * e[1];
+        t0[2] = 2 * mu * e[2];
+        double tra1 = (lam - lam) * (z[0] + z[1]);
+        t1[0] = tra1 + 2 * (mu - mu) * z[0];
+        t1[1] = tra1 + 2 * (mu - mu) * z[1];
+        t1[2] = 2 * (mu - mu) * z[2];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) tau[c] = t0[c] + t1[c];
+        double tr = -1.0 * lam / 2.0 / mu / (2.0 * lam + 2.0 * mu) * (tau[0] + tau[1]);
+        r[0] = tr + tau[0] / 2.0 / mu;
+        r[1] = tr + tau[1] / 2.0 / mu;
+        r[2] = tau[2] / 2.0 / mu;
+    } else {
+        double tra = lam * ((e[0] + e[1]) + e[2]);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) t0[c] = tra + 2 * mu * e[c];
+#pragma unroll
+        for (int c = 3; c < 6; ++c) t0[c] = 2 * mu * e[c];
+        double tra1 = (lam - lam) * ((z[0] + z[1]) + z[2]);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) t1[c] = tra1 + 2 * (mu - mu) * z[c];
+#pragma unroll
+        for (int c = 3; c < 6; ++c) t1[c] = 2 * (mu - mu) * z[c];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) tau[c] = t0[c] + t1[c];
+        double tr = -1.0 * lam / 2.0 / mu / (3.0 * lam + 2.0 * mu) * ((tau[0] + tau[1]) + tau[2]);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) r[c] = tr + tau[c] / 2.0 / mu;
+#pragma unroll
+        for (int c = 3; c < 6; ++c) r[c] = tau[c] / 2.0 / mu;
+    }
+}
+
+// IsotropicMul_R2 (STZ/elastic.c:686-701): sigma = C0:d
+template <int C>
+__device__ __forceinline__ void stress_of(const double *d, double lam, double mu, double *s)
+{
+    if constexpr (C == 3) {
+        double tra = lam * (d[0] + d[1]);
+        s[0] = tra + 2.0 * mu * d[0];
+        s[1] = tra + 2.0 * mu * d[1];
+        s[2] = 2.0 * mu * d[2];
+    } else {
+        double tra = lam * ((d[0] + d[1]) + d[2]);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) s[c] = tra + 2.0 * mu * d[c];
+#pragma unroll
+        for (int c = 3; c < 6; ++c) s[c] = 2.0 * mu * d[c];
+    }
+}
+
+// Gamma-hat applied to one real part (re or im) of the eigenstrain spectrum:
+// IsotropicMul_C, HadamardProdnSum_C x3, HadamardProdnSSum_C x2, HadamardProd2nSSum_C
+// (STZ/elastic.c:802-809); g = unit k vector, gg its products.
+template <typename R, int C>
+__device__ __forceinline__ void gamma_part(const R *e, const R *g, const R *gg, R lam, R mu, R s0, R sh, R sa, R *o)
+{
+    R t[C];
+    if constexpr (C == 3) {
+        R tra = lam * (e[0] + e[1]);
+        t[0] = tra + 2 * mu * e[0];
+        t[1] = tra + 2 * mu * e[1];
+        t[2] = 2 * mu * e[2];
+        R F0 = t[0] * g[0] + t[2] * g[1];
+        R F1 = t[2] * g[0] + t[1] * g[1];
+        R G = F0 * g[0] + F1 * g[1];
+        o[0] = s0 * (F0 * g[0]) + sa * (G * gg[0]);
+        o[1] = s0 * (F1 * g[1]) + sa * (G * gg[1]);
+        o[2] = sh * (F0 * g[1] + F1 * g[0]) + sa * (G * gg[2]);
+    } else {
+        R tra = lam * ((e[0] + e[1]) + e[2]);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) t[c] = tra + 2 * mu * e[c];
+#pragma unroll
+        for (int c = 3; c < 6; ++c) t[c] = 2 * mu * e[c];
+        // Voigt (xx, yy, zz, yz, xz, xy)
+        R F0 = t[0] * g[0] + t[5] * g[1] + t[4] * g[2];
+        R F1 = t[5] * g[0] + t[1] * g[1] + t[3] * g[2];
+        R F2 = t[4] * g[0] + t[3] * g[1] + t[2] * g[2];
+        R G = F0 * g[0] + F1 * g[1] + F2 * g[2];
+        o[0] = s0 * (F0 * g[0]) + sa * (G * gg[0]);
+        o[1] = s0 * (F1 * g[1]) + sa * (G * gg[1]);
+        o[2] = s0 * (F2 * g[2]) + sa * (G * gg[2]);
+        o[3] = sh * (F1 * g[2] + F2 * g[1]) + sa * (G * gg[3]);
+        o[4] = sh * (F0 * g[2] + F2 * g[0]) + sa * (G * gg[4]);
+        o[5] = sh * (F0 * g[1] + F1 * g[0]) + sa * (G * gg[5]);
+    }
+}
+
+// unit vector of the reference k grid (Grad, STZ/init.c:116-137) for integer indices;
+// returns |k| (g2, set to 1 at k = 0)
+template <int DIM>
+__device__ __forceinline__ double unit_k(const int *idx, const int *n, double *g)
+{
+    g[0] = g[1] = g[2] = 0.0;
+#pragma unroll
+    for (int a = 0; a < DIM; ++a) {
+        double s = (double)(idx[a]) * (double)(1.0 / n[a]);
+        g[a] = (s - round(s)) * (2 * STZ_PI_REF);
+    }
+    double g2;
+    if constexpr (DIM == 2) g2 = sqrt(g[0] * g[0] + g[1] * g[1]);
+    else g2 = sqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]);
+    if (idx[0] + idx[1] + (DIM == 3 ? idx[2] : 0) == 0) g2 = 1.0;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) g[a] /= g2;
+    return g2;
+}
+
+// Displacement branch of HomogeneousStressSolver (uflag 1/2, STZ/elastic.c:873-887) on one
+// real part of the eigenstrain spectrum: F = (C0:e).n, G = n.F and
+// tmp_a = F_a + (-alpha)*(G*n_a) (HadamardSumnSProd, :765-777), a < DIM
+template <typename R, int C, int DIM>
+__device__ __forceinline__ void disp_part(const R *e, const R *g, R lam, R mu, R nalpha, R *o)
+{
+    R t[C], F[3], G;
+    if constexpr (C == 3) {
+        R tra = lam * (e[0] + e[1]);
+        t[0] = tra + 2 * mu * e[0];
+        t[1] = tra + 2 * mu * e[1];
+        t[2] = 2 * mu * e[2];
+        F[0] = t[0] * g[0] + t[2] * g[1];
+        F[1] = t[2] * g[0] + t[1] * g[1];
+        G = F[0] * g[0] + F[1] * g[1];
+    } else {
+        R tra = lam * ((e[0] + e[1]) + e[2]);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) t[c] = tra + 2 * mu * e[c];
+#pragma unroll
+        for (int c = 3; c < 6; ++c) t[c] = 2 * mu * e[c];
+        F[0] = t[0] * g[0] + t[5] * g[1] + t[4] * g[2];
+        F[1] = t[5] * g[0] + t[1] * g[1] + t[3] * g[2];
+        F[2] = t[4] * g[0] + t[3] * g[1] + t[2] * g[2];
+        G = F[0] * g[0] + F[1] * g[1] + F[2] * g[2];
+    }
+#pragma unroll
+    for (int a = 0; a < DIM; ++a) o[a] = F[a] + nalpha * (G * g[a]);
+}
+
+// supercell position added to the displacement (STZ/elastic.c:893-899).  2-D as the
+// reference writes it: its slow index i along x, the fast index j along y; 3-D: x_b the
+// grid index along axis b (x = the fast axis), u_a += sum_b x_b (delta_ab + e_ab) L_b.
+template <int DIM>
+__device__ __forceinline__ double position(int a, int i0, int i1, int i2, const double *e, const double *L)
+{
+    if constexpr (DIM == 2) {
+        const double i = (double)i1, j = (double)i0;
+        return a == 0 ? i * (1 + e[0]) * L[0] + j * e[2] * L[1] : i * e[2] * L[0] + j * (1 + e[1]) * L[1];
+    } else {
+        constexpr int vo[3][3] = {{0, 5, 4}, {5, 1, 3}, {4, 3, 2}};
+        const double x[3] = {(double)i0, (double)i1, (double)i2};
+        double pos = 0.0;
+#pragma unroll
+        for (int b = 0; b < 3; ++b) pos = pos + x[b] * ((a == b ? 1.0 : 0.0) + e[vo[a][b]]) * L[b];
+        return pos;
+    }
+}
+
+template <typename R, int DIM>
+__device__ __forceinline__ void products(const R *g, R *gg)
+{
+    if constexpr (DIM == 2) {
+        gg[0] = g[0] * g[0];
+        gg[1] = g[1] * g[1];
+        gg[2] = g[0] * g[1];
+    } else {
+        gg[0] = g[0] * g[0];
+        gg[1] = g[1] * g[1];
+        gg[2] = g[2] * g[2];
+        gg[3] = g[1] * g[2];
+        gg[4] = g[0] * g[2];
+        gg[5] = g[0] * g[1];
+    }
+}
+
+// C:e for isotropic moduli (lam, mu) (IsotropicMul_RF / IsotropicMul_RF_D with the
+// per-voxel moduli or their differences, STZ/elastic.c:905-938)
+template <int C>
+__device__ __forceinline__ void iso_mul(double lam, double mu, const double *e, double *t)
+{
+    if constexpr (C == 3) {
+        double tra = lam * (e[0] + e[1]);
+        t[0] = tra + 2 * mu * e[0];
+        t[1] = tra + 2 * mu * e[1];
+        t[2] = 2 * mu * e[2];
+    } else {
+        double tra = lam * ((e[0] + e[1]) + e[2]);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) t[c] = tra + 2 * mu * e[c];
+#pragma unroll
+        for (int c = 3; c < 6; ++c) t[c] = 2 * mu * e[c];
+    }
+}
+
+// C0^-1 : t (IsotropicInv, STZ/elastic.c:940-955)
+template <int C>
+__device__ __forceinline__ void iso_inv(double lam, double mu, const double *t, double *e)
+{
+    if constexpr (C == 3) {
+        double tra = -1.0 * lam / 2.0 / mu / (2.0 * lam + 2.0 * mu) * (t[0] + t[1]);
+        e[0] = tra + t[0] / 2.0 / mu;
+        e[1] = tra + t[1] / 2.0 / mu;
+        e[2] = t[2] / 2.0 / mu;
+    } else {
+        double tra = -1.0 * lam / 2.0 / mu / (3.0 * lam + 2.0 * mu) * ((t[0] + t[1]) + t[2]);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) e[c] = tra + t[c] / 2.0 / mu;
+#pragma unroll
+        for (int c = 3; c < 6; ++c) e[c] = t[c] / 2.0 / mu;
+    }
+}
+
+// One fixed-point update of the inhomogeneous solve at a voxel (STZ/elastic.c:1027-1046):
+// from the total strain `tot` of the homogeneous solve of the eigenstrain `ein`,
+// new = C0^-1 (Tau0 + (C0 - C(x)):tot), mixed 0.5/0.5 with `ein`.
+template <int C>
+__device__ __forceinline__ void inhom_update(const double *tot, const double *ein, const double *tau0, double lx,
+                                             double mx, double lam, double mu, double *eout)
+{
+    double t1[C], tau[C], r[C];
+    iso_mul<C>(lam - lx, mu - mx, tot, t1);
+#pragma unroll
+    for (int c = 0; c < C; ++c) tau[c] = tau0[c] + t1[c];
+    iso_inv<C>(lam, mu, tau, r);
+    const double mix = 0.5;
+#pragma unroll
+    for (int c = 0; c < C; ++c) eout[c] = mix * r[c] + (1 - mix) * ein[c];
+}
+
+#pragma clang fp contract(on)
+
+// fused step: the solve runs after a thermal event or an athermal avalanche
+__device__ __forceinline__ bool gated_off_p(const int *gate)
+{
+    return gate && *gate != KIND_THERMAL && *gate != KIND_ATHERMAL;
+}
+
+// ---------------------------------------------------------------------------------
+// k_fwd_x: eps0 -> round trip -> R2C along x for NR rows and all C components
+// (MODE 1: the input is the reference-medium eigenstrain itself, no round trip)
+// ---------------------------------------------------------------------------------
+template <typename R, int L0, int NR, int C, int MODE = 0>
+__global__ __launch_bounds__(NT_SOLVE) void k_fwd_x(const double *__restrict__ eps0_a, cpx<R> *__restrict__ spec_a,
+                                                    const cpx<R> *__restrict__ twg, SolveConsts k, const SolveRep *reps)
+{
+    constexpr int NP = NR / 2;            // packed complex lines per component
+    const View<R> vw = view_of<R>(reps, blockIdx.y, k, eps0_a, spec_a, nullptr, nullptr, nullptr);
+    if (gated_off_p(vw.gate)) return;
+    const double *__restrict__ eps0 = vw.eps0;
+    cpx<R> *__restrict__ spec = vw.spec;
+    constexpr int NL = C * NP;
+    constexpr int LS = NL + 1;            // odd stride against bank conflicts
+    __shared__ cpx<R> buf[L0 * LS];
+    __shared__ cpx<R> tw[L0];
+    const int tid = threadIdx.x;
+    for (int i = tid; i < L0; i += NT_SOLVE) tw[i] = twg[i];
+    const long long row0 = (long long)blockIdx.x * NR;
+    R *bufr = reinterpret_cast<R *>(buf);
+    for (int e = tid; e < NR * L0; e += NT_SOLVE) {
+        const int rl = e / L0, i0 = e % L0;
+        const long long v = (row0 + rl) * L0 + i0;
+        double ev[C], rt[C];
+#pragma unroll
+        for (int c = 0; c < C; ++c) ev[c] = eps0[(long long)c * k.N + v];
+        if constexpr (MODE == 0) round_trip<C>(ev, vw.ea, k.lambda, k.mu, rt);
+        else
+#pragma unroll
+            for (int c = 0; c < C; ++c) rt[c] = ev[c];
+#pragma unroll
+        for (int c = 0; c < C; ++c) bufr[2 * (i0 * LS + c * NP + (rl >> 1)) + (rl & 1)] = (R)rt[c];
+    }
+    __syncthreads();
+    fft_lines<R, L0, NL, LS, NT_SOLVE>(buf, tw, (R)-1);
+    constexpr int H0 = L0 / 2 + 1;
+    for (int e = tid; e < C * NR * H0; e += NT_SOLVE) {
+        const int kk = e % H0;
+        const int rl = (e / H0) % NR;
+        const int c = e / (H0 * NR);
+        const int p = rl >> 1;
+        const cpx<R> Z = buf[kk * LS + c * NP + p];
+        const cpx<R> Zm = buf[((L0 - kk) % L0) * LS + c * NP + p];
+        cpx<R> X;
+        if ((rl & 1) == 0) X = {(Z.x + Zm.x) * (R)0.5, (Z.y - Zm.y) * (R)0.5};
+        else X = {(Z.y + Zm.y) * (R)0.5, -(Z.x - Zm.x) * (R)0.5};
+        spec[((long long)c * k.rows + row0 + rl) * k.hp + kk] = X;
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// k_inv_x: C2R along x + stress epilogue + per-block partial sums of sigma
+// (MODE 1: the input is the reference-medium eigenstrain; the epilogue also takes the
+// fixed-point update of the inhomogeneous solve into k.eout and sums its squared change)
+// ---------------------------------------------------------------------------------
+template <typename R, int L0, int NR, int C, int MODE = 0>
+__global__ __launch_bounds__(NT_SOLVE) void k_inv_x(cpx<R> *__restrict__ spec_a, const double *__restrict__ eps0_a,
+                                                    R *__restrict__ sigma_a, double *__restrict__ spart_a,
+                                                    int *__restrict__ nanflag_a, const cpx<R> *__restrict__ twg,
+                                                    SolveConsts k, const SolveRep *reps)
+{
+    constexpr int NP = NR / 2;
+    constexpr int NL = C * NP;
+    constexpr int LS = NL + 1;
+    constexpr int H = L0 / 2;
+    const View<R> vw = view_of<R>(reps, blockIdx.y, k, eps0_a, spec_a, sigma_a, spart_a, nanflag_a);
+    if (gated_off_p(vw.gate)) return;
+    const cpx<R> *__restrict__ spec = vw.spec;
+    const double *__restrict__ eps0 = vw.eps0;
+    R *__restrict__ sigma = vw.sigma;
+    double *__restrict__ spart = vw.spart;
+    int *__restrict__ nanflag = vw.nanflag;
+    __shared__ cpx<R> buf[L0 * LS];
+    __shared__ cpx<R> tw[L0];
+    __shared__ double red[NT_SOLVE];
+    const int tid = threadIdx.x;
+    for (int i = tid; i < L0; i += NT_SOLVE) tw[i] = twg[i];
+    const long long row0 = (long long)blockIdx.x * NR;
+    for (int e = tid; e < C * NP * L0; e += NT_SOLVE) {
+        const int kk = e % L0;
+        const int p = (e / L0) % NP;
+        const int c = e / (L0 * NP);
+        const cpx<R> *ra = spec + ((long long)c * k.rows + row0 + 2 * p) * k.hp;
+        const cpx<R> *rb = ra + k.hp;
+        cpx<R> Z;
+        if (kk == 0 || kk == H) {
+            Z = {ra[kk].x, rb[kk].x};
+        } else if (kk < H) {
+            const cpx<R> a = ra[kk], b = rb[kk];
+            Z = {a.x - b.y, a.y + b.x};
+        } else {
+            const cpx<R> a = ra[L0 - kk], b = rb[L0 - kk];
+            Z = {a.x + b.y, -a.y + b.x};
+        }
+        buf[kk * LS + c * NP + p] = Z;
+    }
+    __syncthreads();
+    fft_lines<R, L0, NL, LS, NT_SOLVE>(buf, tw, (R)1);
+    const R *bufr = reinterpret_cast<const R *>(buf);
+    double acc[C];
+#pragma unroll
+    for (int c = 0; c < C; ++c) acc[c] = 0.0;
+    double cacc = 0.0;
+    int bad = 0;
+    for (int e = tid; e < NR * L0; e += NT_SOLVE) {
+        const int rl = e / L0, i0 = e % L0;
+        const long long v = (row0 + rl) * L0 + i0;
+        double ev[C], rt[C], d[C], s[C], tot[C];
+#pragma unroll
+        for (int c = 0; c < C; ++c) ev[c] = eps0[(long long)c * k.N + v];
+        if constexpr (MODE == 0) round_trip<C>(ev, vw.ea, k.lambda, k.mu, rt);
+        else
+#pragma unroll
+            for (int c = 0; c < C; ++c) rt[c] = ev[c];
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+            const double z = (double)bufr[2 * (i0 * LS + c * NP + (rl >> 1)) + (rl & 1)];
+            bad |= (z != z);
+            tot[c] = z / (double)k.N + vw.ea[c];   // STZ/elastic.c:822
+            d[c] = tot[c] - rt[c];
+        }
+        stress_of<C>(d, k.lambda, k.mu, s);
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+            sigma[(long long)c * k.N + v] = (R)s[c];
+            acc[c] += s[c];
+        }
+        if constexpr (MODE == 1) {
+            double t0[C], nx[C];
+#pragma unroll
+            for (int c = 0; c < C; ++c) t0[c] = k.tau0[(long long)c * k.N + v];
+            inhom_update<C>(tot, rt, t0, k.lamx[v], k.mux[v], k.lambda, k.mu, nx);
+#pragma unroll
+            for (int c = 0; c < C; ++c) {
+                k.eout[(long long)c * k.N + v] = nx[c];
+                const double dd = nx[c] - rt[c];
+                cacc += dd * dd;
+            }
+        }
+    }
+    if (bad) atomicOr(nanflag, 1);
+    // fixed-shape tree reduction of each component
+    for (int c = 0; c < C; ++c) {
+        red[tid] = acc[c];
+        __syncthreads();
+        for (int w = NT_SOLVE / 2; w > 0; w >>= 1) {
+            if (tid < w) red[tid] = red[tid] + red[tid + w];
+            __syncthreads();
+        }
+        if (tid == 0) spart[(long long)blockIdx.x * C + c] = red[0];
+        __syncthreads();
+    }
+    if constexpr (MODE == 1) {
+        red[tid] = cacc;
+        __syncthreads();
+        for (int w = NT_SOLVE / 2; w > 0; w >>= 1) {
+            if (tid < w) red[tid] = red[tid] + red[tid + w];
+            __syncthreads();
+        }
+        if (tid == 0) k.cpart[blockIdx.x] = red[0];
+    }
+}
+
+// rows per workgroup of the x passes: the largest power of two (2..32) whose LDS tile,
+// L0 x (C*NR/2 + 1) complex values, stays within 32 KB
+template <typename R, int C>
+constexpr int rows_for(int L0)
+{
+    int nr = 32;
+    while (nr > 2 && (long long)L0 * (C * nr / 2 + 1) * (long long)sizeof(cpx<R>) > 32768) nr /= 2;
+    return nr;
+}
+template <typename R, int C>
+constexpr bool x_variant(int L0, int NR)
+{
+    return NR >= 2 && NR <= 2 * rows_for<R, C>(L0) &&
+           (long long)L0 * (C * NR / 2 + 1) * (long long)sizeof(cpx<R>) <= 65536;
+}
+
+// launch constants of a context (the batched path overrides the per-replica fields)
+inline SolveConsts solve_consts(const stz_ctx *ctx)
+{
+    SolveConsts k{};
+    k.gate = nullptr;
+    k.lambda = ctx->k.lambda;
+    k.mu = ctx->k.mu;
+    k.alpha = ctx->k.alpha;
+    k.s0 = 1.0 / ctx->k.mu;
+    k.sh = 0.5 / ctx->k.mu;
+    k.sa = -1.0 * ctx->k.alpha / ctx->k.mu;
+    k.invN = 1.0 / (double)ctx->N;
+    for (int c = 0; c < 6; ++c) k.epsavg[c] = ctx->epsavg[c];
+    for (int a = 0; a < 3; ++a) k.n[a] = ctx->n[a];
+    k.N = ctx->N;
+    k.rows = ctx->n[1] * ctx->n[2];
+    k.h0 = ctx->h0;
+    k.hp = ctx->hp;
+    k.C = ctx->C;
+    k.L[0] = ctx->p.lx;
+    k.L[1] = ctx->p.ly;
+    k.L[2] = ctx->p.lz;
+    return k;
+}
+
+}  // namespace

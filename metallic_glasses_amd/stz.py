@@ -139,6 +139,8 @@ def lib():
         L.stz_checkpoint_save.argtypes = [vp, C.c_char_p]
         L.stz_displacement.argtypes = [vp, dp]
         L.stz_checkpoint_load.argtypes = [vp, C.c_char_p]
+        L.stz_set_moduli.argtypes = [vp, dp, dp]
+        L.stz_inhom_iterations.argtypes = [vp]
         _lib = L
     return _lib
 
@@ -245,6 +247,22 @@ class Stz:
         u = np.empty(self.dim * self.N, dtype=np.float64)
         self._check(lib().stz_displacement(self._h, _dp(u)))
         return u
+
+    def set_moduli(self, lam, mu):
+        """Per-voxel Lambda(x), Mu(x) (arrays of N), or None for uniform moduli (see
+        include/stz.h)."""
+        if lam is None and mu is None:
+            self._check(lib().stz_set_moduli(self._h, None, None))
+            return
+        lam = np.ascontiguousarray(lam, dtype=np.float64).reshape(-1)
+        mu = np.ascontiguousarray(mu, dtype=np.float64).reshape(-1)
+        if lam.size != self.N or mu.size != self.N:
+            raise ValueError("moduli must have one value per voxel")
+        self._check(lib().stz_set_moduli(self._h, _dp(lam), _dp(mu)))
+
+    def inhom_iterations(self):
+        """Homogeneous solves the last stress solve took (1 with uniform moduli)."""
+        return int(lib().stz_inhom_iterations(self._h))
 
     def checkpoint_save(self, path):
         """Writes everything the next step reads (see include/stz.h)."""

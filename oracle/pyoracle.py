@@ -93,6 +93,9 @@ def lib():
         L.ora_set_print.argtypes = [C.c_void_p, C.c_int, C.c_double]
         L.ora_set_strainhist.argtypes = [C.c_void_p, C.c_double]
         L.ora_displacement.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
+        L.ora_set_moduli.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.ora_inhom_iterations.argtypes = [C.c_void_p]
+        L.ora_inhom_iterations.restype = C.c_int
         L.ora_set_threads.argtypes = [C.c_int]
         L.ora_set_threads.restype = C.c_int
         _lib = L
@@ -172,6 +175,19 @@ class Oracle:
 
     def solve(self):
         lib().ora_stress_solve(self._h)
+
+    def set_moduli(self, lam, mu):
+        """Per-voxel Lambda, Mu (None: uniform).  Keeps references to the arrays' copies."""
+        if lam is None:
+            lib().ora_set_moduli(self._h, None, None)
+            return
+        self._lam = np.ascontiguousarray(lam, dtype=np.float64)
+        self._mu = np.ascontiguousarray(mu, dtype=np.float64)
+        dp = C.POINTER(C.c_double)
+        lib().ora_set_moduli(self._h, self._lam.ctypes.data_as(dp), self._mu.ctypes.data_as(dp))
+
+    def inhom_iterations(self):
+        return lib().ora_inhom_iterations(self._h)
 
     def displacement(self):
         """Print-step re-solve and displacement field u[dim][N]."""

@@ -31,6 +31,11 @@
  *                     compatible eps0 -> zero fluctuation stress; laminate
  *                     closed form; k.sigma_hat = 0 (equilibrium); brute-force
  *                     dense Green's-operator solve on a 4x4 / 4x4x4 grid.
+ *   ora_set_moduli    Lambda(x), Mu(x) of STZ/init.c:457-465 (uniform when unset); the
+ *                     wrapper then iterates to its fixed point, STZ/elastic.c:990-1062.
+ *                     pins: explicit uniform moduli give the uniform path bit for bit;
+ *                     a laminate under applied shear converges to the closed form
+ *                     (uniform traction, harmonic-mean shear modulus).
  *   ora_displacement  print-step InhomogeneousStressSolver(uflag=2), STZ/bmg.c:289-295,
  *                     STZ/elastic.c:873-900 + helpers :754-777.
  *                     pins: the symmetric spectral gradient of u equals the solved
@@ -294,6 +299,9 @@ struct ora_state {
     /* solver scratch */
     double *spec;           /* C complex fields, N each */
     double *tau0, *epsolast;
+    /* elastic inhomogeneity: Lambda[N], Mu[N] (NULL: uniform, STZ/init.c:457-465) */
+    double *lamx, *mux, *estar;
+    int inhom_iters;
     /* scalars */
     double epsavg[6], stress[6], totime;
     ora_mt rng;
@@ -487,6 +495,7 @@ void ora_destroy(ora_state *s)
     free(s->eps0); free(s->sigma); free(s->epso); free(s->q0); free(s->deps); free(s->deps_last);
     free(s->perm); free(s->temp); free(s->tlast); free(s->q); free(s->soft); free(s->rate);
     free(s->spec); free(s->tau0); free(s->epsolast);
+    free(s->lamx); free(s->mux); free(s->estar);
     for (int a = 0; a < 3; a++) free(s->g[a]);
     for (int a = 0; a < 6; a++) free(s->gg[a]);
     free(s);
@@ -669,7 +678,7 @@ static void hom_solve(ora_state *s)
 }
 
 /* InhomogeneousStressSolver with Lambda = lambda, Mu = mu (STZ/elastic.c:977-1066) */
-void ora_stress_solve(ora_state *s)
+static void stress_solve_uniform(ora_state *s)
 {
     const long N = s->N;
     const int C = s->C;
@@ -698,9 +707,84 @@ void ora_stress_solve(ora_state *s)
         }
     }
     hom_solve(s);
+    s->inhom_iters = 1;
     /* Tau1o = 0 again -> epso = C0^-1 Tau0 == epsolast, mix 0.5 leaves it, convg = 0 -> break
        (STZ/elastic.c:1027-1058).  The stress field and <sigma> of hom_solve are final. */
 }
+
+/* The same wrapper with Lambda(x), Mu(x) set: the fixed point of the reference-medium
+   eigenstrain eps* = C0^-1 (C(x):eps0 + (C0 - C(x)):eps), eps the total strain of the
+   homogeneous solve of eps*, mixed 0.5/0.5, until the RMS change is below
+   1e-3 |epsavg| (or |epsavg| < 1e-4), at most 101 solves (STZ/elastic.c:990-1062).
+   |epsavg| and the RMS change run over all C components (the reference's three in 2-D). */
+static void stress_solve_inhom(ora_state *s)
+{
+    const long N = s->N;
+    const int C = s->C;
+    const double lambda = s->lambda, mu = s->mu;
+#pragma omp parallel for schedule(static)
+    for (long v = 0; v < N; v++) {
+        double e[6] = {0}, z[6] = {0}, t0[6], t1[6], tau[6], r[6];
+        for (int c = 0; c < C; c++) { e[c] = s->eps0[(long)c * N + v]; z[c] = e[c] + s->epsavg[c]; }  /* :993-999 */
+        iso_mul(s, s->lamx[v], s->mux[v], e, t0);                         /* IsotropicMul_RF :1001 */
+        iso_mul(s, lambda - s->lamx[v], mu - s->mux[v], z, t1);           /* IsotropicMul_RF_D :1002 */
+        for (int c = 0; c < C; c++) tau[c] = t0[c] + t1[c];
+        iso_inv(s, lambda, mu, tau, r);                                   /* IsotropicInv :1009 */
+        for (int c = 0; c < C; c++) { s->tau0[(long)c * N + v] = t0[c]; s->estar[(long)c * N + v] = r[c]; }
+    }
+    double ne = 0.0;
+    for (int c = 0; c < C; c++) ne += s->epsavg[c] * s->epsavg[c];
+    const double normepsavg = sqrt(ne);                                   /* :1012 */
+    int step = 0;
+    for (;;) {
+        step++;
+        memcpy(s->epsolast, s->estar, sizeof(double) * (size_t)C * (size_t)N);   /* :1017-1022 */
+        hom_solve(s);                                                     /* epso = total strain :1025 */
+#pragma omp parallel for schedule(static)
+        for (long v = 0; v < N; v++) {
+            double z[6] = {0}, t1[6], tau[6], r[6];
+            for (int c = 0; c < C; c++) z[c] = s->epso[(long)c * N + v];
+            iso_mul(s, lambda - s->lamx[v], mu - s->mux[v], z, t1);       /* :1027 */
+            for (int c = 0; c < C; c++) tau[c] = s->tau0[(long)c * N + v] + t1[c];   /* :1030-1035 */
+            iso_inv(s, lambda, mu, tau, r);                               /* :1038 */
+            const double mix = 0.5;
+            for (int c = 0; c < C; c++)
+                s->estar[(long)c * N + v] = mix * r[c] + (1 - mix) * s->epsolast[(long)c * N + v];   /* :1040-1046 */
+        }
+        double avg = 0.0;                                                 /* AvgChange :957-975 */
+        for (int c = 0; c < C; c++)
+            for (long v = 0; v < N; v++) {
+                const double d = s->estar[(long)c * N + v] - s->epsolast[(long)c * N + v];
+                avg += d * d;
+            }
+        avg /= (double)N;
+        const double convg = sqrt(avg);
+        if ((convg < (1E-3) * normepsavg) || normepsavg < 1E-4) break;   /* :1049 */
+        if (step > 100) break;                                            /* :1057 */
+    }
+    s->inhom_iters = step;
+}
+
+void ora_stress_solve(ora_state *s)
+{
+    if (s->lamx) stress_solve_inhom(s);
+    else stress_solve_uniform(s);
+}
+
+void ora_set_moduli(ora_state *s, const double *lam, const double *mu)
+{
+    free(s->lamx); free(s->mux); free(s->estar);
+    s->lamx = s->mux = s->estar = NULL;
+    if (!lam || !mu) return;
+    const size_t N = (size_t)s->N;
+    s->lamx = (double *)malloc(N * sizeof(double));
+    s->mux = (double *)malloc(N * sizeof(double));
+    s->estar = (double *)malloc((size_t)s->C * N * sizeof(double));
+    memcpy(s->lamx, lam, N * sizeof(double));
+    memcpy(s->mux, mu, N * sizeof(double));
+}
+
+int ora_inhom_iterations(ora_state *s) { return s->inhom_iters; }
 
 /* Print-step displacement: InhomogeneousStressSolver(..., uflag=2) (STZ/bmg.c:289-295)
    re-solves the stress and then runs HomogeneousStressSolver's displacement branch on the

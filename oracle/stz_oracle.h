@@ -99,6 +99,8 @@ void    ora_counters(ora_state *s, long long *out8);
 /* --- hot-path pieces (each follows one reference function) ------------ */
 void   ora_stress_solve(ora_state *s);               /* InhomogeneousStressSolver(uflag=0) */
 void   ora_displacement(ora_state *s, double *u);    /* InhomogeneousStressSolver(uflag=2): re-solve + u[D][N] */
+void   ora_set_moduli(ora_state *s, const double *lam, const double *mu); /* Lambda(x), Mu(x); NULL: uniform */
+int    ora_inhom_iterations(ora_state *s);           /* homogeneous solves of the last stress solve */
 void   ora_softening(ora_state *s);                  /* SofteningFactor */
 long long ora_act_energy(ora_state *s);              /* ActEnergy + Q<0 count */
 double ora_total_rate(ora_state *s);                 /* KMC rate loop + total */
