@@ -43,6 +43,7 @@ def parse():
     ap.add_argument("--resync", type=int, default=1024, help="local updates between full FFT re-solves")
     ap.add_argument("--local-max", type=int, default=16, help="largest avalanche updated locally")
     ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--no-inhom", action="store_true")
     ap.add_argument("--ensemble-replicas", type=int, default=64)
     ap.add_argument("--ensemble-steps", type=int, default=200)
     ap.add_argument("--ensemble-warmup", type=int, default=200)
@@ -432,6 +433,10 @@ def main():
                         "fft_roofline": fft_table(dims, C, N, 4, smsm, snm)}
         print(f"# mixed: {out['mixed']}", file=sys.stderr, flush=True)
 
+    if not a.no_inhom:
+        out["inhom"] = inhom_leg(a, mg, kw, snap, dims)
+        print(f"# inhom: {out['inhom']}", file=sys.stderr, flush=True)
+
     events = ref["events"]
     if not a.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -478,6 +483,56 @@ def main():
         print(f"# ensemble: {out['ensemble']}", file=sys.stderr, flush=True)
 
     print(json.dumps(out))
+
+
+def inhom_leg(a, mg, kw, snap, dims, nsolve=10, nsteps=40):
+    """Per-voxel moduli (the reference's InhomogeneousStressSolver with non-uniform
+    Lambda, Mu): Lambda, Mu = lambda, mu * (1 + 0.25 U(-1, 1)) per voxel (numpy seed 7), on
+    the snapshot state.  Times nsolve stress solves (the fixed-point loop, host decision
+    per iteration included) and nsteps host-path steps, per precision; reports the
+    iterations, the achieved bandwidth of one iteration against its algorithmic bytes, and
+    the mixed solve's stress error against the FP64 solve of the same state."""
+    out = {"moduli": "lambda, mu x (1 + 0.25 U(-1,1)) per voxel, seed 7"}
+    sig64 = None
+    for name, prec, b in (("fp64", mg.FP64, 8), ("mixed", mg.MIXED, 4)):
+        if prec == mg.MIXED and a.no_mixed:
+            continue
+        g = mg.Stz(init_state=False, **dict(kw, precision=prec))
+        restore(g, snap, mg)
+        g.autotune(reps=10)
+        rng = np.random.default_rng(7)
+        lam0, mu0 = g.lame()
+        lam = lam0 * (1 + 0.25 * rng.uniform(-1, 1, g.N))
+        mu = mu0 * (1 + 0.25 * rng.uniform(-1, 1, g.N))
+        g.set_moduli(lam, mu)
+        g.stress_solve()
+        t0 = time.perf_counter()
+        for _ in range(nsolve):
+            g.stress_solve()
+        dt = (time.perf_counter() - t0) / nsolve
+        iters = g.inhom_iterations()
+        C, N = g.C, g.N
+        h0 = dims[0] // 2 + 1
+        spec = C * dims[1] * dims[2] * h0 * 2 * b
+        per_iter = (C * N * 8 + spec) + 6 * spec + (spec + C * N * 8 + C * N * b) + (2 * C * 8 + 16) * N
+        init = (C + 2) * 8 * N + 2 * C * 8 * N
+        gbs = (init + iters * per_iter) / dt / 1e9
+        sig = g.download(mg.F_SIGMA)
+        leg = {"solve_ms": round(dt * 1e3, 4), "iterations": iters, "ms_per_iteration": round(dt * 1e3 / iters, 4),
+               "bytes_per_iteration": int(per_iter), "GB/s": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
+        if prec == mg.FP64:
+            sig64 = sig
+        elif sig64 is not None:
+            leg["stress_rel_err_vs_fp64"] = float(np.linalg.norm(sig - sig64) / np.linalg.norm(sig64))
+        t0 = time.perf_counter()
+        rc, res = g.run_steps(nsteps)
+        dts = time.perf_counter() - t0
+        kinds = np.bincount([r.kind for r in res], minlength=3)
+        leg.update({"steps": len(res), "ms_per_step": round(dts * 1e3 / max(len(res), 1), 3),
+                    "steps_by_kind": kinds.tolist(), "events_per_s": round(float(kinds[1] / dts), 3)})
+        out[name] = leg
+        g.close()
+    return out
 
 
 ENSEMBLE_GRIDS = [(32, 32, 32), (64, 64, 128), (96, 96, 96), (128, 128, 128)]

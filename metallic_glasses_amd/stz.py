@@ -260,6 +260,15 @@ class Stz:
             raise ValueError("moduli must have one value per voxel")
         self._check(lib().stz_set_moduli(self._h, _dp(lam), _dp(mu)))
 
+    def lame(self):
+        """(lambda, mu) of the reference medium, derived from the parameters as the library
+        derives them (STZ/init.c:26-44: nu doubled with double_nu, plane strain in 2-D)."""
+        p = self.params
+        nu = p.nu * 2 if p.double_nu else p.nu
+        mu = p.E / (2 + 2 * nu)
+        lam = p.E * nu / (1 - nu * nu) if p.dim == 2 else p.E * nu / ((1 + nu) * (1 - 2 * nu))
+        return lam, mu
+
     def inhom_iterations(self):
         """Homogeneous solves the last stress solve took (1 with uniform moduli)."""
         return int(lib().stz_inhom_iterations(self._h))

@@ -251,3 +251,11 @@ def test_inhom_checkpoint_restart(tmp_path):
     b = [h.step()[1] for _ in range(60)]
     assert [(r.kind, r.site, r.mode) for r in a] == [(r.kind, r.site, r.mode) for r in b]
     assert np.array_equal(g.download(mg.F_SIGMA), h.download(mg.F_SIGMA))
+
+
+def test_lame_mirrors_the_library_constants():
+    """Stz.lame() gives the reference medium's constants as the oracle derives them."""
+    for kw in (_kw((16, 8)), _kw((8, 8, 8)), _kw((16, 8), double_nu=0)):
+        g = mg.Stz(init_state=False, **kw)
+        o = po.Oracle(empty=True, **kw)
+        assert g.lame() == (o.consts[3], o.consts[2])
