@@ -52,6 +52,7 @@ extern "C" {
 #define STZ_F_TEMP 6
 #define STZ_F_TLAST 7
 #define STZ_F_MINQ 8       /* download only: per-voxel min over modes of Q from the last stz_rates */
+#define STZ_F_TLOCAL 9     /* local temperature Tlocal (STZ/init.c:470-476), FP64 */
 
 /* step kinds */
 #define STZ_STEP_ELASTIC 0
@@ -149,6 +150,18 @@ int stz_stress_solve(stz_ctx *ctx);
    path (stz_step; stz_run_steps falls back to it); the event-local update and ensembles
    return STZ_E_UNSUPPORTED.  Device memory: (2 + 3C) doubles per voxel. */
 int stz_set_moduli(stz_ctx *ctx, const double *lambda, const double *mu);
+
+/* HEAT_CONDUCT (STZ/bmgFlags.h:84; off in the reference's build): with on = 1, every
+   thermal step (after its transform) and every elastic step diffuses the local
+   temperature STZ_F_TLOCAL by HeatConducting(thermal_diffusivity * dt)
+   (STZ/evolution.c:247-249, 325-327; STZ/elastic.c:549-582).  Tlocal is heated by every
+   transform whether or not the flag is on (STZ/elastic.c:381-383) and feeds nothing else.
+   With heat conduction steps run on the host path (stz_run_steps falls back to it) and
+   ensembles return STZ_E_UNSUPPORTED.  No counterpart as a call in the reference. */
+int stz_set_heat_conduct(stz_ctx *ctx, int on);
+/* One HeatConducting(dt) of STZ_F_TLOCAL: spectrum times exp(-|k| dt), |k| of the
+   reference's k grid and 1 at k = 0.  STZ_E_NAN on a NaN (the reference aborts with -4). */
+int stz_heat_conduct(stz_ctx *ctx, double dt);
 
 /* Homogeneous solves the last stress solve took (1 with uniform moduli).  No counterpart
    in the reference (its loop counter `step`, STZ/elastic.c:1014-1015). */

@@ -138,7 +138,7 @@ size_t field_count(const stz_ctx *c, int f)
     case STZ_F_Q0: return (size_t)c->M * N;
     case STZ_F_DEPS: return (size_t)c->M * c->S * N;
     case STZ_F_DEPS_LAST: return (size_t)c->S * N;
-    case STZ_F_PERM: case STZ_F_TEMP: case STZ_F_TLAST: case STZ_F_MINQ: return N;
+    case STZ_F_PERM: case STZ_F_TEMP: case STZ_F_TLAST: case STZ_F_MINQ: case STZ_F_TLOCAL: return N;
     default: return 0;
     }
 }
@@ -242,6 +242,24 @@ int solve(stz_ctx *ctx)
     ctx->nsolve++;
     stz_local_mark_solved(ctx);
     acc_solve(ctx);
+    return STZ_OK;
+}
+
+// HeatConducting(hdt) (STZ/elastic.c:549-582) of the local temperature; a NaN is reported
+// as STZ_E_NAN, where the reference aborts with -4
+int heat_step(stz_ctx *ctx, double hdt)
+{
+    STZ_HIP(hipMemsetAsync(ctx->res + 15, 0, sizeof(double), ctx->stream));
+    int rc = stz_heat_launch(ctx, hdt);
+    if (rc) return rc;
+    STZ_HIP(hipMemcpyAsync(ctx->hres + 15, ctx->res + 15, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    STZ_HIP(hipStreamSynchronize(ctx->stream));
+    int nan;
+    std::memcpy(&nan, ctx->hres + 15, sizeof(int));
+    if (nan) {
+        ctx->err = "NaN in the heat conduction";
+        return STZ_E_NAN;
+    }
     return STZ_OK;
 }
 
@@ -468,6 +486,7 @@ int stz_create(const stz_params *p, stz_ctx **out)
     rc = rc ? rc : dalloc(ctx, (void **)&ctx->perm, N * 8);
     rc = rc ? rc : dalloc(ctx, (void **)&ctx->temp, N * 8);
     rc = rc ? rc : dalloc(ctx, (void **)&ctx->tlast, N * 8);
+    rc = rc ? rc : dalloc(ctx, (void **)&ctx->tloc, N * 8);
     rc = rc ? rc : dalloc(ctx, &ctx->spec, (size_t)ctx->C * rows * ctx->hp * 2 * R);
     rc = rc ? rc : dalloc(ctx, &ctx->minq, N * R);
     rc = rc ? rc : dalloc(ctx, &ctx->q0eff, ctx->M * N * R);
@@ -501,6 +520,9 @@ int stz_create(const stz_params *p, stz_ctx **out)
     // tlast = -100*tTemp (STZ/init.c:312)
     hipLaunchKernelGGL((k_fill<double>), dim3((unsigned)((N + 255) / 256)), dim3(256), 0, ctx->stream, ctx->tlast,
                        (long long)N, -100 * ctx->k.tTemp);
+    // Tlocal = T__K (STZ/init.c:470-476)
+    hipLaunchKernelGGL((k_fill<double>), dim3((unsigned)((N + 255) / 256)), dim3(256), 0, ctx->stream, ctx->tloc,
+                       (long long)N, p->temperature);
     if (hipGetLastError() != hipSuccess || hipStreamSynchronize(ctx->stream) != hipSuccess) return fail(STZ_E_HIP);
     *out = ctx;
     return STZ_OK;
@@ -513,7 +535,7 @@ void stz_destroy(stz_ctx *ctx)
     stz_resident_free(ctx);
     stz_local_release(ctx);
     stz_inhom_free(ctx);
-    void *ptrs[] = {ctx->q0eff, ctx->scache, ctx->eps0, ctx->sigma, ctx->q0, ctx->deps, ctx->deps_last, ctx->perm, ctx->temp, ctx->tlast,
+    void *ptrs[] = {ctx->q0eff, ctx->scache, ctx->eps0, ctx->sigma, ctx->q0, ctx->deps, ctx->deps_last, ctx->perm, ctx->temp, ctx->tlast, ctx->tloc,
                     ctx->spec, ctx->minq, ctx->argm, ctx->bsum, ctx->gsum, ctx->res, ctx->spart,
                     ctx->acnt, ctx->aoff, ctx->alist, ctx->dnew, ctx->socp, ctx->gneg};
     for (void *q : ptrs)
@@ -595,11 +617,12 @@ int stz_upload(stz_ctx *ctx, int field, const double *host)
     case STZ_F_PERM: STZ_HIP(hipMemcpyAsync(ctx->perm, host, n * 8, hipMemcpyHostToDevice, ctx->stream)); break;
     case STZ_F_TEMP: STZ_HIP(hipMemcpyAsync(ctx->temp, host, n * 8, hipMemcpyHostToDevice, ctx->stream)); break;
     case STZ_F_TLAST: STZ_HIP(hipMemcpyAsync(ctx->tlast, host, n * 8, hipMemcpyHostToDevice, ctx->stream)); break;
+    case STZ_F_TLOCAL: STZ_HIP(hipMemcpyAsync(ctx->tloc, host, n * 8, hipMemcpyHostToDevice, ctx->stream)); break;
     default: return STZ_E_ARG;
     }
     if (rc) return rc;
     if (field == STZ_F_EPS0 || field == STZ_F_SIGMA) ctx->nlocal = 1ll << 60;   // full solve before a local update
-    if (field != STZ_F_EPS0 && field != STZ_F_SIGMA) {
+    if (field != STZ_F_EPS0 && field != STZ_F_SIGMA && field != STZ_F_TLOCAL) {
         rc = invalidate_cache(ctx);
         if (rc) return rc;
     }
@@ -634,6 +657,7 @@ int stz_download(stz_ctx *ctx, int field, double *host)
     case STZ_F_PERM: STZ_HIP(hipMemcpyAsync(host, ctx->perm, n * 8, hipMemcpyDeviceToHost, ctx->stream)); break;
     case STZ_F_TEMP: STZ_HIP(hipMemcpyAsync(host, ctx->temp, n * 8, hipMemcpyDeviceToHost, ctx->stream)); break;
     case STZ_F_TLAST: STZ_HIP(hipMemcpyAsync(host, ctx->tlast, n * 8, hipMemcpyDeviceToHost, ctx->stream)); break;
+    case STZ_F_TLOCAL: STZ_HIP(hipMemcpyAsync(host, ctx->tloc, n * 8, hipMemcpyDeviceToHost, ctx->stream)); break;
     default: return STZ_E_ARG;
     }
     STZ_HIP(hipStreamSynchronize(ctx->stream));
@@ -744,6 +768,20 @@ int stz_displacement(stz_ctx *ctx, double *host_u)
     (void)hipFree(u);
     if (rc == STZ_E_HIP && ctx->err.empty()) ctx->err = "displacement: HIP error";
     return rc;
+}
+
+int stz_set_heat_conduct(stz_ctx *ctx, int on)
+{
+    if (!ctx) return STZ_E_ARG;
+    ctx->heat = on != 0;
+    ctx->tune_epoch++;
+    return STZ_OK;
+}
+
+int stz_heat_conduct(stz_ctx *ctx, double dt)
+{
+    if (!ctx || !(dt >= 0)) return STZ_E_ARG;
+    return heat_step(ctx, dt);
 }
 
 int stz_set_solver(stz_ctx *ctx, int mode, long long resync, int local_max)
@@ -933,6 +971,10 @@ int stz_step_kmc(stz_ctx *ctx, stz_step_result *res)
         rc = load_thermal(ctx, t);
         if (rc) return rc;
         ctx->totime += t;                         // STZ/evolution.c:317
+        if (ctx->heat) {
+            rc = heat_step(ctx, STZ_THERMAL_DIFFUSIVITY * t);   // STZ/evolution.c:247-249
+            if (rc) return rc;
+        }
     } else if (kind == KIND_ATHERMAL || kind == KIND_ATH_OVERFLOW) {
         // athermal_plasticity (STZ/evolution.c:47-100)
         ctx->astep++;
@@ -979,6 +1021,10 @@ int stz_step_kmc(stz_ctx *ctx, stz_step_result *res)
         res->dt_kmc = 1 / R;
         ctx->estep++;
         ctx->totime += ctx->k.dt;                 // STZ/evolution.c:324
+        if (ctx->heat) {
+            rc = heat_step(ctx, STZ_THERMAL_DIFFUSIVITY * ctx->k.dt);   // STZ/evolution.c:325-327
+            if (rc) return rc;
+        }
         rc = load_elastic(ctx);
         if (rc) return rc;
         res->kind = STZ_STEP_ELASTIC;

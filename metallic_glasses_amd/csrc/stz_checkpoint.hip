@@ -6,7 +6,8 @@
 // holds everything the next step reads: the generator (624 words + position), every
 // per-voxel array in its storage precision (bit for bit), EpsAvg, <sigma>, the clock,
 // the pending uniform stress increment, the print/termination state, the counters and
-// the event-local update's bookkeeping, and per-voxel moduli when they are set (the
+// the event-local update's bookkeeping, the local temperature and the heat-conduction
+// switch, and per-voxel moduli when they are set (the
 // reference keeps them fixed for a run, STZ/init.c:457-465).  Derived arrays (the Q0*exp(-D*S) cache, min Q,
 // block sums) are rebuilt by the next rate pass.  A run restarted from a checkpoint
 // continues exactly as the saved run does.
@@ -32,7 +33,7 @@ struct CkptScalars {
     double epsavg[6], stress[6], soff[6], ebase[6], usum[6];
     double totime, pend_delta, strainhist_last, soc_elast, endstrain;
     int64_t step, astep, kstep, estep, ntrans, nsolve, avlan, nlocal, rng_count;
-    int32_t pend_comp, printperiod, rng_pos, pad;
+    int32_t pend_comp, printperiod, rng_pos, heat;
     uint32_t rng_state[624];
 };
 
@@ -55,6 +56,7 @@ std::vector<Field> fields_of(stz_ctx *c)
         {c->perm, N * 8},
         {c->temp, N * 8},
         {c->tlast, N * 8},
+        {c->tloc, N * 8},
     };
     if (c->lamx) {
         f.push_back({c->lamx, N * 8});
@@ -128,6 +130,7 @@ int stz_checkpoint_save(stz_ctx *ctx, const char *path)
     s.pend_comp = ctx->pend_comp;
     s.printperiod = ctx->printperiod;
     s.rng_pos = ctx->rng.pos;
+    s.heat = ctx->heat;
     std::memcpy(s.rng_state, ctx->rng.state, sizeof(s.rng_state));
     if (std::fwrite(&h, sizeof(h), 1, f) != 1 || std::fwrite(&s, sizeof(s), 1, f) != 1)
         return fail(ctx, f, "checkpoint write failed", STZ_E_ARG);
@@ -198,6 +201,7 @@ int stz_checkpoint_load(stz_ctx *ctx, const char *path)
     std::memcpy(ctx->rng.state, s.rng_state, sizeof(s.rng_state));
     ctx->rng.pos = s.rng_pos;
     ctx->rng.count = s.rng_count;
+    ctx->heat = s.heat != 0;
     stz_rng_replaced(ctx);                  // resident lookup windows are rebuilt from here
     return stz_invalidate_derived(ctx);
 }

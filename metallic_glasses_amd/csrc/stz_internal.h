@@ -58,6 +58,10 @@
 #define STZ_EPSILON 1e-17
 #define STZ_BOLZ 1.380658e-23
 #define STZ_PI_REF 3.14159265359
+// heating of a transform and heat conduction, STZ/bmg.h:185-188: thermal_coeff1 = 1/Cp/rho
+// (Cp = 420 J/kg-K, rho = 6125 kg/m^3), thermal_diffusivity = 3e-6 m^2/s
+#define STZ_THERMAL_COEFF1 (1 / 420.0 / 6125.0)
+#define STZ_THERMAL_DIFFUSIVITY 3E-6
 
 template <typename R>
 struct alignas(2 * sizeof(R)) cpx {
@@ -117,6 +121,7 @@ struct RepDev {
     double *eps0;
     void *sigma, *q0, *deps, *deps_last, *q0eff, *scache, *minq, *spec;
     double *perm, *temp, *tlast;
+    double *tloc;
     uint8_t *argm;
     double *bsum, *gsum, *socp, *spart, *res;
     int *acnt, *aoff, *alist;
@@ -167,6 +172,8 @@ struct stz_ctx {
     void *deps = nullptr;         // [M][S][N] R
     void *deps_last = nullptr;    // [S][N] R
     double *perm = nullptr, *temp = nullptr, *tlast = nullptr;
+    double *tloc = nullptr;       // [N] local temperature (Tlocal), FP64
+    int heat = 0;                 // HEAT_CONDUCT: diffuse tloc after thermal and elastic steps
     void *spec = nullptr;         // [C][n1*n2][hp] cpx<R>
     void *minq = nullptr;         // [N] R
     void *q0eff = nullptr;        // [M][N] R, cached Q0*exp(-D*S)
@@ -479,6 +486,8 @@ int stz_solve_spectral(stz_ctx *ctx);   // y passes + fused Gamma-hat pass of ct
 int stz_inhom_fwd_x(stz_ctx *ctx, const double *src);   // R2C x pass of an eigenstrain [C][N], no round trip
 int stz_inhom_run(stz_ctx *ctx);        // the fixed-point loop; <sigma> in res[0..C), sigma written
 int stz_inhom_free(stz_ctx *ctx);
+// heat conduction (stz_heat.hip): one HeatConducting(hdt) of ctx->tloc, NaN flag in res[15]
+int stz_heat_launch(stz_ctx *ctx, double hdt);
 int stz_rates_launch(stz_ctx *ctx, int decide); // enqueues rates + group sums (+ step-kind decision)
 int stz_select_launch(stz_ctx *ctx, double target, int gated);
 int stz_apply_launch(stz_ctx *ctx, int site, int mode, int gated);

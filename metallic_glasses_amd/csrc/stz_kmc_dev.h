@@ -346,6 +346,9 @@ struct ApplyArgs {
     void *deps, *deps_last;
     void *scache;
     double *perm, *temp, *tlast;
+    double *tloc;                 // local temperature, heated by each transform
+    const void *sigma;            // stress field (storage precision) + soff: the current stress
+    double soff[6];
     long long N;
     int M;
     double permsoft, tempsoft, permcap, totime;
@@ -370,6 +373,15 @@ __device__ void transform_voxel_state(const ApplyArgs &a, long long v, int mm)
     for (int c = 0; c < C; ++c) dd[c] = (double)d[c];
 #pragma unroll
     for (int c = 0; c < C; ++c) a.eps0[(long long)c * a.N + v] += dd[c];             // :377-379
+    {
+        // local heating (STZ/elastic.c:381-383): |sigma:deps| * 1e9 J/m^3 * (1/Cp/rho)
+        const R *sg = (const R *)a.sigma;
+        double sv[C];
+#pragma unroll
+        for (int c = 0; c < C; ++c) sv[c] = (double)sg[(long long)c * a.N + v] + a.soff[c];
+        const double dh = contract<double, DIM>(sv, dd) * 1E9;
+        a.tloc[v] += fabs(dh) * STZ_THERMAL_COEFF1;
+    }
     const double ds = contract<double, DIM>(dd, dd);                                   // :385-386
     double pv = a.perm[v] + ds * a.permsoft;                                          // :390
     if (pv > a.permcap) pv = a.permcap;                                               // :391

@@ -287,6 +287,11 @@ ApplyArgs apply_args(stz_ctx *ctx)
     a.perm = ctx->perm;
     a.temp = ctx->temp;
     a.tlast = ctx->tlast;
+    a.tloc = ctx->tloc;
+    a.sigma = ctx->sigma;
+    for (int c = 0; c < 6; ++c) a.soff[c] = ctx->soff[c];
+    // FP64: a uniform increment not yet consumed by a rate pass is not in the field
+    if (ctx->prec == STZ_FP64 && ctx->pend_comp >= 0) a.soff[ctx->pend_comp] += ctx->pend_delta;
     a.N = ctx->N;
     a.M = ctx->M;
     a.permsoft = ctx->k.permsoft;

@@ -85,6 +85,9 @@ __device__ __forceinline__ ApplyArgs apply_args_of(const RepDev &d, const RepSta
     a.perm = d.perm;
     a.temp = d.temp;
     a.tlast = d.tlast;
+    a.tloc = d.tloc;
+    a.sigma = d.sigma;
+    for (int c = 0; c < 6; ++c) a.soff[c] = st->soff[c];
     a.N = N;
     a.M = M;
     a.permsoft = d.permsoft;
@@ -686,6 +689,7 @@ void fill_desc(stz_ctx *ctx)
     d.perm = ctx->perm;
     d.temp = ctx->temp;
     d.tlast = ctx->tlast;
+    d.tloc = ctx->tloc;
     d.argm = ctx->argm;
     d.bsum = ctx->bsum;
     d.gsum = ctx->gsum;
@@ -1309,9 +1313,9 @@ int stz_run_steps(stz_ctx *ctx, int nsteps, stz_step_result *results, int *done)
 {
     if (!ctx || nsteps < 0) return STZ_E_ARG;
     int d = 0, rc = 0;
-    if (ctx->lamx) {
-        // per-voxel moduli: the fixed-point loop ends on a host decision each iteration,
-        // so every step runs on the host path (same random numbers, same results)
+    if (ctx->lamx || ctx->heat) {
+        // per-voxel moduli (the fixed-point loop ends on a host decision each iteration) or
+        // heat conduction: every step runs on the host path (same random numbers, same results)
         const bool stop_at_print = ctx->rs && ctx->rs->stop_at_print;
         for (int i = 0; i < nsteps; ++i) {
             stz_step_result r;
@@ -1359,8 +1363,8 @@ int stz_ensemble_create(stz_ctx *const *members, int n, stz_ensemble **out)
     for (int i = 0; i < n; ++i)
         if (!members[i]) return STZ_E_ARG;
     for (int i = 0; i < n; ++i)
-        if (members[i]->lamx) {
-            members[i]->err = "an ensemble steps uniform-moduli replicas only";
+        if (members[i]->lamx || members[i]->heat) {
+            members[i]->err = "an ensemble steps uniform-moduli replicas without heat conduction only";
             return STZ_E_UNSUPPORTED;
         }
     *out = nullptr;
@@ -1400,8 +1404,8 @@ int stz_ensemble_run(stz_ensemble *e, int nsteps, stz_step_result *const *result
 {
     if (!e || nsteps < 0 || !done || !rcs) return STZ_E_ARG;
     for (stz_ctx *c : e->pl->all)
-        if (c->lamx) {
-            c->err = "an ensemble steps uniform-moduli replicas only";
+        if (c->lamx || c->heat) {
+            c->err = "an ensemble steps uniform-moduli replicas without heat conduction only";
             return STZ_E_UNSUPPORTED;
         }
     return stz_plan_run(e->pl, nsteps, results, done, rcs, 0);

@@ -30,169 +30,6 @@
 namespace {
 
 // ---------------------------------------------------------------------------------
-// k_ypass: complex FFT along y (stride hp) for a tile of W k0-columns, in place
-// ---------------------------------------------------------------------------------
-template <typename R, int L1, int W>
-__global__ __launch_bounds__(NT_SOLVE) void k_ypass(cpx<R> *__restrict__ spec_a, const cpx<R> *__restrict__ twg,
-                                                    SolveConsts k, R sgn, const SolveRep *reps)
-{
-    const View<R> vw = view_of<R>(reps, blockIdx.z / k.C, k, nullptr, spec_a, nullptr, nullptr, nullptr);
-    if (gated_off_p(vw.gate)) return;
-    cpx<R> *__restrict__ spec = vw.spec;
-    __shared__ cpx<R> buf[L1 * W];
-    __shared__ cpx<R> tw[L1];
-    const int tid = threadIdx.x;
-    for (int i = tid; i < L1; i += NT_SOLVE) tw[i] = twg[i];
-    const int k0base = blockIdx.x * W;
-    const int i2 = blockIdx.y;
-    const int c = blockIdx.z % k.C;
-    cpx<R> *base = spec + ((long long)c * k.rows + (long long)i2 * L1) * k.hp + k0base;
-    for (int e = tid; e < L1 * W; e += NT_SOLVE) {
-        const int col = e % W, p = e / W;
-        cpx<R> val = {0, 0};
-        if (k0base + col < k.h0) val = base[(long long)p * k.hp + col];
-        buf[p * W + col] = val;
-    }
-    __syncthreads();
-    fft_lines<R, L1, W, W, NT_SOLVE>(buf, tw, sgn);
-    for (int e = tid; e < L1 * W; e += NT_SOLVE) {
-        const int col = e % W, p = e / W;
-        if (k0base + col < k.h0) base[(long long)p * k.hp + col] = buf[p * W + col];
-    }
-}
-
-// ---------------------------------------------------------------------------------
-// k_fused: last axis forward FFT, Gamma-hat, inverse FFT for one tile of W k0-columns
-// (3-D: at fixed y index blockIdx.y; 2-D: the y axis itself is the last axis).  The
-// tile of all C components (LZ x C*W complex values) stays in LDS from load to store:
-// the C*W lines are transformed together, Gamma-hat mixes the C components of each of
-// the LZ*W wavevectors in place, and the inverse transform follows.
-// ---------------------------------------------------------------------------------
-template <typename R, int LZ, int W, int C, int DIM, int NTF, int OP = 0>
-__global__ __launch_bounds__(NTF) void k_fused(cpx<R> *__restrict__ spec_a, const cpx<R> *__restrict__ twg,
-                                               SolveConsts k, const SolveRep *reps)
-{
-    constexpr int NL = C * W;             // lines, component-major: line = c*W + col
-    constexpr int NE = LZ * NL;
-    const View<R> vw = view_of<R>(reps, blockIdx.z, k, nullptr, spec_a, nullptr, nullptr, nullptr);
-    if (gated_off_p(vw.gate)) return;
-    cpx<R> *__restrict__ spec = vw.spec;
-    __shared__ cpx<R> buf[NE];
-    __shared__ cpx<R> tw[LZ];
-    const int tid = threadIdx.x;
-    for (int i = tid; i < LZ; i += NTF) tw[i] = twg[i];
-    const int k0base = blockIdx.x * W;
-    const int i1 = (DIM == 3) ? blockIdx.y : 0;
-    // element (p, col) of component c lives at spec[(c*rows + p*pstride + i1) * hp + k0]
-    const long long pstride = (DIM == 3) ? k.n[1] : 1;
-    for (int e = tid; e < NE; e += NTF) {
-        const int line = e % NL, p = e / NL;
-        const int c = line / W, col = line % W;
-        cpx<R> val = {0, 0};
-        if (k0base + col < k.h0) val = spec[((long long)c * k.rows + (long long)p * pstride + i1) * k.hp + k0base + col];
-        buf[e] = val;
-    }
-    __syncthreads();
-    fft_lines<R, LZ, NL, NL, NTF>(buf, tw, (R)-1);
-    // Gamma-hat (OP 0) or the displacement operator (OP 1) on each wavevector of the tile
-    const R lam = (R)k.lambda, mu = (R)k.mu, s0 = (R)k.s0, sh = (R)k.sh, sa = (R)k.sa;
-    if constexpr (OP == 1) {
-        // u_hat_a = (tmp_a(im)*(1/mu)/|k|, -1.0*tmp_a(re)*(1/mu)/|k|) (HadamardSiDivd,
-        // STZ/elastic.c:754-763) into component slots 0..DIM-1; on Nyquist bins the
-        // average with the mirror unit vector, as Re() of the full transform applies
-        const R nalpha = (R)(-1.0 * k.alpha);
-        for (int kp = tid; kp < LZ * W; kp += NTF) {
-            const int col = kp % W, p = kp / W;
-            int idx[3];
-            idx[0] = k0base + col;
-            if constexpr (DIM == 3) { idx[1] = i1; idx[2] = p; }
-            else { idx[1] = p; idx[2] = 0; }
-            double gd[3];
-            const R kmag = (R)unit_k<DIM>(idx, k.n, gd);
-            bool nyq = false;
-#pragma unroll
-            for (int a = 0; a < DIM; ++a) nyq |= (2 * idx[a] == k.n[a]);
-            const R g[3] = {(R)gd[0], (R)gd[1], (R)gd[2]};
-            R er[C], ei[C], pr[3], pi[3];
-#pragma unroll
-            for (int c = 0; c < C; ++c) {
-                const cpx<R> x = buf[p * NL + c * W + col];
-                er[c] = x.x;
-                ei[c] = x.y;
-            }
-            disp_part<R, C, DIM>(er, g, lam, mu, nalpha, pr);
-            disp_part<R, C, DIM>(ei, g, lam, mu, nalpha, pi);
-            if (nyq) {
-                R gm[3] = {g[0], g[1], g[2]};
-#pragma unroll
-                for (int a = 0; a < DIM; ++a)
-                    if (2 * idx[a] == k.n[a]) gm[a] = -gm[a];
-                R pr2[3], pi2[3];
-                disp_part<R, C, DIM>(er, gm, lam, mu, nalpha, pr2);
-                disp_part<R, C, DIM>(ei, gm, lam, mu, nalpha, pi2);
-#pragma unroll
-                for (int a = 0; a < DIM; ++a) {
-                    pr[a] = (pr[a] + pr2[a]) * (R)0.5;
-                    pi[a] = (pi[a] + pi2[a]) * (R)0.5;
-                }
-            }
-#pragma unroll
-            for (int a = 0; a < DIM; ++a) buf[p * NL + a * W + col] = {pi[a] * s0 / kmag, (R)-1.0 * pr[a] * s0 / kmag};
-        }
-    } else
-    for (int kp = tid; kp < LZ * W; kp += NTF) {
-        const int col = kp % W, p = kp / W;
-        int idx[3];
-        idx[0] = k0base + col;
-        if constexpr (DIM == 3) { idx[1] = i1; idx[2] = p; }
-        else { idx[1] = p; idx[2] = 0; }
-        double gd[3];
-        unit_k<DIM>(idx, k.n, gd);
-        bool nyq = false;
-#pragma unroll
-        for (int a = 0; a < DIM; ++a) nyq |= (2 * idx[a] == k.n[a]);
-        R g[3] = {(R)gd[0], (R)gd[1], (R)gd[2]};
-        R gg[6];
-        products<R, DIM>(g, gg);
-        R er[C], ei[C], orr[C], oi[C];
-#pragma unroll
-        for (int c = 0; c < C; ++c) {
-            const cpx<R> x = buf[p * NL + c * W + col];
-            er[c] = x.x;
-            ei[c] = x.y;
-        }
-        gamma_part<R, C>(er, g, gg, lam, mu, s0, sh, sa, orr);
-        gamma_part<R, C>(ei, g, gg, lam, mu, s0, sh, sa, oi);
-        if (nyq) {
-            // average with the mirror bin's unit vector (Nyquist components negated)
-            R gm[3] = {g[0], g[1], g[2]};
-#pragma unroll
-            for (int a = 0; a < DIM; ++a)
-                if (2 * idx[a] == k.n[a]) gm[a] = -gm[a];
-            R ggm[6];
-            products<R, DIM>(gm, ggm);
-            R o2r[C], o2i[C];
-            gamma_part<R, C>(er, gm, ggm, lam, mu, s0, sh, sa, o2r);
-            gamma_part<R, C>(ei, gm, ggm, lam, mu, s0, sh, sa, o2i);
-#pragma unroll
-            for (int c = 0; c < C; ++c) {
-                orr[c] = (orr[c] + o2r[c]) * (R)0.5;
-                oi[c] = (oi[c] + o2i[c]) * (R)0.5;
-            }
-        }
-#pragma unroll
-        for (int c = 0; c < C; ++c) buf[p * NL + c * W + col] = {orr[c], oi[c]};
-    }
-    __syncthreads();
-    fft_lines<R, LZ, NL, NL, NTF>(buf, tw, (R)1);
-    for (int e = tid; e < NE; e += NTF) {
-        const int line = e % NL, p = e / NL;
-        const int c = line / W, col = line % W;
-        if (k0base + col < k.h0) spec[((long long)c * k.rows + (long long)p * pstride + i1) * k.hp + k0base + col] = buf[e];
-    }
-}
-
-// ---------------------------------------------------------------------------------
 // k_inv_x_disp: C2R along x of the DIM displacement components left in the component
 // slots 0..DIM-1 by the OP 1 fused pass, u_a = Re/N + supercell position
 // (STZ/elastic.c:884-899), into u[a][v] (FP64)
@@ -265,35 +102,6 @@ bool supported_len(int L)
     default:
         return false;
     }
-}
-
-// k0 columns per tile for the strided passes
-constexpr int cols_for(int L) { return L >= 512 ? 4 : (L >= 256 ? 8 : 16); }
-// k0 columns of the fused tile: the largest power of two (2..16) keeping the
-// LZ x C*W tile within 96 KB of LDS
-template <typename R, int C>
-constexpr int fused_cols_for(int L)
-{
-    int w = 16;
-    while (w > 2 && (long long)L * C * w * (long long)sizeof(cpx<R>) > 98304) w /= 2;
-    return w;
-}
-constexpr int fused_threads(int elems) { return elems >= 4096 ? 1024 : (elems >= 2048 ? 512 : 256); }
-
-// Tile variants the autotuner may pick (compiled for every length): row counts
-// NR in [2, 2*rows_for] for the x passes with a tile of at most 64 KB (the default may
-// be below rows_for when the row total is not a multiple of it), y-pass
-// widths W in [cols_for/2, 2*cols_for] with at most 64 KB, fused widths W in
-// [fused_cols_for/4, fused_cols_for].
-template <typename R>
-constexpr bool y_variant(int L, int W)
-{
-    return W >= 4 && W * 2 >= cols_for(L) && W <= 2 * cols_for(L) && (long long)L * W * (long long)sizeof(cpx<R>) <= 65536;
-}
-template <typename R, int C>
-constexpr bool f_variant(int L, int W)
-{
-    return W >= 2 && W * 4 >= fused_cols_for<R, C>(L) && W <= fused_cols_for<R, C>(L);
 }
 
 template <typename R>

@@ -17,7 +17,7 @@ LIB_PATH = os.path.join(PKG, "libstz.so")
 FP64, MIXED = 0, 1
 TENSION, SHEAR = 0, 1
 SOLVER_FFT, SOLVER_LOCAL = 0, 1
-F_EPS0, F_SIGMA, F_Q0, F_DEPS, F_DEPS_LAST, F_PERM, F_TEMP, F_TLAST, F_MINQ = range(9)
+F_EPS0, F_SIGMA, F_Q0, F_DEPS, F_DEPS_LAST, F_PERM, F_TEMP, F_TLAST, F_MINQ, F_TLOCAL = range(10)
 ELASTIC, THERMAL, ATHERMAL = 0, 1, 2
 ERRORS = {-1: "selection failed", -2: "HIP error", -3: "bad argument", -4: "NaN in stress solve",
           -5: "unsupported (crystal-like MRO)"}
@@ -140,6 +140,8 @@ def lib():
         L.stz_displacement.argtypes = [vp, dp]
         L.stz_checkpoint_load.argtypes = [vp, C.c_char_p]
         L.stz_set_moduli.argtypes = [vp, dp, dp]
+        L.stz_set_heat_conduct.argtypes = [vp, C.c_int]
+        L.stz_heat_conduct.argtypes = [vp, C.c_double]
         L.stz_inhom_iterations.argtypes = [vp]
         _lib = L
     return _lib
@@ -259,6 +261,14 @@ class Stz:
         if lam.size != self.N or mu.size != self.N:
             raise ValueError("moduli must have one value per voxel")
         self._check(lib().stz_set_moduli(self._h, _dp(lam), _dp(mu)))
+
+    def set_heat_conduct(self, on=True):
+        """HEAT_CONDUCT: diffuse F_TLOCAL after thermal and elastic steps (see include/stz.h)."""
+        self._check(lib().stz_set_heat_conduct(self._h, 1 if on else 0))
+
+    def heat_conduct(self, dt):
+        """One HeatConducting(dt) of the local temperature field."""
+        self._check(lib().stz_heat_conduct(self._h, float(dt)))
 
     def lame(self):
         """(lambda, mu) of the reference medium, derived from the parameters as the library

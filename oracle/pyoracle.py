@@ -42,7 +42,7 @@ class OraMT(C.Structure):
     _fields_ = [("mt", C.c_uint32 * 624), ("mti", C.c_int)]
 
 
-F_EPS0, F_SIGMA, F_Q0, F_DEPS, F_DEPS_LAST, F_PERM, F_TEMP, F_TLAST, F_Q, F_SOFT, F_EPSO = range(11)
+F_EPS0, F_SIGMA, F_Q0, F_DEPS, F_DEPS_LAST, F_PERM, F_TEMP, F_TLAST, F_Q, F_SOFT, F_EPSO, F_TLOCAL = range(12)
 
 _lib = None
 
@@ -94,6 +94,8 @@ def lib():
         L.ora_set_strainhist.argtypes = [C.c_void_p, C.c_double]
         L.ora_displacement.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
         L.ora_set_moduli.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.ora_heat_conduct.argtypes = [C.c_void_p, C.c_double]
+        L.ora_set_heat.argtypes = [C.c_void_p, C.c_int]
         L.ora_inhom_iterations.argtypes = [C.c_void_p]
         L.ora_inhom_iterations.restype = C.c_int
         L.ora_set_threads.argtypes = [C.c_int]
@@ -185,6 +187,12 @@ class Oracle:
         self._mu = np.ascontiguousarray(mu, dtype=np.float64)
         dp = C.POINTER(C.c_double)
         lib().ora_set_moduli(self._h, self._lam.ctypes.data_as(dp), self._mu.ctypes.data_as(dp))
+
+    def heat_conduct(self, dt):
+        return lib().ora_heat_conduct(self._h, float(dt))
+
+    def set_heat(self, on):
+        lib().ora_set_heat(self._h, 1 if on else 0)
 
     def inhom_iterations(self):
         return lib().ora_inhom_iterations(self._h)

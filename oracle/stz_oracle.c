@@ -31,6 +31,11 @@
  *                     compatible eps0 -> zero fluctuation stress; laminate
  *                     closed form; k.sigma_hat = 0 (equilibrium); brute-force
  *                     dense Green's-operator solve on a 4x4 / 4x4x4 grid.
+ *   ora_heat_conduct  HeatConducting, STZ/elastic.c:549-582 (HEAT_CONDUCT; Tlocal is heated by
+ *                     every transform, STZ/elastic.c:381-383, flag or not).  pins: a single
+ *                     Fourier mode decays by exactly exp(-|k| dt); the k = 0 mode by
+ *                     exp(-dt) (g2 = 1 there); the transform's heating against a
+ *                     hand-computed sigma:deps.
  *   ora_set_moduli    Lambda(x), Mu(x) of STZ/init.c:457-465 (uniform when unset); the
  *                     wrapper then iterates to its fixed point, STZ/elastic.c:990-1062.
  *                     pins: explicit uniform moduli give the uniform path bit for bit;
@@ -90,6 +95,10 @@
 #define BOLZ 1.380658e-23
 #define ORA_PI 3.14159265359
 #define ORA_TWO_PI (2 * ORA_PI)
+/* thermal_coeff1 = 1/Cp/rho with Cp = 420 J/kg-K, rho = 6125 kg/m^3; thermal_diffusivity
+   3e-6 m^2/s (STZ/bmg.h:185-188) */
+#define ORA_THERMAL_COEFF1 (1 / 420.0 / 6125.0)
+#define ORA_THERMAL_DIFFUSIVITY 3E-6
 
 /* ------------------------------------------------------------------ */
 /* MT19937 as GSL's mt19937 (seed 0 -> 4357)                             */
@@ -299,6 +308,9 @@ struct ora_state {
     /* solver scratch */
     double *spec;           /* C complex fields, N each */
     double *tau0, *epsolast;
+    /* local temperature (STZ/init.c:470-476) and |k| of the reference grid (Grad's g2) */
+    double *tlocal, *g2;
+    int heat;               /* HEAT_CONDUCT on */
     /* elastic inhomogeneity: Lambda[N], Mu[N] (NULL: uniform, STZ/init.c:457-465) */
     double *lamx, *mux, *estar;
     int inhom_iters;
@@ -410,6 +422,7 @@ static void grad_tables(ora_state *s)
         if (s->p.dim == 2) g2 = sqrt(g[0] * g[0] + g[1] * g[1]);
         else g2 = sqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]);
         if (idx[0] + idx[1] + idx[2] == 0) g2 = 1.0;
+        s->g2[v] = g2;
         for (int a = 0; a < 3; a++) { g[a] /= g2; s->g[a][v] = g[a]; }
         if (s->p.dim == 2) {
             s->gg[0][v] = g[0] * g[0];
@@ -474,13 +487,17 @@ static ora_state *alloc_state(const ora_params *p)
     s->spec = (double *)calloc((size_t)(2 * C * N), sizeof(double));
     s->tau0 = (double *)calloc((size_t)(C * N), sizeof(double));
     s->epsolast = (double *)calloc((size_t)(C * N), sizeof(double));
+    s->tlocal = (double *)calloc((size_t)N, sizeof(double));
+    s->g2 = (double *)calloc((size_t)N, sizeof(double));
     if (!s->eps0 || !s->sigma || !s->epso || !s->q0 || !s->deps || !s->deps_last || !s->perm ||
-        !s->temp || !s->tlast || !s->q || !s->soft || !s->rate || !s->spec || !s->tau0 || !s->epsolast) {
+        !s->temp || !s->tlast || !s->q || !s->soft || !s->rate || !s->spec || !s->tau0 || !s->epsolast ||
+        !s->tlocal || !s->g2) {
         ora_destroy(s);
         return NULL;
     }
     grad_tables(s);
     for (long v = 0; v < N; v++) s->tlast[v] = -100 * s->tTemp; /* STZ/init.c:312 */
+    for (long v = 0; v < N; v++) s->tlocal[v] = p->temperature; /* STZ/init.c:470-476 */
     ora_mt_seed(&s->rng, (unsigned long)p->seed);              /* STZ/init.c:143-147, rank 0 */
     s->printperiod = 0;
     s->endstrain = 1e300;
@@ -496,6 +513,7 @@ void ora_destroy(ora_state *s)
     free(s->perm); free(s->temp); free(s->tlast); free(s->q); free(s->soft); free(s->rate);
     free(s->spec); free(s->tau0); free(s->epsolast);
     free(s->lamx); free(s->mux); free(s->estar);
+    free(s->tlocal); free(s->g2);
     for (int a = 0; a < 3; a++) free(s->g[a]);
     for (int a = 0; a < 6; a++) free(s->gg[a]);
     free(s);
@@ -545,6 +563,7 @@ double *ora_field(ora_state *s, int which)
     case ORA_F_Q: return s->q;
     case ORA_F_SOFTENING: return s->soft;
     case ORA_F_EPSO: return s->epso;
+    case ORA_F_TLOCAL: return s->tlocal;
     default: return NULL;
     }
 }
@@ -953,6 +972,12 @@ void ora_transform(ora_state *s, int mm, int site)
     deps_of(s, mm, v, d);
     for (int k = 0; k < s->S; k++) s->deps_last[(long)k * N + v] = s->deps[((long)mm * s->S + k) * N + v];
     for (int c = 0; c < s->C; c++) s->eps0[(long)c * N + v] += d[c];            /* :377-379 */
+    {
+        double sg[6] = {0};
+        for (int c = 0; c < s->C; c++) sg[c] = s->sigma[(long)c * N + v];
+        const double dh = contract(s, sg, d) * 1E9;                              /* :381-382, J/m^3 */
+        s->tlocal[v] += fabs(dh) * ORA_THERMAL_COEFF1;                          /* :383 */
+    }
     double ds = contract(s, d, d);                                             /* :385-386 */
     s->perm[v] += ds * s->permsoft;                                            /* :390 */
     if (s->perm[v] > s->permcap) s->perm[v] = s->permcap;                      /* :391 */
@@ -987,6 +1012,33 @@ static void athermal_correction(ora_state *s)
     isotropicinv(s, ts, te);
     for (int c = 0; c < s->C; c++) s->epsavg[c] -= te[c];
 }
+
+/* HeatConducting(dt), STZ/elastic.c:549-582: Tlocal through the full complex transform,
+   both parts times exp(-g2*dt) (g2 = |k| of the reference grid, 1 at k = 0), back, /N.
+   Returns -4 on a NaN, as the reference aborts with -4. */
+int ora_heat_conduct(ora_state *s, double dt)
+{
+    const long N = s->N;
+    double *f = s->spec;                      /* scratch: one complex field */
+#pragma omp parallel for schedule(static)
+    for (long v = 0; v < N; v++) { f[2 * v] = s->tlocal[v]; f[2 * v + 1] = 0.0; }
+    ora_fft3d(f, s->p.n[0], s->p.n[1], s->p.n[2], -1);
+    int bad = 0;
+#pragma omp parallel for schedule(static) reduction(|:bad)
+    for (long v = 0; v < N; v++) {
+        const double m = exp(-s->g2[v] * dt);
+        f[2 * v] *= m;
+        f[2 * v + 1] *= m;
+        if (isnan(f[2 * v]) || isnan(f[2 * v + 1])) bad = 1;
+    }
+    if (bad) return -4;
+    ora_fft3d(f, s->p.n[0], s->p.n[1], s->p.n[2], +1);
+#pragma omp parallel for schedule(static)
+    for (long v = 0; v < N; v++) s->tlocal[v] = f[2 * v] / (double)N;
+    return 0;
+}
+
+void ora_set_heat(ora_state *s, int on) { s->heat = on != 0; }
 
 static int athermal(ora_state *s, ora_step_info *info)
 {
@@ -1039,6 +1091,7 @@ static int kmc(ora_state *s, ora_step_info *info)
         if (ora_select(s, dice, &site, &mode)) return -1;
         double qch = s->q[(long)mode * N + site];
         ora_transform(s, mode, site);
+        if (s->heat && ora_heat_conduct(s, ORA_THERMAL_DIFFUSIVITY * t_kmc)) return -4;   /* STZ/evolution.c:247-249 */
         ora_stress_solve(s);
         double ts[6] = {0, 0, 0, 0, 0, 0}, te[6];
         if (s->p.loading == 1) {              /* SHEAR, STZ/evolution.c:298-315 */
@@ -1073,6 +1126,7 @@ static int kmc(ora_state *s, ora_step_info *info)
     } else {
         s->estep++;
         s->totime += s->dt;
+        if (s->heat && ora_heat_conduct(s, ORA_THERMAL_DIFFUSIVITY * s->dt)) return -4;  /* STZ/evolution.c:325-327 */
         const double inc = s->p.strainincrement;
         if (s->p.loading == 1) {              /* STZ/evolution.c:349-356 */
             s->epsavg[lc] += sr * 1;

@@ -87,7 +87,8 @@ enum {
     ORA_F_PERM = 5, ORA_F_TEMP = 6, ORA_F_TLAST = 7,
     ORA_F_Q = 8,          /* [M][N] last ActEnergy output */
     ORA_F_SOFTENING = 9,  /* [N] */
-    ORA_F_EPSO = 10       /* [C][N] total strain of the last solve */
+    ORA_F_EPSO = 10,      /* [C][N] total strain of the last solve */
+    ORA_F_TLOCAL = 11     /* [N] local temperature */
 };
 double *ora_field(ora_state *s, int which);
 void    ora_scalars(ora_state *s, double *epsavg, double *stress, double *totime);
@@ -99,6 +100,8 @@ void    ora_counters(ora_state *s, long long *out8);
 /* --- hot-path pieces (each follows one reference function) ------------ */
 void   ora_stress_solve(ora_state *s);               /* InhomogeneousStressSolver(uflag=0) */
 void   ora_displacement(ora_state *s, double *u);    /* InhomogeneousStressSolver(uflag=2): re-solve + u[D][N] */
+int    ora_heat_conduct(ora_state *s, double dt);    /* HeatConducting(dt); -4 on NaN */
+void   ora_set_heat(ora_state *s, int on);           /* HEAT_CONDUCT: diffuse Tlocal after thermal/elastic steps */
 void   ora_set_moduli(ora_state *s, const double *lam, const double *mu); /* Lambda(x), Mu(x); NULL: uniform */
 int    ora_inhom_iterations(ora_state *s);           /* homogeneous solves of the last stress solve */
 void   ora_softening(ora_state *s);                  /* SofteningFactor */
