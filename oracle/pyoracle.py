@@ -26,6 +26,7 @@ class OraParams(C.Structure):
         ("strainrate", C.c_double), ("strainincrement", C.c_double),
         ("perm_soft", C.c_double), ("temp_soft", C.c_double), ("perm_cap", C.c_double),
         ("temp_soft_barrier", C.c_double), ("athermal_threshold", C.c_double),
+        ("mro_fraction", C.c_double), ("mro_q0", C.c_double), ("random_ratio", C.c_double),
     ]
 
 
@@ -34,7 +35,7 @@ class OraStepInfo(C.Structure):
         ("kind", C.c_int), ("site", C.c_int), ("mode", C.c_int), ("n_transformed", C.c_int),
         ("n_negative", C.c_longlong), ("q_chosen", C.c_double), ("total_rate", C.c_double),
         ("dt_kmc", C.c_double), ("totime", C.c_double), ("epsavg", C.c_double * 6),
-        ("stress", C.c_double * 6), ("printed", C.c_int),
+        ("stress", C.c_double * 6), ("printed", C.c_int), ("mro", C.c_int),
     ]
 
 
@@ -95,6 +96,10 @@ def lib():
         L.ora_displacement.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
         L.ora_set_moduli.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.ora_heat_conduct.argtypes = [C.c_void_p, C.c_double]
+        L.ora_create_mro.argtypes = [C.POINTER(OraParams), C.POINTER(C.c_int), C.POINTER(C.c_double)]
+        L.ora_create_mro.restype = C.c_void_p
+        L.ora_mro_map.argtypes = [C.c_void_p]
+        L.ora_mro_map.restype = C.POINTER(C.c_ubyte)
         L.ora_set_heat.argtypes = [C.c_void_p, C.c_int]
         L.ora_inhom_iterations.argtypes = [C.c_void_p]
         L.ora_inhom_iterations.restype = C.c_int
@@ -111,6 +116,7 @@ REFERENCE_2D = dict(
     gamma0=0.14, temperature=300.0, trial_frequency=1e13, lx=0.7, ly=0.7, lz=1.0,
     strainrate=1e-4, strainincrement=1e-4, perm_soft=13.0, temp_soft=39.0,
     perm_cap=float(-np.log(0.6)), temp_soft_barrier=0.37, athermal_threshold=0.33,
+    mro_fraction=0.0, mro_q0=2.0, random_ratio=0.2,
 )
 
 
@@ -130,10 +136,17 @@ def make_params(**kw):
 class Oracle:
     """One oracle state (the reference's global state, one rank)."""
 
-    def __init__(self, empty=False, **kw):
+    def __init__(self, empty=False, mro_map=None, **kw):
+        """mro_map = (types[N], thetas[N]): the InitMRO 1 path (a map read from a file)."""
         self.params = make_params(**kw)
         L = lib()
-        self._h = (L.ora_create_empty if empty else L.ora_create)(C.byref(self.params))
+        if mro_map is not None:
+            t = np.ascontiguousarray(mro_map[0], dtype=np.int32)
+            th = np.ascontiguousarray(mro_map[1], dtype=np.float64)
+            self._h = L.ora_create_mro(C.byref(self.params), t.ctypes.data_as(C.POINTER(C.c_int)),
+                                       th.ctypes.data_as(C.POINTER(C.c_double)))
+        else:
+            self._h = (L.ora_create_empty if empty else L.ora_create)(C.byref(self.params))
         if not self._h:
             raise RuntimeError("ora_create failed")
         c = np.zeros(16)
@@ -187,6 +200,9 @@ class Oracle:
         self._mu = np.ascontiguousarray(mu, dtype=np.float64)
         dp = C.POINTER(C.c_double)
         lib().ora_set_moduli(self._h, self._lam.ctypes.data_as(dp), self._mu.ctypes.data_as(dp))
+
+    def mro_map(self):
+        return np.ctypeslib.as_array(lib().ora_mro_map(self._h), shape=(self.N,)).copy()
 
     def heat_conduct(self, dt):
         return lib().ora_heat_conduct(self._h, float(dt))

@@ -1,3 +1,4 @@
+#define _GNU_SOURCE
 /*
  * stz_oracle.c -- TEST INFRASTRUCTURE ONLY (see stz_oracle.h).
  *
@@ -308,6 +309,10 @@ struct ora_state {
     /* solver scratch */
     double *spec;           /* C complex fields, N each */
     double *tau0, *epsolast;
+    /* crystal-like MRO: type per voxel (0 glass, 1 six-fold, 2 two-fold, 3 ten-fold) and
+       the orientation MROthe0 (STZ/init.c:345-437) */
+    unsigned char *mro;
+    double *mroth;
     /* local temperature (STZ/init.c:470-476) and |k| of the reference grid (Grad's g2) */
     double *tlocal, *g2;
     int heat;               /* HEAT_CONDUCT on */
@@ -489,7 +494,9 @@ static ora_state *alloc_state(const ora_params *p)
     s->epsolast = (double *)calloc((size_t)(C * N), sizeof(double));
     s->tlocal = (double *)calloc((size_t)N, sizeof(double));
     s->g2 = (double *)calloc((size_t)N, sizeof(double));
-    if (!s->eps0 || !s->sigma || !s->epso || !s->q0 || !s->deps || !s->deps_last || !s->perm ||
+    s->mro = (unsigned char *)calloc((size_t)N, 1);
+    s->mroth = (double *)calloc((size_t)N, sizeof(double));
+    if (!s->mro || !s->mroth || !s->eps0 || !s->sigma || !s->epso || !s->q0 || !s->deps || !s->deps_last || !s->perm ||
         !s->temp || !s->tlast || !s->q || !s->soft || !s->rate || !s->spec || !s->tau0 || !s->epsolast ||
         !s->tlocal || !s->g2) {
         ora_destroy(s);
@@ -514,6 +521,7 @@ void ora_destroy(ora_state *s)
     free(s->spec); free(s->tau0); free(s->epsolast);
     free(s->lamx); free(s->mux); free(s->estar);
     free(s->tlocal); free(s->g2);
+    free(s->mro); free(s->mroth);
     for (int a = 0; a < 3; a++) free(s->g[a]);
     for (int a = 0; a < 6; a++) free(s->gg[a]);
     free(s);
@@ -526,28 +534,106 @@ static void draw_deps(ora_state *s, int m, long v)
         s->deps[((long)m * s->S + k) * s->N + v] = a * ora_gaussian(&s->rng, 1.0);
 }
 
-ora_state *ora_create(const ora_params *p)
+/* modes of a crystal-like MRO family and the scale of its mode-0 strain
+   (STZ/bmg.h:30-32; STZ/elastic.c:67-69, 86-88, 105-107) */
+static inline int mro_modes(int t) { return t == 1 ? 6 : (t == 2 ? 2 : 12); }
+
+/* DStrain for one (mode, voxel), STZ/elastic.c:60-138: glass draws two Gaussians; a
+   crystal-like voxel's mode 0 is the family's strain along MROthe0, modes 1..XMODES-1 its
+   rotations by 2*theta_m, theta_m = m*2*PI/XMODES*(1 + 0.2(2U - 1)), each term scaled by
+   (1 + 0.2(2U - 1)); modes from XMODES on have zero strain.  The four uniforms inside the
+   two expressions are drawn left operand first, the order g++ -O3 -std=c++11 (the
+   reference's build, STZ/Makefile:1-3) evaluates them in. */
+static void dstrain(ora_state *s, int m, long v)
 {
-    ora_state *s = alloc_state(p);
-    if (!s) return NULL;
+    const int t = s->mro[v];
+    if (t == 0) { draw_deps(s, m, v); return; }
     const long N = s->N;
-    /* MRO map, one uniform per voxel (STZ/init.c:355-396); only glass is restated */
-    for (long v = 0; v < N; v++) {
-        double tmp = ora_mt_uniform(&s->rng);
-        if (tmp <= 0.0) { ora_destroy(s); return NULL; } /* crystal-like MRO voxel: out of scope */
+    double *xx = &s->deps[((long)m * s->S + 0) * N + v];
+    double *xy = &s->deps[((long)m * s->S + 1) * N + v];
+    const int XM = mro_modes(t);
+    if (m == 0) {
+        const double amp = t == 2 ? 0.8 * s->p.gamma0 : 0.5 * s->p.gamma0;
+        /* two separate calls: the reference reads MROthe0[idx] again after storing through
+           the DEps0 pointers, so the compiler cannot pair them */
+        *xx = amp * cos(s->mroth[v]);
+        *xy = amp * sin(s->mroth[v]);
+    } else if (m < XM) {
+        const double a0 = s->deps[(0 + 0) * N + v], a2 = s->deps[(0 + 1) * N + v];
+        const double th = m * 2. * ORA_PI / XM * (1. + 0.2 * (2 * ora_mt_uniform(&s->rng) - 1.));
+        /* cos(2*th) and sin(2*th) of one local: g++ computes the pair with one sincos call,
+           whose sine can differ from sin() in the last bit */
+        double s2, c2;
+        sincos(2 * th, &s2, &c2);
+        const double u1 = ora_mt_uniform(&s->rng);
+        const double u2 = ora_mt_uniform(&s->rng);
+        *xx = a0 * (1. + 0.2 * (2 * u1 - 1.)) * c2 + a2 * (1. + 0.2 * (2 * u2 - 1.)) * s2;
+        const double u3 = ora_mt_uniform(&s->rng);
+        const double u4 = ora_mt_uniform(&s->rng);
+        *xy = -a0 * (1. + 0.2 * (2 * u3 - 1.)) * s2 + a2 * (1. + 0.2 * (2 * u4 - 1.)) * c2;
+    } else {
+        *xx = 0.;
+        *xy = 0.;
     }
-    /* Energy_Init, STZ/init.c:181-198 */
+}
+
+/* Energy_Init (STZ/init.c:181-198, every voxel alike), DStrain, initial solve */
+static ora_state *init_fields(ora_state *s)
+{
+    const long N = s->N;
+    for (long v = 0; v < N; v++)
+        if (s->mro[v] && s->p.dim == 3) { ora_destroy(s); return NULL; }   /* 2-D families only */
     for (int m = 0; m < s->M; m++)
         for (long v = 0; v < N; v++)
             s->q0[(long)m * N + v] = ora_gaussian(&s->rng, s->p.q0_std) + s->p.q0_mean;
-    /* DStrain glass branch, STZ/elastic.c:124-129 */
     for (int m = 0; m < s->M; m++)
-        for (long v = 0; v < N; v++) draw_deps(s, m, v);
+        for (long v = 0; v < N; v++) dstrain(s, m, v);
     /* initial solve with eps0 = 0, epsavg = 0 (STZ/bmg.c:117-127) */
     for (int c = 0; c < 6; c++) s->epsavg[c] = 0.0;
     ora_stress_solve(s);
     return s;
 }
+
+ora_state *ora_create(const ora_params *p)
+{
+    ora_state *s = alloc_state(p);
+    if (!s) return NULL;
+    const long N = s->N;
+    /* MRO map, InitMRO 0 (STZ/init.c:350-397): one uniform per voxel, six-fold crystal-like
+       MRO when it is <= xMROFrac, with MROthe0 = U*PI - PI/2 from a second uniform */
+    for (long v = 0; v < N; v++) {
+        const double tmp = ora_mt_uniform(&s->rng);
+        if (tmp <= s->p.mro_fraction) {
+            s->mro[v] = 1;
+            s->mroth[v] = ora_mt_uniform(&s->rng) * ORA_PI - ORA_PI / 2.0;
+        } else {
+            s->mro[v] = 0;
+            s->mroth[v] = 0.;
+        }
+    }
+    return init_fields(s);
+}
+
+ora_state *ora_create_mro(const ora_params *p, const int *type, const double *theta)
+{
+    ora_state *s = alloc_state(p);
+    if (!s) return NULL;
+    const long N = s->N;
+    for (long v = 0; v < N; v++) {
+        if (type[v] < 0 || type[v] > 3) { ora_destroy(s); return NULL; }   /* STZ/elastic.c:131-136 */
+        s->mro[v] = (unsigned char)type[v];
+        s->mroth[v] = theta[v];
+    }
+    /* InitMRO 1 (STZ/init.c:398-431): ten-fold voxels become two-fold with probability 0.5 */
+    for (long v = 0; v < N; v++)
+        if (s->mro[v] == 3) {
+            const double tmp = ora_mt_uniform(&s->rng);
+            if (tmp < 0.5) s->mro[v] = 2;
+        }
+    return init_fields(s);
+}
+
+const unsigned char *ora_mro_map(ora_state *s) { return s->mro; }
 
 double *ora_field(ora_state *s, int which)
 {
@@ -918,6 +1004,14 @@ long long ora_act_energy(ora_state *s)
             double norm2 = sqrt(contract(s, pd, pd));                              /* :617-618 */
             double dm = dot / (norm1 + ORA_EPSILON) / (norm2 + ORA_EPSILON);       /* :620 */
             dm = (1 + dm) * (1 + dm) / 4;                                          /* :621 */
+            /* NO_SF_XTAL_MRO (:645-667): the modifier and the voxel's Softening shrink by
+               0.1 / 0.01 / 0.9, Softening in place, so it compounds over the mode loop */
+            const int t = s->mro[v];
+            if (t) {
+                const double f = t == 1 ? 0.1 : (t == 2 ? 0.01 : 0.9);
+                dm *= f;
+                s->soft[v] *= f;
+            }
             double qv = s->q0[(long)m * N + v] * exp(-dm * s->soft[v]) - dwork / 2; /* :684 */
             s->q[(long)m * N + v] = qv;
             if (qv < 0) nneg++;                                                    /* bmg.c:152 */
@@ -979,11 +1073,18 @@ void ora_transform(ora_state *s, int mm, int site)
         s->tlocal[v] += fabs(dh) * ORA_THERMAL_COEFF1;                          /* :383 */
     }
     double ds = contract(s, d, d);                                             /* :385-386 */
-    s->perm[v] += ds * s->permsoft;                                            /* :390 */
+    const int six = s->mro[v] == 1;                                            /* PermSoftmro, TempSoftmro */
+    s->perm[v] += ds * (six ? 20.0 : s->permsoft);                             /* :387-390; STZ/init.c:60-63 */
     if (s->perm[v] > s->permcap) s->perm[v] = s->permcap;                      /* :391 */
-    s->temp[v] = ds * s->tempsoft;                                             /* :396 */
+    s->temp[v] = ds * (six ? 60.0 : s->tempsoft);                              /* :393-396 */
     s->tlast[v] = s->totime;                                                   /* :397 */
-    for (int m = 0; m < s->M; m++) draw_deps(s, m, v);                         /* :508-512 */
+    if (s->mro[v] == 0) {
+        for (int m = 0; m < s->M; m++) draw_deps(s, m, v);                     /* :508-512 */
+    } else {
+        /* crystal-like MRO: new barriers, the strains stay (:446-506) */
+        for (int m = 0; m < s->M; m++)
+            s->q0[(long)m * N + v] = s->p.mro_q0 * (1 - s->p.random_ratio + s->p.random_ratio * (ora_mt_uniform(&s->rng)));
+    }
 }
 
 double ora_elastic_energy(ora_state *s)
@@ -1123,6 +1224,7 @@ static int kmc(ora_state *s, ora_step_info *info)
         info->site = site;
         info->mode = mode;
         info->q_chosen = qch;
+        info->mro = s->mro[site];                                              /* chosenM, :206, 220 */
     } else {
         s->estep++;
         s->totime += s->dt;

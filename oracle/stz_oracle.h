@@ -38,6 +38,9 @@ typedef struct ora_params {
     double strainrate, strainincrement;
     double perm_soft, temp_soft, perm_cap, temp_soft_barrier;
     double athermal_threshold;
+    double mro_fraction;     /* xMROFrac: volume fraction of crystal-like MRO (InitMRO 0) */
+    double mro_q0;           /* xQ0amp: barrier scale of crystal-like MRO voxels */
+    double random_ratio;     /* RandomRatio: spread of their redrawn barriers */
 } ora_params;
 
 typedef struct ora_state ora_state;
@@ -58,6 +61,7 @@ typedef struct ora_step_info {
     double epsavg[6];
     double stress[6];
     int printed;       /* 1 if the print block ran (and re-solved the stress) */
+    int mro;           /* thermal: MRO type of the selected voxel (actQ.out's first column) */
 } ora_step_info;
 
 /* --- RNG (GSL mt19937 + polar Gaussian) ------------------------------- */
@@ -75,6 +79,9 @@ int  ora_fft3d(double *data, int n0, int n1, int n2, int sign);
 /* --- state ------------------------------------------------------------- */
 ora_state *ora_create(const ora_params *p);   /* Init + Alloc + Energy_Init + DStrain + initial solve */
 ora_state *ora_create_empty(const ora_params *p); /* same sizes, fields zero, no RNG draws */
+/* InitMRO 1: the MRO map (types 0..3) and angles of MROstructure.ms given by the caller */
+ora_state *ora_create_mro(const ora_params *p, const int *type, const double *theta);
+const unsigned char *ora_mro_map(ora_state *s);      /* [N] MRO types */
 void       ora_destroy(ora_state *s);
 
 /* fields: index into the state; returns pointer to N doubles (or M*N for per-mode) */
