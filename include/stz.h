@@ -53,6 +53,7 @@ extern "C" {
 #define STZ_F_TLAST 7
 #define STZ_F_MINQ 8       /* download only: per-voxel min over modes of Q from the last stz_rates */
 #define STZ_F_TLOCAL 9     /* local temperature Tlocal (STZ/init.c:470-476), FP64 */
+#define STZ_F_MRO 10       /* download only: MRO type per voxel (0 glass, 1 six-, 2 two-, 3 ten-fold) */
 
 /* step kinds */
 #define STZ_STEP_ELASTIC 0
@@ -87,6 +88,10 @@ typedef struct stz_params {
     double perm_soft, temp_soft, perm_cap; /* STZ/init.c:60-68 */
     double temp_soft_barrier;              /* 0.37 eV in tTemp, STZ/init.c:49 */
     double athermal_threshold;             /* 0.33 eV, STZ/evolution.c:65 */
+    /* crystal-like medium-range order (2-D only; all 0: glass everywhere) */
+    double mro_fraction;     /* "Volume fraction of crystal-like MRO", STZ/io.c:83-86 (InitMRO 0) */
+    double mro_q0;           /* "Stress-free energy barrier for crystal-like MRO", STZ/io.c:93-97 */
+    double random_ratio;     /* "Random ratio of fluctuation in energy barrier", STZ/io.c:37-40 */
 } stz_params;
 
 typedef struct stz_step_result {
@@ -113,9 +118,22 @@ void stz_destroy(stz_ctx *ctx);
 const char *stz_last_error(const stz_ctx *ctx);
 
 /* Draws the initial state from the context's MT19937 stream in the reference order
-   (MRO map, Energy_Init, DStrain: STZ/init.c:355-396, 181-198; STZ/elastic.c:124-129)
-   and runs the initial stress solve (STZ/bmg.c:117-127). */
+   (MRO map, Energy_Init, DStrain: STZ/init.c:350-397, 181-198; STZ/elastic.c:60-138)
+   and runs the initial stress solve (STZ/bmg.c:117-127).  The MRO map is InitMRO 0: a
+   voxel whose uniform is <= mro_fraction is six-fold crystal-like MRO, oriented by a
+   second uniform (2-D only). */
 int stz_init_state(stz_ctx *ctx);
+
+/* InitMRO 1 (STZ/init.c:398-431): stz_init_state with the MRO map read from
+   MROstructure.ms by the caller -- type[v] in {0 glass, 1 six-fold, 2 two-fold, 3 ten-fold}
+   and the orientation MROthe0 theta[v] per voxel in the reference's voxel order; ten-fold
+   voxels turn two-fold with probability 0.5 (one uniform each, voxel order), then Q0, the
+   transformation strains of each family (STZ/elastic.c:60-138) and the initial solve.
+   Crystal-like MRO is 2-D only (STZ_E_UNSUPPORTED in 3-D).  With MRO voxels, steps run on
+   the host path (stz_run_steps falls back to it) and ensembles return STZ_E_UNSUPPORTED;
+   a crystal-like voxel's transform redraws its M barriers from M uniforms (the first M of
+   the M*S values stz_apply_event / stz_athermal_apply take per voxel). */
+int stz_init_state_mro(stz_ctx *ctx, const int *type, const double *theta);
 
 /* Initial state drawn on the GPU for large grids, where the reference's sequential
    host stream (stz_init_state) would take minutes: the same distributions from the

@@ -138,7 +138,7 @@ size_t field_count(const stz_ctx *c, int f)
     case STZ_F_Q0: return (size_t)c->M * N;
     case STZ_F_DEPS: return (size_t)c->M * c->S * N;
     case STZ_F_DEPS_LAST: return (size_t)c->S * N;
-    case STZ_F_PERM: case STZ_F_TEMP: case STZ_F_TLAST: case STZ_F_MINQ: case STZ_F_TLOCAL: return N;
+    case STZ_F_PERM: case STZ_F_TEMP: case STZ_F_TLAST: case STZ_F_MINQ: case STZ_F_TLOCAL: case STZ_F_MRO: return N;
     default: return 0;
     }
 }
@@ -506,12 +506,14 @@ int stz_create(const stz_params *p, stz_ctx **out)
     // host buffers that kernels read directly (fine-grained, coherent)
     const unsigned hflags = hipHostMallocCoherent | hipHostMallocMapped;
     const size_t ms = (size_t)ctx->M * ctx->S;
-    if (hipHostMalloc((void **)&ctx->hspec, (1 + ms) * sizeof(double), hflags) != hipSuccess) return fail(STZ_E_HIP);
+    // dice, the M*S Gaussians of a glass site's redraw, the M uniforms of a crystal-like one
+    if (hipHostMalloc((void **)&ctx->hspec, (1 + ms + ctx->M) * sizeof(double), hflags) != hipSuccess)
+        return fail(STZ_E_HIP);
     if (hipHostMalloc((void **)&ctx->hath, (size_t)ctx->acap_max * ms * sizeof(double), hflags) != hipSuccess)
         return fail(STZ_E_HIP);
     if (hipHostGetDevicePointer((void **)&ctx->dspec_dev, ctx->hspec, 0) != hipSuccess) return fail(STZ_E_HIP);
     if (hipHostGetDevicePointer((void **)&ctx->dath_dev, ctx->hath, 0) != hipSuccess) return fail(STZ_E_HIP);
-    std::memset(ctx->hspec, 0, (1 + ms) * sizeof(double));
+    std::memset(ctx->hspec, 0, (1 + ms + ctx->M) * sizeof(double));
     rc = ensure_dnew(ctx, (size_t)ctx->M * ctx->S);
     if (rc) return fail(rc);
     rc = stz_fft_setup(ctx);
@@ -536,7 +538,7 @@ void stz_destroy(stz_ctx *ctx)
     stz_local_release(ctx);
     stz_inhom_free(ctx);
     void *ptrs[] = {ctx->q0eff, ctx->scache, ctx->eps0, ctx->sigma, ctx->q0, ctx->deps, ctx->deps_last, ctx->perm, ctx->temp, ctx->tlast, ctx->tloc,
-                    ctx->spec, ctx->minq, ctx->argm, ctx->bsum, ctx->gsum, ctx->res, ctx->spart,
+                    ctx->spec, ctx->minq, ctx->argm, ctx->bsum, ctx->gsum, ctx->res, ctx->spart, ctx->mro,
                     ctx->acnt, ctx->aoff, ctx->alist, ctx->dnew, ctx->socp, ctx->gneg};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
@@ -552,37 +554,140 @@ void stz_destroy(stz_ctx *ctx)
 
 const char *stz_last_error(const stz_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
 
-int stz_init_state(stz_ctx *ctx)
+}  // extern "C"
+
+namespace {
+
+// modes of a crystal-like MRO family (STZ/bmg.h:30-32)
+int mro_modes(int t) { return t == 1 ? 6 : (t == 2 ? 2 : 12); }
+
+// Energy_Init (STZ/init.c:181-198), DStrain (STZ/elastic.c:60-138) and the initial solve
+// (STZ/bmg.c:117-127) after the MRO map: glass voxels draw two (2-D) Gaussians per mode;
+// a crystal-like voxel's mode 0 lies along its orientation, modes 1..XMODES-1 are
+// rotations by 2*theta_m with (1 + 0.2(2U - 1)) factors (uniforms taken left operand
+// first, the order g++ evaluates the reference's expressions in; the cos/sin pair of one
+// angle from one sincos call, as g++ compiles it), later modes have zero strain.
+int init_fields(stz_ctx *ctx, const std::vector<double> &theta)
 {
-    if (!ctx) return STZ_E_ARG;
-    stz_rng_sync(ctx);
     const long long N = ctx->N;
-    // MRO map: one uniform per voxel, glass unless u <= xMROFrac = 0 (STZ/init.c:355-396)
-    for (long long v = 0; v < N; ++v) {
-        const double u = ctx->rng.uniform();
-        if (u <= 0.0) {
-            ctx->err = "crystal-like MRO voxel drawn; only the glass branch is supported";
-            return STZ_E_UNSUPPORTED;
-        }
-    }
     std::vector<double> buf((size_t)N * ctx->S);
-    // Energy_Init, STZ/init.c:181-198: mode-major, Q0 = N(0, std) + mean
     for (int m = 0; m < ctx->M; ++m) {
         for (long long v = 0; v < N; ++v) buf[v] = ctx->rng.gaussian(ctx->p.q0_std) + ctx->p.q0_mean;
         int rc = put_real(ctx, ctx->q0, buf.data(), (size_t)N, (size_t)m * N);
         if (rc) return rc;
         STZ_HIP(hipStreamSynchronize(ctx->stream));
     }
-    // DStrain glass branch, STZ/elastic.c:124-129: mode-major, voxel, then components
+    std::vector<double> first;                 // mode 0 of the crystal-like voxels: (xx, xy)
+    if (ctx->nmro) first.assign((size_t)2 * N, 0.0);
+    const double g0 = ctx->p.gamma0;
     for (int m = 0; m < ctx->M; ++m) {
-        for (long long v = 0; v < N; ++v)
-            for (int k = 0; k < ctx->S; ++k) buf[(size_t)k * N + v] = ctx->k.half_gamma0 * ctx->rng.gaussian(1.0);
+        for (long long v = 0; v < N; ++v) {
+            const int t = ctx->nmro ? ctx->mro_h[v] : 0;
+            if (t == 0) {
+                for (int k = 0; k < ctx->S; ++k) buf[(size_t)k * N + v] = ctx->k.half_gamma0 * ctx->rng.gaussian(1.0);
+                continue;
+            }
+            double xx, xy;
+            const int XM = mro_modes(t);
+            if (m == 0) {
+                const double amp = t == 2 ? 0.8 * g0 : 0.5 * g0;
+                xx = amp * std::cos(theta[v]);
+                xy = amp * std::sin(theta[v]);
+                first[2 * v] = xx;
+                first[2 * v + 1] = xy;
+            } else if (m < XM) {
+                const double a0 = first[2 * v], a2 = first[2 * v + 1];
+                const double th = m * 2. * STZ_PI_REF / XM * (1. + 0.2 * (2 * ctx->rng.uniform() - 1.));
+                double s2, c2;
+                sincos(2 * th, &s2, &c2);
+                const double u1 = ctx->rng.uniform();
+                const double u2 = ctx->rng.uniform();
+                xx = a0 * (1. + 0.2 * (2 * u1 - 1.)) * c2 + a2 * (1. + 0.2 * (2 * u2 - 1.)) * s2;
+                const double u3 = ctx->rng.uniform();
+                const double u4 = ctx->rng.uniform();
+                xy = -a0 * (1. + 0.2 * (2 * u3 - 1.)) * s2 + a2 * (1. + 0.2 * (2 * u4 - 1.)) * c2;
+            } else {
+                xx = 0.;
+                xy = 0.;
+            }
+            buf[v] = xx;
+            buf[(size_t)N + v] = xy;
+        }
         int rc = put_real(ctx, ctx->deps, buf.data(), (size_t)N * ctx->S, (size_t)m * ctx->S * N);
         if (rc) return rc;
         STZ_HIP(hipStreamSynchronize(ctx->stream));
     }
     for (int c = 0; c < 6; ++c) ctx->epsavg[c] = 0.0;
     return solve(ctx);   // STZ/bmg.c:117-127
+}
+
+}  // namespace
+
+// installs ctx->mro_h on the device (no crystal-like voxel: none, all glass)
+int stz_set_mro_map(stz_ctx *ctx)
+{
+    long long n = 0;
+    for (uint8_t t : ctx->mro_h) n += t != 0;
+    if (n && ctx->dim != 2) {
+        ctx->err = "crystal-like MRO families are defined for 2-D grids only";
+        return STZ_E_UNSUPPORTED;
+    }
+    if (ctx->mro) STZ_HIP(hipFree(ctx->mro));
+    ctx->mro = nullptr;
+    ctx->nmro = n;
+    if (!n) {
+        ctx->mro_h.clear();
+        return STZ_OK;
+    }
+    STZ_HIP(hipMalloc((void **)&ctx->mro, (size_t)ctx->N));
+    STZ_HIP(hipMemcpy(ctx->mro, ctx->mro_h.data(), (size_t)ctx->N, hipMemcpyHostToDevice));
+    ctx->acap = 0;              // avalanches: the host draws per voxel by type
+    return STZ_OK;
+}
+
+extern "C" {
+
+int stz_init_state(stz_ctx *ctx)
+{
+    if (!ctx) return STZ_E_ARG;
+    stz_rng_sync(ctx);
+    const long long N = ctx->N;
+    // MRO map, InitMRO 0 (STZ/init.c:350-397): one uniform per voxel; six-fold
+    // crystal-like MRO when it is <= xMROFrac, oriented by a second uniform
+    ctx->mro_h.assign((size_t)N, 0);
+    std::vector<double> theta((size_t)N, 0.0);
+    for (long long v = 0; v < N; ++v) {
+        const double u = ctx->rng.uniform();
+        if (u <= ctx->p.mro_fraction) {
+            ctx->mro_h[v] = 1;
+            theta[v] = ctx->rng.uniform() * STZ_PI_REF - STZ_PI_REF / 2.0;
+        }
+    }
+    int rc = stz_set_mro_map(ctx);
+    if (rc) return rc;
+    return init_fields(ctx, theta);
+}
+
+int stz_init_state_mro(stz_ctx *ctx, const int *type, const double *theta)
+{
+    if (!ctx || !type || !theta) return STZ_E_ARG;
+    stz_rng_sync(ctx);
+    const long long N = ctx->N;
+    ctx->mro_h.assign((size_t)N, 0);
+    std::vector<double> th(theta, theta + N);
+    for (long long v = 0; v < N; ++v) {
+        if (type[v] < 0 || type[v] > 3) {
+            ctx->err = "MRO type outside {0, 1, 2, 3}";      // STZ/elastic.c:131-136
+            return STZ_E_ARG;
+        }
+        ctx->mro_h[v] = (uint8_t)type[v];
+    }
+    // InitMRO 1 (STZ/init.c:420-428): ten-fold voxels become two-fold with probability 0.5
+    for (long long v = 0; v < N; ++v)
+        if (ctx->mro_h[v] == 3 && ctx->rng.uniform() < 0.5) ctx->mro_h[v] = 2;
+    int rc = stz_set_mro_map(ctx);
+    if (rc) return rc;
+    return init_fields(ctx, th);
 }
 
 int stz_init_state_device(stz_ctx *ctx, unsigned long long seed, const double *spheres, int nsph, double inc_mean,
@@ -658,6 +763,9 @@ int stz_download(stz_ctx *ctx, int field, double *host)
     case STZ_F_TEMP: STZ_HIP(hipMemcpyAsync(host, ctx->temp, n * 8, hipMemcpyDeviceToHost, ctx->stream)); break;
     case STZ_F_TLAST: STZ_HIP(hipMemcpyAsync(host, ctx->tlast, n * 8, hipMemcpyDeviceToHost, ctx->stream)); break;
     case STZ_F_TLOCAL: STZ_HIP(hipMemcpyAsync(host, ctx->tloc, n * 8, hipMemcpyDeviceToHost, ctx->stream)); break;
+    case STZ_F_MRO:
+        for (size_t v = 0; v < n; ++v) host[v] = ctx->nmro ? (double)ctx->mro_h[v] : 0.0;
+        return STZ_OK;
     default: return STZ_E_ARG;
     }
     STZ_HIP(hipStreamSynchronize(ctx->stream));
@@ -907,8 +1015,16 @@ int stz_step_kmc(stz_ctx *ctx, stz_step_result *res)
     // kind is known; afterwards the stream is set to exactly what the reference consumed.
     const StzRng s0 = ctx->rng;
     ctx->hspec[0] = ctx->rng.uniform();
+    const StzRng s_dice = ctx->rng;
     draw_voxel_deps(ctx, ctx->hspec + 1);
     const StzRng s_th = ctx->rng;
+    StzRng s_thx = s_th;
+    if (ctx->nmro) {
+        // a crystal-like site redraws its M barriers from M uniforms instead (STZ/elastic.c:446-506)
+        ctx->rng = s_dice;
+        for (int m = 0; m < ctx->M; ++m) ctx->hspec[1 + ms + m] = ctx->rng.uniform();
+        s_thx = ctx->rng;
+    }
     int rc = stz_rates_launch(ctx, 1);            // rates + group sums + step kind on the device
     if (rc) return rc;
     ctx->pend_comp = -1;
@@ -957,7 +1073,8 @@ int stz_step_kmc(stz_ctx *ctx, stz_step_result *res)
     }
     if (kind == KIND_THERMAL) {
         acc_select(ctx);
-        ctx->rng = s_th;
+        const int tsite = (int)ctx->hres[10];
+        ctx->rng = (ctx->nmro && ctx->mro_h[tsite]) ? s_thx : s_th;
         ctx->avlan = 0;                           // UpdateSOC (STZ/evolution.c:5-45)
         ctx->soc_elast = 0.0;
         const double t = 1 / R;                   // STZ/evolution.c:134
@@ -986,7 +1103,22 @@ int stz_step_kmc(stz_ctx *ctx, stz_step_result *res)
         } else {
             // more voxels than pre-drawn: transform the rest with the continuing stream
             ctx->hbuf.resize((size_t)n * ms);
-            for (long long i = 0; i < n; ++i) draw_voxel_deps(ctx, ctx->hbuf.data() + (size_t)i * ms);
+            if (ctx->nmro) {
+                // glass voxels draw M*S Gaussians, crystal-like ones M uniforms, in list order
+                std::vector<int> list((size_t)n);
+                STZ_HIP(hipMemcpyAsync(list.data(), ctx->alist, (size_t)n * sizeof(int), hipMemcpyDeviceToHost,
+                                       ctx->stream));
+                STZ_HIP(hipStreamSynchronize(ctx->stream));
+                for (long long i = 0; i < n; ++i) {
+                    double *slot = ctx->hbuf.data() + (size_t)i * ms;
+                    if (ctx->mro_h[list[i]])
+                        for (int m = 0; m < ctx->M; ++m) slot[m] = ctx->rng.uniform();
+                    else
+                        draw_voxel_deps(ctx, slot);
+                }
+            } else {
+                for (long long i = 0; i < n; ++i) draw_voxel_deps(ctx, ctx->hbuf.data() + (size_t)i * ms);
+            }
             const long long rest = n - ctx->acap;
             rc = ensure_dnew(ctx, (size_t)rest * ms);
             if (rc) return rc;
@@ -1010,7 +1142,7 @@ int stz_step_kmc(stz_ctx *ctx, stz_step_result *res)
         long long want = 2 * n;
         if (want < 8) want = 8;
         if (want > ctx->acap_max) want = ctx->acap_max;
-        if (want > ctx->acap) ctx->acap = want;
+        if (want > ctx->acap && !ctx->nmro) ctx->acap = want;
         res->kind = STZ_STEP_ATHERMAL;
         res->n_transformed = (int)n;
         res->soc_elast = soc;

@@ -7,7 +7,7 @@
 // per-voxel array in its storage precision (bit for bit), EpsAvg, <sigma>, the clock,
 // the pending uniform stress increment, the print/termination state, the counters and
 // the event-local update's bookkeeping, the local temperature and the heat-conduction
-// switch, and per-voxel moduli when they are set (the
+// switch, the crystal-like MRO map, and per-voxel moduli when they are set (the
 // reference keeps them fixed for a run, STZ/init.c:457-465).  Derived arrays (the Q0*exp(-D*S) cache, min Q,
 // block sums) are rebuilt by the next rate pass.  A run restarted from a checkpoint
 // continues exactly as the saved run does.
@@ -32,7 +32,7 @@ struct CkptHeader {
 struct CkptScalars {
     double epsavg[6], stress[6], soff[6], ebase[6], usum[6];
     double totime, pend_delta, strainhist_last, soc_elast, endstrain;
-    int64_t step, astep, kstep, estep, ntrans, nsolve, avlan, nlocal, rng_count;
+    int64_t step, astep, kstep, estep, ntrans, nsolve, avlan, nlocal, rng_count, nmro;
     int32_t pend_comp, printperiod, rng_pos, heat;
     uint32_t rng_state[624];
 };
@@ -131,8 +131,11 @@ int stz_checkpoint_save(stz_ctx *ctx, const char *path)
     s.printperiod = ctx->printperiod;
     s.rng_pos = ctx->rng.pos;
     s.heat = ctx->heat;
+    s.nmro = ctx->nmro;
     std::memcpy(s.rng_state, ctx->rng.state, sizeof(s.rng_state));
     if (std::fwrite(&h, sizeof(h), 1, f) != 1 || std::fwrite(&s, sizeof(s), 1, f) != 1)
+        return fail(ctx, f, "checkpoint write failed", STZ_E_ARG);
+    if (ctx->nmro && std::fwrite(ctx->mro_h.data(), 1, (size_t)ctx->N, f) != (size_t)ctx->N)
         return fail(ctx, f, "checkpoint write failed", STZ_E_ARG);
     std::vector<char> buf(kChunk);
     for (const Field &fd : fields_of(ctx)) {
@@ -166,6 +169,15 @@ int stz_checkpoint_load(stz_ctx *ctx, const char *path)
         s.pend_comp >= ctx->C)
         return fail(ctx, f, "truncated or corrupt checkpoint", STZ_E_ARG);
     if (hipStreamSynchronize(ctx->stream) != hipSuccess) return fail(ctx, f, "stream synchronisation failed", STZ_E_HIP);
+    // crystal-like MRO map (static after initialisation)
+    if (s.nmro < 0 || s.nmro > ctx->N) return fail(ctx, f, "truncated or corrupt checkpoint", STZ_E_ARG);
+    ctx->mro_h.assign((size_t)ctx->N, 0);
+    if (s.nmro && std::fread(ctx->mro_h.data(), 1, (size_t)ctx->N, f) != (size_t)ctx->N)
+        return fail(ctx, f, "truncated checkpoint", STZ_E_ARG);
+    {
+        const int rc = stz_set_mro_map(ctx);
+        if (rc) return fail(ctx, f, "cannot restore the MRO map", rc);
+    }
     std::vector<char> buf(kChunk);
     for (const Field &fd : fields_of(ctx)) {
         for (size_t off = 0; off < fd.bytes; off += kChunk) {

@@ -195,8 +195,11 @@ int main(int argc, char **argv)
     p.perm_cap = -std::log(0.6);
     p.temp_soft_barrier = 0.37;
     p.athermal_threshold = 0.33;
-    if (in.mrofrac != 0.0 || in.initmro != 0) {
-        std::fprintf(stderr, "crystal-like MRO (fraction %g, init %d) is not supported\n", in.mrofrac, in.initmro);
+    p.mro_fraction = in.mrofrac;
+    p.mro_q0 = in.xq0;
+    p.random_ratio = in.randomratio;
+    if (in.initmro != 0 && in.initmro != 1) {
+        std::fprintf(stderr, "Error: wrong option for MRO initialization\n");   // STZ/init.c:433-435
         return 1;
     }
 
@@ -206,7 +209,32 @@ int main(int argc, char **argv)
         std::fprintf(stderr, "stz_create failed (%d)\n", rc);
         return 1;
     }
-    rc = stz_init_state(ctx);
+    if (in.initmro == 1) {
+        // MROstructure.ms: one line "i j type theta" per voxel, 1-based, slow index first,
+        // in the voxel order (STZ/init.c:398-419)
+        const long long NV = (long long)p.n[0] * p.n[1] * p.n[2];
+        std::vector<int> type((size_t)NV);
+        std::vector<double> theta((size_t)NV);
+        FILE *fm = std::fopen("MROstructure.ms", "r");
+        if (!fm) {
+            std::perror("MROstructure.ms");
+            return 1;
+        }
+        char line[256];
+        for (long long m = 0; m < NV; ++m) {
+            int ii = 0, jj = 0;
+            if (!std::fgets(line, sizeof line, fm) ||
+                std::sscanf(line, "%d %d %d %lf", &ii, &jj, &type[(size_t)m], &theta[(size_t)m]) != 4 ||
+                (long long)(ii - 1) * p.n[0] + (jj - 1) != m) {
+                std::fprintf(stderr, "Error in reading MROstructure.ms file (inconsistent index system)!!\n");
+                return 1;
+            }
+        }
+        std::fclose(fm);
+        rc = stz_init_state_mro(ctx, type.data(), theta.data());
+    } else {
+        rc = stz_init_state(ctx);
+    }
     if (rc) {
         std::fprintf(stderr, "stz_init_state failed (%d): %s\n", rc, stz_last_error(ctx));
         return 1;
@@ -222,6 +250,19 @@ int main(int argc, char **argv)
     std::printf("PE#0:step=0\ttime=%lf\tstrain=[%.3e, %.3e, %.3e]\nstress=[%.3e, %.3e, %.3e]\n\n", totime, epsavg[0],
                 epsavg[1], epsavg[C - 1], stress[0], stress[1], stress[C - 1]);
 
+    // the MRO map: Initial_MROstructure_S000.iout, native int32 in voxel order
+    // (WriteMROMPI_INT, STZ/io.c:321-347, STZ/init.c:437), and the actQ.out type column
+    std::vector<double> mrod((size_t)N);
+    std::vector<int> mro((size_t)N);
+    stz_download(ctx, STZ_F_MRO, mrod.data());
+    for (long long v = 0; v < N; ++v) mro[(size_t)v] = (int)mrod[(size_t)v];
+    {
+        FILE *fo = std::fopen("Initial_MROstructure_S000.iout", "wb");
+        if (fo) {
+            std::fwrite(mro.data(), sizeof(int), (size_t)N, fo);
+            std::fclose(fo);
+        }
+    }
     FILE *fq = std::fopen("actQ.out", "w");
     std::vector<double> strainhist{0.0}, stresshist{0.0}, elhist{0.0};
     Soc soc;
@@ -254,7 +295,7 @@ int main(int argc, char **argv)
                 soc.update(prev_time);                      // UpdateSOC before KMC, STZ/bmg.c:167
                 if (r.kind == STZ_STEP_THERMAL) {
                     ++kstep;
-                    std::fprintf(fq, "%d\t%.6e\n", 0, r.q_chosen);   // MRO type (glass) and Q, STZ/evolution.c:243
+                    std::fprintf(fq, "%d\t%.6e\n", mro[(size_t)r.site], r.q_chosen);   // MRO type and Q, STZ/evolution.c:243
                 } else {
                     ++estep;
                 }

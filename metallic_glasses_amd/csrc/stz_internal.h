@@ -62,6 +62,9 @@
 // (Cp = 420 J/kg-K, rho = 6125 kg/m^3), thermal_diffusivity = 3e-6 m^2/s
 #define STZ_THERMAL_COEFF1 (1 / 420.0 / 6125.0)
 #define STZ_THERMAL_DIFFUSIVITY 3E-6
+// crystal-like MRO (six-fold) softening increments, PermSoftmro and TempSoftmro (STZ/init.c:61-63)
+#define STZ_PERM_SOFT_MRO 20.0
+#define STZ_TEMP_SOFT_MRO 60.0
 
 template <typename R>
 struct alignas(2 * sizeof(R)) cpx {
@@ -173,6 +176,9 @@ struct stz_ctx {
     void *deps_last = nullptr;    // [S][N] R
     double *perm = nullptr, *temp = nullptr, *tlast = nullptr;
     double *tloc = nullptr;       // [N] local temperature (Tlocal), FP64
+    uint8_t *mro = nullptr;       // [N] crystal-like MRO type (nullptr: glass everywhere)
+    std::vector<uint8_t> mro_h;   // host copy of the map (static after initialisation)
+    long long nmro = 0;           // crystal-like voxels
     int heat = 0;                 // HEAT_CONDUCT: diffuse tloc after thermal and elastic steps
     void *spec = nullptr;         // [C][n1*n2][hp] cpx<R>
     void *minq = nullptr;         // [N] R
@@ -486,6 +492,8 @@ int stz_solve_spectral(stz_ctx *ctx);   // y passes + fused Gamma-hat pass of ct
 int stz_inhom_fwd_x(stz_ctx *ctx, const double *src);   // R2C x pass of an eigenstrain [C][N], no round trip
 int stz_inhom_run(stz_ctx *ctx);        // the fixed-point loop; <sigma> in res[0..C), sigma written
 int stz_inhom_free(stz_ctx *ctx);
+// crystal-like MRO map (stz_api.hip): ctx->mro_h to the device, nmro, avalanche drawing
+int stz_set_mro_map(stz_ctx *ctx);
 // heat conduction (stz_heat.hip): one HeatConducting(hdt) of ctx->tloc, NaN flag in res[15]
 int stz_heat_launch(stz_ctx *ctx, double hdt);
 int stz_rates_launch(stz_ctx *ctx, int decide); // enqueues rates + group sums (+ step-kind decision)

@@ -28,6 +28,7 @@ struct RateArgs {
     const double *perm, *temp, *tlast;
     R *minq;
     uint8_t *argm;
+    const uint8_t *mro;        // crystal-like MRO types (nullptr: glass everywhere)
     double *bsum;
     int *acnt;                 // per voxel block: voxels meeting the athermal criterion
     unsigned long long *nneg;  // Q<0 counter (exact integer atomics)
@@ -85,8 +86,17 @@ struct VoxelState {
     R pd[6];
     R norm2;
     R soft;
+    R fac;          // crystal-like MRO: factor on the modifier and the softening per mode
+    int t;          // MRO type (0 glass)
     bool fresh;     // the cached Q0*exp(-D*S) of this voxel is valid for this step's S
 };
+
+// NO_SF_XTAL_MRO factors (STZ/init.c:645-667): six-fold 0.1, two-fold 0.01, ten-fold 0.9
+template <typename R>
+__device__ __forceinline__ R mro_factor(int t)
+{
+    return t == 1 ? (R)0.1 : (t == 2 ? (R)0.01 : (R)0.9);
+}
 
 // Softening S (STZ/init.c:731) and stress of one voxel.  Q0*exp(-D*S) (STZ/init.c:684)
 // changes only when S changes or the voxel transforms, so it is cached per (mode,
@@ -113,6 +123,8 @@ __device__ __forceinline__ void load_voxel(const RateArgs<R> &a, long long v, bo
         }
         s.sg[c] = x;
     }
+    s.t = a.mro ? (int)a.mro[v] : 0;
+    s.fac = s.t ? mro_factor<R>(s.t) : (R)1;
     s.fresh = (a.scache[v] == s.soft);       // false for the NaN of an invalidated entry
     if (!s.fresh) {
         R st[5];
@@ -138,10 +150,12 @@ __device__ __forceinline__ void load_mode(const RateArgs<R> &a, long long v, int
     for (int k = 0; k < S; ++k) d.st[k] = a.deps[((long long)m * S + k) * a.N + v];
 }
 
-// Q and rate of one mode at a voxel; returns Q0*exp(-D*S)
+// Q and rate of one mode at a voxel; returns Q0*exp(-D*S).  softm: the voxel's softening
+// for this mode (s.soft for glass; for crystal-like MRO s.soft times the factor once per
+// mode up to this one, the reference scaling Softening[idx] in place in its mode loop)
 template <typename R, int DIM>
-__device__ __forceinline__ R mode_q(const RateArgs<R> &a, const ModeData<R> &md, const VoxelState<R> &s, R &q,
-                                    double &rate)
+__device__ __forceinline__ R mode_q(const RateArgs<R> &a, const ModeData<R> &md, const VoxelState<R> &s, R softm,
+                                    R &q, double &rate)
 {
     constexpr int C = Comp<DIM>::C;
     R d[C];
@@ -155,7 +169,8 @@ __device__ __forceinline__ R mode_q(const RateArgs<R> &a, const ModeData<R> &md,
         const R norm1 = sqrt(contract<R, DIM>(d, d));                                      // :615-616
         R dm = dot / (norm1 + (R)STZ_EPSILON) / (s.norm2 + (R)STZ_EPSILON);                // :620
         dm = ((R)1 + dm) * ((R)1 + dm) / (R)4;                                             // :621
-        qe = md.q0 * exp(-dm * s.soft);
+        if (s.t) dm = dm * s.fac;                                                          // :647-658
+        qe = md.q0 * exp(-dm * softm);
     }
     q = qe - dwork / (R)2;                                                                 // :684
     if constexpr (sizeof(R) == 8) rate = a.ratepre * exp(-q / a.kT);                       // evolution.c:125
@@ -230,12 +245,14 @@ __device__ __forceinline__ void rates_block(const RateArgs<R> &a, long long blk)
     }
     R minq = (R)0;
     int am = 0, nneg = 0;
+    R softm = valid ? s.soft : (R)0;
     for (int m = 0; m < a.M; ++m) {
         double rate = 0.0;
         if (valid) {
             if (m + STZ_RATE_DEPTH < a.M) load_mode<R, DIM>(a, v, m + STZ_RATE_DEPTH, s.fresh, md[STZ_RATE_DEPTH]);
+            if (s.t) softm = softm * s.fac;
             R q;
-            const R qe = mode_q<R, DIM>(a, md[0], s, q, rate);
+            const R qe = mode_q<R, DIM>(a, md[0], s, softm, q, rate);
             if (!s.fresh) a.q0eff[(long long)m * a.N + v] = qe;
             if (m == 0 || minq > q) { minq = q; am = m; }     // first argmin (STZ/evolution.c:56-63)
             nneg += (q < (R)0);
@@ -349,6 +366,11 @@ struct ApplyArgs {
     double *tloc;                 // local temperature, heated by each transform
     const void *sigma;            // stress field (storage precision) + soff: the current stress
     double soff[6];
+    // crystal-like MRO voxels redraw their M barriers instead of their strains
+    void *q0;
+    const uint8_t *mro;           // nullptr: glass everywhere
+    double mro_q0, rratio;        // xQ0amp, RandomRatio
+    const double *unew;           // thermal event: M uniforms drawn for a crystal-like site
     long long N;
     int M;
     double permsoft, tempsoft, permcap, totime;
@@ -383,10 +405,11 @@ __device__ void transform_voxel_state(const ApplyArgs &a, long long v, int mm)
         a.tloc[v] += fabs(dh) * STZ_THERMAL_COEFF1;
     }
     const double ds = contract<double, DIM>(dd, dd);                                   // :385-386
-    double pv = a.perm[v] + ds * a.permsoft;                                          // :390
+    const bool six = a.mro && a.mro[v] == 1;
+    double pv = a.perm[v] + ds * (six ? STZ_PERM_SOFT_MRO : a.permsoft);               // :387-390
     if (pv > a.permcap) pv = a.permcap;                                               // :391
     a.perm[v] = pv;
-    a.temp[v] = ds * a.tempsoft;                                                      // :396
+    a.temp[v] = ds * (six ? STZ_TEMP_SOFT_MRO : a.tempsoft);                          // :393-396
     a.tlast[v] = a.totime;                                                            // :397
     ((R *)a.scache)[v] = (R)__builtin_nan("");                                        // Q0*exp(-D*S) must be recomputed
 }
@@ -397,6 +420,13 @@ __device__ void transform_voxel(const ApplyArgs &a, long long v, int mm, const d
     constexpr int S = Comp<DIM>::S;
     R *deps = (R *)a.deps;
     transform_voxel_state<R, DIM>(a, v, mm);
+    if (a.mro && a.mro[v]) {
+        // crystal-like MRO: M new barriers xQ0amp*(1 - RandomRatio + RandomRatio*U) from
+        // the first M values (uniforms), strains unchanged (STZ/elastic.c:446-506)
+        R *q0 = (R *)a.q0;
+        for (int m = 0; m < a.M; ++m) q0[(long long)m * a.N + v] = (R)(a.mro_q0 * (1 - a.rratio + a.rratio * (dnew[m])));
+        return;
+    }
     for (int m = 0; m < a.M; ++m)
 #pragma unroll
         for (int k = 0; k < S; ++k)
@@ -485,7 +515,10 @@ __device__ bool select_block(const RateArgs<R> &a, const double *gsum, long long
         ModeData<R> md;
         load_voxel<R, DIM>(a, v, false, s);
         load_mode<R, DIM>(a, v, m, s.fresh, md);
-        (void)mode_q<R, DIM>(a, md, s, q, rate);
+        R softm = s.soft;
+        if (s.t)
+            for (int j = 0; j <= m; ++j) softm = softm * s.fac;
+        (void)mode_q<R, DIM>(a, md, s, softm, q, rate);
     }
     const double incl0 = block_scan(rate, sh);
     const double P0 = base1 + incl0;
@@ -507,9 +540,18 @@ __device__ bool select_block(const RateArgs<R> &a, const double *gsum, long long
         __syncthreads();
         R *deps = (R *)ap.deps;
         const long long vv = blk * NT + f0;
-        for (int i = tid; i < a.M * S; i += NT) {
+        if (ap.mro && ap.mro[vv]) {
+            // crystal-like MRO site: M new barriers from the event's uniforms (STZ/elastic.c:446-506)
+            R *q0 = (R *)ap.q0;
+            for (int i = tid; i < a.M; i += NT) {
 #pragma clang fp contract(off)
-            deps[(long long)i * a.N + vv] = (R)(dscale * dnew[i]);
+                q0[(long long)i * a.N + vv] = (R)(ap.mro_q0 * (1 - ap.rratio + ap.rratio * (ap.unew[i])));
+            }
+        } else {
+            for (int i = tid; i < a.M * S; i += NT) {
+#pragma clang fp contract(off)
+                deps[(long long)i * a.N + vv] = (R)(dscale * dnew[i]);
+            }
         }
     }
     return true;

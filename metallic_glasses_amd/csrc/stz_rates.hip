@@ -256,6 +256,7 @@ RateArgs<R> rate_args(stz_ctx *ctx)
     a.temp = ctx->temp;
     a.tlast = ctx->tlast;
     a.minq = (R *)ctx->minq;
+    a.mro = ctx->mro;
     a.argm = ctx->argm;
     a.bsum = ctx->bsum;
     a.acnt = ctx->acnt;
@@ -292,6 +293,11 @@ ApplyArgs apply_args(stz_ctx *ctx)
     for (int c = 0; c < 6; ++c) a.soff[c] = ctx->soff[c];
     // FP64: a uniform increment not yet consumed by a rate pass is not in the field
     if (ctx->prec == STZ_FP64 && ctx->pend_comp >= 0) a.soff[ctx->pend_comp] += ctx->pend_delta;
+    a.q0 = ctx->q0;
+    a.mro = ctx->mro;
+    a.mro_q0 = ctx->p.mro_q0;
+    a.rratio = ctx->p.random_ratio;
+    a.unew = ctx->dspec_dev + 1 + (size_t)ctx->M * ctx->S;   // pinned: the thermal event's uniforms
     a.N = ctx->N;
     a.M = ctx->M;
     a.permsoft = ctx->k.permsoft;

@@ -54,6 +54,7 @@ __device__ __forceinline__ RateArgs<R> rate_args_of(const RepDev &d, const RepSt
     a.temp = d.temp;
     a.tlast = d.tlast;
     a.minq = (R *)d.minq;
+    a.mro = nullptr;                 // crystal-like MRO runs take the host path
     a.argm = d.argm;
     a.bsum = d.bsum;
     a.acnt = d.acnt;
@@ -88,6 +89,11 @@ __device__ __forceinline__ ApplyArgs apply_args_of(const RepDev &d, const RepSta
     a.tloc = d.tloc;
     a.sigma = d.sigma;
     for (int c = 0; c < 6; ++c) a.soff[c] = st->soff[c];
+    a.q0 = d.q0;
+    a.mro = nullptr;
+    a.mro_q0 = 0.0;
+    a.rratio = 0.0;
+    a.unew = nullptr;
     a.N = N;
     a.M = M;
     a.permsoft = d.permsoft;
@@ -1313,9 +1319,10 @@ int stz_run_steps(stz_ctx *ctx, int nsteps, stz_step_result *results, int *done)
 {
     if (!ctx || nsteps < 0) return STZ_E_ARG;
     int d = 0, rc = 0;
-    if (ctx->lamx || ctx->heat) {
-        // per-voxel moduli (the fixed-point loop ends on a host decision each iteration) or
-        // heat conduction: every step runs on the host path (same random numbers, same results)
+    if (ctx->lamx || ctx->heat || ctx->nmro) {
+        // per-voxel moduli (the fixed-point loop ends on a host decision each iteration), heat
+        // conduction, or crystal-like MRO voxels (their transforms draw uniforms instead of
+        // Gaussians): every step runs on the host path (same random numbers, same results)
         const bool stop_at_print = ctx->rs && ctx->rs->stop_at_print;
         for (int i = 0; i < nsteps; ++i) {
             stz_step_result r;
@@ -1363,8 +1370,8 @@ int stz_ensemble_create(stz_ctx *const *members, int n, stz_ensemble **out)
     for (int i = 0; i < n; ++i)
         if (!members[i]) return STZ_E_ARG;
     for (int i = 0; i < n; ++i)
-        if (members[i]->lamx || members[i]->heat) {
-            members[i]->err = "an ensemble steps uniform-moduli replicas without heat conduction only";
+        if (members[i]->lamx || members[i]->heat || members[i]->nmro) {
+            members[i]->err = "an ensemble steps glass replicas of uniform moduli without heat conduction only";
             return STZ_E_UNSUPPORTED;
         }
     *out = nullptr;
@@ -1404,8 +1411,8 @@ int stz_ensemble_run(stz_ensemble *e, int nsteps, stz_step_result *const *result
 {
     if (!e || nsteps < 0 || !done || !rcs) return STZ_E_ARG;
     for (stz_ctx *c : e->pl->all)
-        if (c->lamx || c->heat) {
-            c->err = "an ensemble steps uniform-moduli replicas without heat conduction only";
+        if (c->lamx || c->heat || c->nmro) {
+            c->err = "an ensemble steps glass replicas of uniform moduli without heat conduction only";
             return STZ_E_UNSUPPORTED;
         }
     return stz_plan_run(e->pl, nsteps, results, done, rcs, 0);

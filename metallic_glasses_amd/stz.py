@@ -17,7 +17,7 @@ LIB_PATH = os.path.join(PKG, "libstz.so")
 FP64, MIXED = 0, 1
 TENSION, SHEAR = 0, 1
 SOLVER_FFT, SOLVER_LOCAL = 0, 1
-F_EPS0, F_SIGMA, F_Q0, F_DEPS, F_DEPS_LAST, F_PERM, F_TEMP, F_TLAST, F_MINQ, F_TLOCAL = range(10)
+F_EPS0, F_SIGMA, F_Q0, F_DEPS, F_DEPS_LAST, F_PERM, F_TEMP, F_TLAST, F_MINQ, F_TLOCAL, F_MRO = range(11)
 ELASTIC, THERMAL, ATHERMAL = 0, 1, 2
 ERRORS = {-1: "selection failed", -2: "HIP error", -3: "bad argument", -4: "NaN in stress solve",
           -5: "unsupported (crystal-like MRO)"}
@@ -41,6 +41,7 @@ class Params(C.Structure):
         ("strainrate", C.c_double), ("strainincrement", C.c_double),
         ("perm_soft", C.c_double), ("temp_soft", C.c_double), ("perm_cap", C.c_double),
         ("temp_soft_barrier", C.c_double), ("athermal_threshold", C.c_double),
+        ("mro_fraction", C.c_double), ("mro_q0", C.c_double), ("random_ratio", C.c_double),
     ]
 
 
@@ -60,6 +61,7 @@ REFERENCE_2D = dict(
     gamma0=0.14, temperature=300.0, trial_frequency=1e13, lx=0.7, ly=0.7, lz=1.0,
     strainrate=1e-4, strainincrement=1e-4, perm_soft=13.0, temp_soft=39.0,
     perm_cap=float(-np.log(0.6)), temp_soft_barrier=0.37, athermal_threshold=0.33,
+    mro_fraction=0.0, mro_q0=2.0, random_ratio=0.2,
 )
 
 
@@ -140,6 +142,7 @@ def lib():
         L.stz_displacement.argtypes = [vp, dp]
         L.stz_checkpoint_load.argtypes = [vp, C.c_char_p]
         L.stz_set_moduli.argtypes = [vp, dp, dp]
+        L.stz_init_state_mro.argtypes = [vp, P(C.c_int), dp]
         L.stz_set_heat_conduct.argtypes = [vp, C.c_int]
         L.stz_heat_conduct.argtypes = [vp, C.c_double]
         L.stz_inhom_iterations.argtypes = [vp]
@@ -169,10 +172,11 @@ def _dp(a):
 class Stz:
     """One simulation context on the current HIP device."""
 
-    def __init__(self, init_state=True, device_init=None, **kw):
+    def __init__(self, init_state=True, device_init=None, mro_map=None, **kw):
         """init_state: draw the reference's MT19937 initial state on the host;
         device_init = dict(seed=..., spheres=[(x, y, z, r), ...], inc_mean=..., inc_std=...):
-        draw it on the GPU instead (counter-based, for large grids)."""
+        draw it on the GPU instead (counter-based, for large grids);
+        mro_map = (types[N], thetas[N]): the MRO map of an MROstructure.ms file (InitMRO 1)."""
         self.params = make_params(**kw)
         self._h = C.c_void_p()
         rc = lib().stz_create(C.byref(self.params), C.byref(self._h))
@@ -193,6 +197,12 @@ class Stz:
                 self._h, int(device_init.get("seed", p.seed)), _dp(sph) if len(sph) else None, len(sph),
                 float(device_init.get("inc_mean", 0.0)), float(device_init.get("inc_std", 0.0)), C.byref(cnt)))
             self.inclusion_voxels = cnt.value
+        elif mro_map is not None:
+            t = np.ascontiguousarray(mro_map[0], dtype=np.int32).reshape(-1)
+            th = np.ascontiguousarray(mro_map[1], dtype=np.float64).reshape(-1)
+            if t.size != self.N or th.size != self.N:
+                raise ValueError("the MRO map needs one type and one angle per voxel")
+            self._check(lib().stz_init_state_mro(self._h, t.ctypes.data_as(C.POINTER(C.c_int)), _dp(th)))
         elif init_state:
             self._check(lib().stz_init_state(self._h))
 

@@ -1,0 +1,144 @@
+"""GPU: crystal-like MRO (stz_params.mro_fraction / stz_init_state_mro) against the
+oracle, which the CPU tests pin against an independent MT19937 stream.
+
+Exact in FP64: the MRO map, the initial barriers and transformation strains of every
+family, the event sequence (site, mode, MRO type of the site), the barriers redrawn by
+crystal-like transforms; <sigma> within 1e-10.  Mixed precision: rates within 1e-5.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import pyoracle as po
+import metallic_glasses_amd as mg
+
+pytestmark = pytest.mark.gpu
+
+
+def _kw(n=(32, 32), **extra):
+    # tension, the reference input's loading: under 2-D shear the reference's athermal
+    # correction zeroes <sigma_xy> and its termination test |<sigma_xy>| < 1e-17 then
+    # compares rounding noise, which no two implementations reproduce alike
+    kw = dict(dim=2, n=(n[0], n[1], 1), modes=20, seed=21, loading=0, mro_fraction=0.3, mro_q0=1.5,
+              random_ratio=0.2)
+    kw.update(extra)
+    return kw
+
+
+def _file_map(N, seed=3):
+    rng = np.random.default_rng(seed)
+    return rng.integers(0, 4, N), rng.uniform(-np.pi / 2, np.pi / 2, N)
+
+
+@pytest.mark.parametrize("source", ["random", "file"])
+def test_initial_state_matches_oracle(source):
+    print("mro init", source, flush=True)
+    kw = _kw(n=(32, 16), modes=14)
+    N = 32 * 16
+    if source == "file":
+        mp = _file_map(N)
+        o = po.Oracle(mro_map=mp, **kw)
+        g = mg.Stz(mro_map=mp, **kw)
+    else:
+        o = po.Oracle(**kw)
+        g = mg.Stz(**kw)
+    t = o.mro_map()
+    assert np.array_equal(g.download(mg.F_MRO), t.astype(float))
+    assert (t > 0).sum() > 0.1 * N
+    assert np.array_equal(g.download(mg.F_Q0), o.field(po.F_Q0))
+    assert np.array_equal(g.download(mg.F_DEPS), o.field(po.F_DEPS))
+    ref = o.field(po.F_SIGMA)
+    scale = max(np.max(np.abs(ref)), 1e-30)
+    assert np.max(np.abs(g.download(mg.F_SIGMA) - ref)) <= 1e-10 * scale
+    # both streams stand at the same position: the next draws agree
+    assert [g.rng_uniform() for _ in range(3)] == [po.lib().ora_mt_uniform(o.rng()) for _ in range(3)]
+
+
+@pytest.mark.parametrize("precision", [mg.FP64, mg.MIXED])
+def test_rates_with_compounding_factors(precision):
+    kw = _kw(n=(32, 32), softening=1)
+    mp = _file_map(32 * 32, seed=8)
+    o = po.Oracle(mro_map=mp, **kw)
+    g = mg.Stz(mro_map=mp, **dict(kw, precision=precision))
+    rng = np.random.default_rng(2)
+    perm = rng.uniform(0.05, 0.5, o.N)
+    o.field(po.F_PERM)[:] = perm
+    g.upload(mg.F_PERM, perm)
+    o.softening()
+    nneg_o = o.act_energy()
+    R_o = o.total_rate()
+    nneg_g, R_g = g.rates()
+    assert nneg_g == nneg_o
+    tol = 1e-12 if precision == mg.FP64 else 1e-5
+    assert abs(R_g - R_o) <= tol * R_o
+    minq_o = o.field(po.F_Q).reshape(o.M, o.N).min(axis=0)
+    minq_g = g.download(mg.F_MINQ)
+    assert np.max(np.abs(minq_g - minq_o)) <= tol * np.max(np.abs(minq_o)) * 10
+
+
+@pytest.mark.parametrize("source,nsteps", [("random", 1500), ("file", 1200)])
+def test_trajectory_matches_oracle(source, nsteps):
+    print("mro trajectory", source, flush=True)
+    kw = _kw()
+    if source == "file":
+        mp = _file_map(32 * 32, seed=5)
+        o = po.Oracle(mro_map=mp, **kw)
+        g = mg.Stz(mro_map=mp, **kw)
+    else:
+        o = po.Oracle(**kw)
+        g = mg.Stz(**kw)
+    t = o.mro_map()
+    o.set_print(5, 1.0)
+    g.set_print(5, 1.0)
+    worst, hits = 0.0, 0
+    for i in range(nsteps):
+        rc_o, io = o.step()
+        rc_g, ig = g.step()
+        assert (io.kind, io.n_transformed) == (ig.kind, ig.n_transformed), i
+        if io.kind == 1:
+            assert (io.site, io.mode) == (ig.site, ig.mode), i
+            hits += t[io.site] > 0
+        so = np.array(io.stress[:6])
+        sg = np.array(ig.stress[:6])
+        worst = max(worst, np.max(np.abs(so - sg)) / max(np.max(np.abs(so)), 1e-3))
+        assert rc_o == rc_g
+        if rc_o:
+            break
+    print("  crystal events", hits, "worst %.2e" % worst, flush=True)
+    assert hits > 0
+    assert worst < 1e-10
+    assert np.array_equal(g.download(mg.F_Q0), o.field(po.F_Q0))
+    assert np.array_equal(g.download(mg.F_DEPS), o.field(po.F_DEPS))
+
+
+def test_run_steps_checkpoint_and_refusals(tmp_path):
+    kw = _kw(seed=33)
+    g1 = mg.Stz(**kw)
+    g2 = mg.Stz(**kw)
+    for g in (g1, g2):
+        g.set_print(5, 1.0)
+    seq1 = []
+    for _ in range(400):
+        rc, r = g1.step()
+        seq1.append((r.kind, r.site, r.mode, r.n_transformed))
+        if rc:
+            break
+    rc2, res2 = g2.run_steps(len(seq1))
+    assert seq1 == [(r.kind, r.site, r.mode, r.n_transformed) for r in res2]
+    assert np.array_equal(g1.download(mg.F_Q0), g2.download(mg.F_Q0))
+    path = os.path.join(str(tmp_path), "mro.ckpt")
+    g1.checkpoint_save(path)
+    a = [g1.step()[1] for _ in range(200)]
+    h = mg.Stz(init_state=False, **kw)
+    h.checkpoint_load(path)
+    assert np.array_equal(h.download(mg.F_MRO), g1.download(mg.F_MRO))
+    b = [h.step()[1] for _ in range(200)]
+    assert [(r.kind, r.site, r.mode) for r in a] == [(r.kind, r.site, r.mode) for r in b]
+    assert np.array_equal(g1.download(mg.F_Q0), h.download(mg.F_Q0))
+    with pytest.raises(mg.StzError) as e:
+        mg.Ensemble([g2])
+    assert e.value.code == -5
+    with pytest.raises(mg.StzError) as e:
+        mg.Stz(dim=3, n=(8, 8, 8), E=94.5, nu=0.35, double_nu=0, gamma0=0.1, mro_fraction=0.5)
+    assert e.value.code == -5
