@@ -103,7 +103,7 @@ __device__ __forceinline__ R mro_factor(int t)
 // voxel) with the S it was computed for; S equals Perm bit for bit except shortly after
 // a transform, so almost every voxel takes the cached value, which is the bit-identical
 // result of the same expression.
-template <typename R, int DIM>
+template <typename R, int DIM, bool XTAL = false>
 __device__ __forceinline__ void load_voxel(const RateArgs<R> &a, long long v, bool apply_pending, VoxelState<R> &s)
 {
     constexpr int C = Comp<DIM>::C, S = Comp<DIM>::S;
@@ -123,8 +123,13 @@ __device__ __forceinline__ void load_voxel(const RateArgs<R> &a, long long v, bo
         }
         s.sg[c] = x;
     }
-    s.t = a.mro ? (int)a.mro[v] : 0;
-    s.fac = s.t ? mro_factor<R>(s.t) : (R)1;
+    if constexpr (XTAL) {
+        s.t = (int)a.mro[v];
+        s.fac = s.t ? mro_factor<R>(s.t) : (R)1;
+    } else {
+        s.t = 0;
+        s.fac = (R)1;
+    }
     s.fresh = (a.scache[v] == s.soft);       // false for the NaN of an invalidated entry
     if (!s.fresh) {
         R st[5];
@@ -153,7 +158,7 @@ __device__ __forceinline__ void load_mode(const RateArgs<R> &a, long long v, int
 // Q and rate of one mode at a voxel; returns Q0*exp(-D*S).  softm: the voxel's softening
 // for this mode (s.soft for glass; for crystal-like MRO s.soft times the factor once per
 // mode up to this one, the reference scaling Softening[idx] in place in its mode loop)
-template <typename R, int DIM>
+template <typename R, int DIM, bool XTAL = false>
 __device__ __forceinline__ R mode_q(const RateArgs<R> &a, const ModeData<R> &md, const VoxelState<R> &s, R softm,
                                     R &q, double &rate)
 {
@@ -169,7 +174,8 @@ __device__ __forceinline__ R mode_q(const RateArgs<R> &a, const ModeData<R> &md,
         const R norm1 = sqrt(contract<R, DIM>(d, d));                                      // :615-616
         R dm = dot / (norm1 + (R)STZ_EPSILON) / (s.norm2 + (R)STZ_EPSILON);                // :620
         dm = ((R)1 + dm) * ((R)1 + dm) / (R)4;                                             // :621
-        if (s.t) dm = dm * s.fac;                                                          // :647-658
+        if constexpr (XTAL)
+            if (s.t) dm = dm * s.fac;                                                      // :647-658
         qe = md.q0 * exp(-dm * softm);
     }
     q = qe - dwork / (R)2;                                                                 // :684
@@ -224,9 +230,10 @@ __device__ __forceinline__ double block_scan(double x, double *sh)
 // ---------------------------------------------------------------------------------
 // rate pass over one block of NT voxels (all modes): block rate sums per mode, the
 // per-voxel minimum Q and its first argmin, the athermal flag count of the block, the
-// Q<0 count, and the Q0*exp(-D*S) cache refresh
+// Q<0 count, and the Q0*exp(-D*S) cache refresh.  XTAL: the context has crystal-like
+// MRO voxels (a.mro set); glass contexts compile without the per-voxel type.
 // ---------------------------------------------------------------------------------
-template <typename R, int DIM>
+template <typename R, int DIM, bool XTAL = false>
 __device__ __forceinline__ void rates_block(const RateArgs<R> &a, long long blk)
 {
     __shared__ double wsum[STZ_MAXM][NW];
@@ -238,7 +245,7 @@ __device__ __forceinline__ void rates_block(const RateArgs<R> &a, long long blk)
     // modes are loaded STZ_RATE_DEPTH ahead of the one being evaluated
     ModeData<R> md[STZ_RATE_DEPTH + 1];
     if (valid) {
-        load_voxel<R, DIM>(a, v, true, s);
+        load_voxel<R, DIM, XTAL>(a, v, true, s);
 #pragma unroll
         for (int j = 0; j < STZ_RATE_DEPTH; ++j)
             if (j < a.M) load_mode<R, DIM>(a, v, j, s.fresh, md[j]);
@@ -250,9 +257,10 @@ __device__ __forceinline__ void rates_block(const RateArgs<R> &a, long long blk)
         double rate = 0.0;
         if (valid) {
             if (m + STZ_RATE_DEPTH < a.M) load_mode<R, DIM>(a, v, m + STZ_RATE_DEPTH, s.fresh, md[STZ_RATE_DEPTH]);
-            if (s.t) softm = softm * s.fac;
+            if constexpr (XTAL)
+                if (s.t) softm = softm * s.fac;
             R q;
-            const R qe = mode_q<R, DIM>(a, md[0], s, softm, q, rate);
+            const R qe = mode_q<R, DIM, XTAL>(a, md[0], s, softm, q, rate);
             if (!s.fresh) a.q0eff[(long long)m * a.N + v] = qe;
             if (m == 0 || minq > q) { minq = q; am = m; }     // first argmin (STZ/evolution.c:56-63)
             nneg += (q < (R)0);
@@ -454,7 +462,7 @@ __device__ __forceinline__ int first_true(bool f, int *sh_min)
 // level re-evaluating the NT rates of one block with the rate pass's own code.  Writes
 // site, mode, Q to res[10..12] and, when dnew is given, transforms the chosen voxel.
 // Returns false (res[10] = -1) when tau lies beyond the total.
-template <typename R, int DIM>
+template <typename R, int DIM, bool XTAL = false>
 __device__ bool select_block(const RateArgs<R> &a, const double *gsum, long long ng, double tau, double *res,
                              const ApplyArgs &ap, const double *dnew, double dscale)
 {
@@ -513,12 +521,13 @@ __device__ bool select_block(const RateArgs<R> &a, const double *gsum, long long
     if (v < a.N) {
         VoxelState<R> s;
         ModeData<R> md;
-        load_voxel<R, DIM>(a, v, false, s);
+        load_voxel<R, DIM, XTAL>(a, v, false, s);
         load_mode<R, DIM>(a, v, m, s.fresh, md);
         R softm = s.soft;
-        if (s.t)
-            for (int j = 0; j <= m; ++j) softm = softm * s.fac;
-        (void)mode_q<R, DIM>(a, md, s, softm, q, rate);
+        if constexpr (XTAL)
+            if (s.t)
+                for (int j = 0; j <= m; ++j) softm = softm * s.fac;
+        (void)mode_q<R, DIM, XTAL>(a, md, s, softm, q, rate);
     }
     const double incl0 = block_scan(rate, sh);
     const double P0 = base1 + incl0;

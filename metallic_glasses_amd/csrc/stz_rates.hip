@@ -19,10 +19,10 @@
 
 namespace {
 
-template <typename R, int DIM>
+template <typename R, int DIM, bool XTAL>
 __global__ __launch_bounds__(NT) void k_rates(RateArgs<R> a)
 {
-    rates_block<R, DIM>(a, blockIdx.x);
+    rates_block<R, DIM, XTAL>(a, blockIdx.x);
 }
 
 // Fixed-tree sums of STZ_GRP consecutive block sums, STZ_GRP_PER_BLOCK groups per workgroup
@@ -73,7 +73,7 @@ __global__ __launch_bounds__(NT) void k_decide(const double *__restrict__ gsum, 
     }
 }
 
-template <typename R, int DIM>
+template <typename R, int DIM, bool XTAL>
 __global__ __launch_bounds__(NT) void k_select(RateArgs<R> a, const double *__restrict__ gsum, long long ng, double tau,
                                                double *__restrict__ res, int gated, ApplyArgs ap,
                                                const double *__restrict__ dnew)
@@ -82,7 +82,7 @@ __global__ __launch_bounds__(NT) void k_select(RateArgs<R> a, const double *__re
         if (reinterpret_cast<const int *>(res + RES_KIND)[0] != KIND_THERMAL) return;
         tau = res[RES_TAU];
     }
-    const bool ok = select_block<R, DIM>(a, gsum, ng, tau, res, ap, dnew, 1.0);
+    const bool ok = select_block<R, DIM, XTAL>(a, gsum, ng, tau, res, ap, dnew, 1.0);
     if (!ok && threadIdx.x == 0) reinterpret_cast<int *>(res + RES_KIND)[0] = KIND_FAIL;
 }
 
@@ -312,7 +312,10 @@ int rates_impl(stz_ctx *ctx, int decide)
 {
     RateArgs<R> a = rate_args<R>(ctx);
     tmark(ctx, 8);
-    hipLaunchKernelGGL((k_rates<R, DIM>), dim3((unsigned)ctx->nblk), dim3(NT), 0, ctx->stream, a);
+    if (ctx->mro)
+        hipLaunchKernelGGL((k_rates<R, DIM, true>), dim3((unsigned)ctx->nblk), dim3(NT), 0, ctx->stream, a);
+    else
+        hipLaunchKernelGGL((k_rates<R, DIM, false>), dim3((unsigned)ctx->nblk), dim3(NT), 0, ctx->stream, a);
     STZ_CHECK_LAUNCH();
     tmark(ctx, 9);
     const long long nb = (long long)ctx->M * ctx->nblk;
@@ -332,8 +335,12 @@ int select_impl(stz_ctx *ctx, double tau, int gated)
     a.pend_comp = -1;
     tmark(ctx, 11);
     const double *dnew = gated ? ctx->dspec_dev + 1 : nullptr;
-    hipLaunchKernelGGL((k_select<R, DIM>), dim3(1), dim3(NT), 0, ctx->stream, a, ctx->gsum, ctx->ngrp, tau, ctx->res,
-                       gated, apply_args(ctx), dnew);
+    if (ctx->mro)
+        hipLaunchKernelGGL((k_select<R, DIM, true>), dim3(1), dim3(NT), 0, ctx->stream, a, ctx->gsum, ctx->ngrp, tau,
+                           ctx->res, gated, apply_args(ctx), dnew);
+    else
+        hipLaunchKernelGGL((k_select<R, DIM, false>), dim3(1), dim3(NT), 0, ctx->stream, a, ctx->gsum, ctx->ngrp, tau,
+                           ctx->res, gated, apply_args(ctx), dnew);
     STZ_CHECK_LAUNCH();
     tmark(ctx, 12);
     return STZ_OK;

@@ -56,20 +56,20 @@ Residual stress data file:
 NullSpaceBasis_128x128_Nullity254.out
 
 Volume fraction of crystal-like MRO:
-0.0
+{frac}
 
 Initialization condition for MRO:
-0
+{init}
 
 Stress-free energy barrier for crystal-like MRO:
-2.0
+{xq0}
 
 """
 
 
 def test_driver_reproduces_oracle_outputs(tmp_path):
     ext = "Grid:\n32 32 1\n\nLoading:\nshear\n\n"
-    (tmp_path / "input").write_text(INPUT.format(end="0.03") + ext)
+    (tmp_path / "input").write_text(INPUT.format(end="0.03", frac="0.0", init="0", xq0="2.0") + ext)
     r = subprocess.run([DRIVER], cwd=tmp_path, capture_output=True, text=True, timeout=240)
     assert r.returncode == 0, r.stderr[-2000:]
     q = np.loadtxt(tmp_path / "actQ.out", ndmin=2)
@@ -105,3 +105,40 @@ def test_driver_reproduces_oracle_outputs(tmp_path):
         x = np.loadtxt(tmp_path / name)
         assert x.shape == (32, 32)
         assert np.max(np.abs(x.ravel() - u1[a * 1024:(a + 1) * 1024])) <= 1.5e-6, name
+
+
+@pytest.mark.parametrize("init", [0, 1])
+def test_driver_crystal_like_mro(tmp_path, init):
+    """InitMRO 0 (random map) and 1 (MROstructure.ms): the initial map file, and actQ.out's
+    MRO type and Q of every thermal event, against the oracle (tension, 32 x 32)."""
+    ext = "Grid:\n32 32 1\n\n"
+    (tmp_path / "input").write_text(INPUT.format(end="0.06", frac="0.3", init=str(init), xq0="1.5") + ext)
+    kw = dict(dim=2, n=(32, 32, 1), seed=123, mro_fraction=0.3, mro_q0=1.5, random_ratio=0.2)
+    if init == 1:
+        rng = np.random.default_rng(9)
+        types = rng.integers(0, 4, 1024)
+        theta = rng.uniform(-1.5, 1.5, 1024)
+        with open(tmp_path / "MROstructure.ms", "w") as f:
+            for v in range(1024):
+                f.write("%d %d %d %.17g\n" % (v // 32 + 1, v % 32 + 1, types[v], theta[v]))
+        o = po.Oracle(mro_map=(types, theta), **kw)
+    else:
+        o = po.Oracle(**kw)
+    r = subprocess.run([DRIVER], cwd=tmp_path, capture_output=True, text=True, timeout=240)
+    assert r.returncode == 0, r.stderr[-2000:]
+    mro = np.fromfile(tmp_path / "Initial_MROstructure_S000.iout", dtype=np.int32)
+    assert np.array_equal(mro, o.mro_map().astype(np.int32))
+    assert (mro > 0).sum() > 100
+    q = np.loadtxt(tmp_path / "actQ.out", ndmin=2)
+    o.set_print(5, 0.06)
+    rows = []
+    while True:
+        rc, info = o.step()
+        if info.kind == 1:
+            rows.append((info.mro, info.q_chosen))
+        if rc:
+            break
+    rows = np.array(rows)
+    assert len(rows) == len(q) and len(rows) > 100 and (rows[:, 0] > 0).sum() >= 3
+    assert np.array_equal(q[:, 0], rows[:, 0])
+    assert np.allclose(q[:, 1], rows[:, 1], rtol=1e-6, atol=0)
