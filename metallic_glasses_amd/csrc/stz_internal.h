@@ -66,6 +66,19 @@
 #define STZ_PERM_SOFT_MRO 20.0
 #define STZ_TEMP_SOFT_MRO 60.0
 
+#ifndef STZ_NT_LOADS
+#define STZ_NT_LOADS 1
+#endif
+// Operands read once per step are loaded non-temporally, so the streams do not evict what
+// the step reuses (bit 0: per-mode operands of the rate pass, bit 1: its per-voxel
+// operands, bit 2: the local update's response table and stress field)
+template <int BIT, typename T>
+__device__ __forceinline__ T ld_stream(const T *p)
+{
+    if constexpr ((STZ_NT_LOADS & BIT) != 0) return __builtin_nontemporal_load(p);
+    else return *p;
+}
+
 template <typename R>
 struct alignas(2 * sizeof(R)) cpx {
     R x, y;

@@ -108,11 +108,12 @@ __device__ __forceinline__ void load_voxel(const RateArgs<R> &a, long long v, bo
 {
     constexpr int C = Comp<DIM>::C, S = Comp<DIM>::S;
     double soft = 0.0;
-    if (a.softening) soft = a.perm[v] + a.temp[v] * exp((a.tlast[v] - a.totime) / a.tTemp);   // :731
+    if (a.softening)
+        soft = ld_stream<2>(a.perm + v) + ld_stream<2>(a.temp + v) * exp((ld_stream<2>(a.tlast + v) - a.totime) / a.tTemp);   // :731
     s.soft = (R)soft;
 #pragma unroll
     for (int c = 0; c < C; ++c) {
-        R x = a.sigma[(long long)c * a.N + v];
+        R x = ld_stream<2>(a.sigma + (long long)c * a.N + v);
         if constexpr (sizeof(R) == 8) {
             if (apply_pending && c == a.pend_comp) {
                 x = x + (R)a.pend_delta;      // Sigma[c][idx] += delta (STZ/evolution.c:311-314)
@@ -130,11 +131,11 @@ __device__ __forceinline__ void load_voxel(const RateArgs<R> &a, long long v, bo
         s.t = 0;
         s.fac = (R)1;
     }
-    s.fresh = (a.scache[v] == s.soft);       // false for the NaN of an invalidated entry
+    s.fresh = (ld_stream<2>(a.scache + v) == s.soft);   // false for the NaN of an invalidated entry
     if (!s.fresh) {
         R st[5];
 #pragma unroll
-        for (int k = 0; k < S; ++k) st[k] = a.deps_last[(long long)k * a.N + v];
+        for (int k = 0; k < S; ++k) st[k] = ld_stream<2>(a.deps_last + (long long)k * a.N + v);
         expand<R, DIM>(st, s.pd);
         s.norm2 = sqrt(contract<R, DIM>(s.pd, s.pd));                    // :617-618
     }
@@ -150,9 +151,9 @@ template <typename R, int DIM>
 __device__ __forceinline__ void load_mode(const RateArgs<R> &a, long long v, int m, bool fresh, ModeData<R> &d)
 {
     constexpr int S = Comp<DIM>::S;
-    d.q0 = fresh ? a.q0eff[(long long)m * a.N + v] : a.q0[(long long)m * a.N + v];
+    d.q0 = fresh ? ld_stream<1>(a.q0eff + (long long)m * a.N + v) : ld_stream<1>(a.q0 + (long long)m * a.N + v);
 #pragma unroll
-    for (int k = 0; k < S; ++k) d.st[k] = a.deps[((long long)m * S + k) * a.N + v];
+    for (int k = 0; k < S; ++k) d.st[k] = ld_stream<1>(a.deps + ((long long)m * S + k) * a.N + v);
 }
 
 // Q and rate of one mode at a voxel; returns Q0*exp(-D*S).  softm: the voxel's softening

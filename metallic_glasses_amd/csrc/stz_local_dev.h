@@ -125,7 +125,7 @@ __device__ __forceinline__ void local_store(const LocalGeom &g, R *sigma, int v,
 {
 #pragma unroll
     for (int c = 0; c < C; ++c) {
-        const double s = ((double)sigma[(long long)c * g.N + v] + uni[c]) + (double)ds[c];
+        const double s = ((double)ld_stream<4>(sigma + (long long)c * g.N + v) + uni[c]) + (double)ds[c];
         bad |= (s != s);
         sigma[(long long)c * g.N + v] = (R)s;
         acc[c] += s;
@@ -201,7 +201,7 @@ __device__ void local_rows(const LocalGeom &g, const R *__restrict__ K, const Lo
             for (int k = 0; k < S; ++k) {
                 const R dk = s0.d[k];
 #pragma unroll
-                for (int c = 0; c < C; ++c) da[c] = da[c] + dk * K[((long long)k * C + c) * g.H + hj];
+                for (int c = 0; c < C; ++c) da[c] = da[c] + dk * ld_stream<4>(K + ((long long)k * C + c) * g.H + hj);
             }
 #pragma unroll
             for (int c = 0; c < C; ++c) db[c] = da[c];
