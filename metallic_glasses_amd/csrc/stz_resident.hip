@@ -264,8 +264,15 @@ __global__ __launch_bounds__(NT_LOCAL) void kb_local(const RepDev *__restrict__ 
                        reinterpret_cast<int *>(res + 15), wred);
 }
 
+// Workgroups per replica of kb_athermal: most steps have no avalanche and every
+// workgroup returns at once, so the grid is capped and each workgroup strides over the
+// voxel blocks (an empty launch over 16 replicas of 128^3 costs ~20 us at one workgroup
+// per block, ~3 us capped).
+constexpr int ATH_GRID = 512;
+
 template <typename R, int DIM>
-__global__ __launch_bounds__(NT) void kb_athermal(const RepDev *__restrict__ reps, long long N, int M)
+__global__ __launch_bounds__(NT) void kb_athermal(const RepDev *__restrict__ reps, long long N, int M,
+                                                  long long nblk)
 {
     const RepDev &d = reps[blockIdx.y];
     const RepState *st = d.st;
@@ -287,7 +294,10 @@ __global__ __launch_bounds__(NT) void kb_athermal(const RepDev *__restrict__ rep
     a.cap = 1ll << 62;
     a.N = N;
     a.socp = d.socp;
-    athermal_block<R, DIM>(a, blockIdx.x);
+    for (long long blk = blockIdx.x; blk < nblk; blk += gridDim.x) {
+        athermal_block<R, DIM>(a, blk);
+        __syncthreads();              // the block's shared scratch is reused by the next one
+    }
 }
 
 // --- host-loop arithmetic of the reference, on the device, with the host's operation order
@@ -569,6 +579,10 @@ struct StzClass {
     bool fft = true;                  // some replica takes the full solve on the device
     RepDev *d_reps = nullptr;
     SolveRep *d_solve = nullptr;
+    // plans of several classes: the class's own stream (default priority) and the event
+    // that joins it back into the plan's stream after each step
+    hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr;
 };
 
 }  // namespace
@@ -580,6 +594,7 @@ struct StzPlan {
     stz_ctx *owner = nullptr;         // context whose error string receives messages
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    hipEvent_t fork = nullptr;        // several classes: their steps run concurrently
     int batch = 64;
     int use_graph = 0;                // steps per captured graph; 0: direct launches
     int graph_steps = 0;              // steps in the instantiated graph
@@ -844,14 +859,15 @@ int launch_class(StzPlan *pl, StzClass &k, hipEvent_t ea, hipEvent_t eb)
 {
     stz_ctx *p = k.reps[0];
     const int nrep = (int)k.reps.size();
-    hipStream_t s = pl->stream;
+    hipStream_t s = k.stream ? k.stream : pl->stream;
     if (ea) (void)hipEventRecord(ea, s);
     hipLaunchKernelGGL((kb_rates<R, DIM>), dim3((unsigned)p->nblk, nrep), dim3(NT), 0, s, k.d_reps, p->N, p->nblk, p->M);
     if (eb) (void)hipEventRecord(eb, s);
     const unsigned gb = (unsigned)((p->ngrp + STZ_GRP_PER_BLOCK - 1) / STZ_GRP_PER_BLOCK);
     hipLaunchKernelGGL(kb_group, dim3(gb, nrep), dim3(NT), 0, s, k.d_reps, (long long)p->M * p->nblk, p->ngrp);
     hipLaunchKernelGGL((kb_decide<R, DIM>), dim3(nrep), dim3(NT), 0, s, k.d_reps, p->N, p->nblk, p->ngrp, p->M);
-    hipLaunchKernelGGL((kb_athermal<R, DIM>), dim3((unsigned)p->nblk, nrep), dim3(NT), 0, s, k.d_reps, p->N, p->M);
+    hipLaunchKernelGGL((kb_athermal<R, DIM>), dim3((unsigned)std::min<long long>(p->nblk, ATH_GRID), nrep), dim3(NT), 0,
+                       s, k.d_reps, p->N, p->M, p->nblk);
     if (hipGetLastError() != hipSuccess) return STZ_E_HIP;
     int nparts = 0;
     if (k.fft) {            // replicas whose every transform is followed by the full solve
@@ -870,8 +886,17 @@ int launch_class(StzPlan *pl, StzClass &k, hipEvent_t ea, hipEvent_t eb)
     return hipGetLastError() == hipSuccess ? STZ_OK : STZ_E_HIP;
 }
 
+// One step of every class.  With several classes each runs on its own stream between a
+// fork and a join on the plan's stream: the classes share no memory, and the short
+// latency-bound kernels of one class overlap the bandwidth-bound passes of another
+// (a captured graph holds them as parallel branches).
 int launch_step(StzPlan *pl, hipEvent_t ea = nullptr, hipEvent_t eb = nullptr)
 {
+    if (pl->fork) {
+        if (hipEventRecord(pl->fork, pl->stream) != hipSuccess) return STZ_E_HIP;
+        for (StzClass &k : pl->cls)
+            if (hipStreamWaitEvent(k.stream, pl->fork, 0) != hipSuccess) return STZ_E_HIP;
+    }
     for (StzClass &k : pl->cls) {
         stz_ctx *p = k.reps[0];
         int rc;
@@ -880,6 +905,12 @@ int launch_step(StzPlan *pl, hipEvent_t ea = nullptr, hipEvent_t eb = nullptr)
         else
             rc = p->dim == 2 ? launch_class<float, 2>(pl, k, ea, eb) : launch_class<float, 3>(pl, k, ea, eb);
         if (rc) return rc;
+    }
+    if (pl->fork) {
+        for (StzClass &k : pl->cls) {
+            if (hipEventRecord(k.done, k.stream) != hipSuccess) return STZ_E_HIP;
+            if (hipStreamWaitEvent(pl->stream, k.done, 0) != hipSuccess) return STZ_E_HIP;
+        }
     }
     return STZ_OK;
 }
@@ -892,10 +923,15 @@ void plan_free(StzPlan *pl)
     for (hipEvent_t e : pl->evb) (void)hipEventDestroy(e);
     if (pl->exec) (void)hipGraphExecDestroy(pl->exec);
     if (pl->graph) (void)hipGraphDestroy(pl->graph);
+    if (pl->stream) (void)hipStreamSynchronize(pl->stream);
     for (StzClass &k : pl->cls) {
+        if (k.stream) (void)hipStreamSynchronize(k.stream);
         if (k.d_reps) (void)hipFree(k.d_reps);
         if (k.d_solve) (void)hipFree(k.d_solve);
+        if (k.done) (void)hipEventDestroy(k.done);
+        if (k.stream) (void)hipStreamDestroy(k.stream);
     }
+    if (pl->fork) (void)hipEventDestroy(pl->fork);
     delete pl;
 }
 
@@ -926,6 +962,17 @@ int plan_make(stz_ctx *const *ctxs, int n, StzPlan **out)
     for (StzClass &k : pl->cls) {
         const size_t nr = k.reps.size();
         if (hipMalloc(&k.d_reps, nr * sizeof(RepDev)) != hipSuccess || hipMalloc(&k.d_solve, nr * sizeof(SolveRep)) != hipSuccess) {
+            plan_free(pl);
+            return STZ_E_HIP;
+        }
+    }
+    if (pl->cls.size() > 1) {
+        bool ok = hipEventCreateWithFlags(&pl->fork, hipEventDisableTiming) == hipSuccess;
+        for (StzClass &k : pl->cls) {
+            ok = ok && hipStreamCreateWithFlags(&k.stream, hipStreamNonBlocking) == hipSuccess;
+            ok = ok && hipEventCreateWithFlags(&k.done, hipEventDisableTiming) == hipSuccess;
+        }
+        if (!ok) {
             plan_free(pl);
             return STZ_E_HIP;
         }
