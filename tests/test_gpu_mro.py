@@ -142,3 +142,46 @@ def test_run_steps_checkpoint_and_refusals(tmp_path):
     with pytest.raises(mg.StzError) as e:
         mg.Stz(dim=3, n=(8, 8, 8), E=94.5, nu=0.35, double_nu=0, gamma0=0.1, mro_fraction=0.5)
     assert e.value.code == -5
+
+
+def test_mro_with_moduli_and_heat_matches_oracle():
+    """Every host-path option at once: crystal-like MRO, per-voxel moduli (the fixed-point
+    solve) and heat conduction, against the oracle with the same options."""
+    print("mro + moduli + heat", flush=True)
+    kw = _kw(seed=44)
+    o = po.Oracle(**kw)
+    g = mg.Stz(**kw)
+    rng = np.random.default_rng(12)
+    lam0, mu0 = o.consts[3], o.consts[2]
+    lam = lam0 * (1 + 0.2 * rng.uniform(-1, 1, o.N))
+    mu = mu0 * (1 + 0.2 * rng.uniform(-1, 1, o.N))
+    o.set_moduli(lam, mu)
+    g.set_moduli(lam, mu)
+    o.solve()
+    g.stress_solve()
+    o.set_heat(True)
+    g.set_heat_conduct(True)
+    o.set_print(5, 1.0)
+    g.set_print(5, 1.0)
+    t = o.mro_map()
+    worst, hits, events = 0.0, 0, 0
+    for i in range(1000):
+        rc_o, io = o.step()
+        rc_g, ig = g.step()
+        assert (io.kind, io.n_transformed) == (ig.kind, ig.n_transformed), i
+        if io.kind == 1:
+            assert (io.site, io.mode) == (ig.site, ig.mode), i
+            hits += t[io.site] > 0
+        events += io.kind != 0
+        so = np.array(io.stress[:6])
+        sg = np.array(ig.stress[:6])
+        worst = max(worst, np.max(np.abs(so - sg)) / max(np.max(np.abs(so)), 1e-3))
+        assert rc_o == rc_g
+        if rc_o:
+            break
+    T_o = o.field(po.F_TLOCAL)
+    T_err = np.max(np.abs(g.download(mg.F_TLOCAL) - T_o)) / np.max(np.abs(T_o))
+    print("  events", events, "crystal", hits, "worst %.2e" % worst, "Tlocal %.2e" % T_err, flush=True)
+    assert events > 100 and hits > 0
+    assert worst < 1e-10 and T_err < 1e-10
+    assert np.array_equal(g.download(mg.F_Q0), o.field(po.F_Q0))
