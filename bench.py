@@ -536,6 +536,7 @@ def inhom_leg(a, mg, kw, snap, dims, nsolve=10, nsteps=40):
 
 
 ENSEMBLE_GRIDS = [(32, 32, 32), (64, 64, 128), (96, 96, 96), (128, 128, 128)]
+ENSEMBLE_GRAPH_STEPS = 4          # ensemble steps per captured hipGraph (shape classes as parallel branches)
 
 
 def ensemble_plan(nrep):
@@ -585,7 +586,7 @@ def ensemble_leg(a, mg):
             t.set_print(5, 1.0)
             t.autotune(reps=5)
             t.set_solver(SOLVERS[solver], a.resync, a.local_max)
-    ens = mg.Ensemble(members, batch=64)
+    ens = mg.Ensemble(members, batch=64, use_graph=ENSEMBLE_GRAPH_STEPS)
     ens.run(a.ensemble_warmup, keep=False)
     for solver in twins:
         for t in twins[solver]:

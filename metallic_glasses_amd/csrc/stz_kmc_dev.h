@@ -86,6 +86,7 @@ struct VoxelState {
     R pd[6];
     R norm2;
     R soft;
+    const R *q0src; // the cached Q0*exp(-D*S) when fresh, else Q0 (selected once per voxel)
     R fac;          // crystal-like MRO: factor on the modifier and the softening per mode
     int t;          // MRO type (0 glass)
     bool fresh;     // the cached Q0*exp(-D*S) of this voxel is valid for this step's S
@@ -132,6 +133,10 @@ __device__ __forceinline__ void load_voxel(const RateArgs<R> &a, long long v, bo
         s.fac = (R)1;
     }
     s.fresh = (ld_stream<2>(a.scache + v) == s.soft);   // false for the NaN of an invalidated entry
+    // one address select per voxel (a select between two struct fields inside the mode
+    // loop is compiled through private memory)
+    const uintptr_t qa = (uintptr_t)a.q0eff, qb = (uintptr_t)a.q0;
+    s.q0src = (const R *)(s.fresh ? qa : qb);
     if (!s.fresh) {
         R st[5];
 #pragma unroll
@@ -148,10 +153,10 @@ struct ModeData {
 };
 
 template <typename R, int DIM>
-__device__ __forceinline__ void load_mode(const RateArgs<R> &a, long long v, int m, bool fresh, ModeData<R> &d)
+__device__ __forceinline__ void load_mode(const RateArgs<R> &a, long long v, int m, const R *q0src, ModeData<R> &d)
 {
     constexpr int S = Comp<DIM>::S;
-    d.q0 = fresh ? ld_stream<1>(a.q0eff + (long long)m * a.N + v) : ld_stream<1>(a.q0 + (long long)m * a.N + v);
+    d.q0 = ld_stream<1>(q0src + (long long)m * a.N + v);
 #pragma unroll
     for (int k = 0; k < S; ++k) d.st[k] = ld_stream<1>(a.deps + ((long long)m * S + k) * a.N + v);
 }
@@ -249,7 +254,7 @@ __device__ __forceinline__ void rates_block(const RateArgs<R> &a, long long blk)
         load_voxel<R, DIM, XTAL>(a, v, true, s);
 #pragma unroll
         for (int j = 0; j < STZ_RATE_DEPTH; ++j)
-            if (j < a.M) load_mode<R, DIM>(a, v, j, s.fresh, md[j]);
+            if (j < a.M) load_mode<R, DIM>(a, v, j, s.q0src, md[j]);
     }
     R minq = (R)0;
     int am = 0, nneg = 0;
@@ -257,7 +262,7 @@ __device__ __forceinline__ void rates_block(const RateArgs<R> &a, long long blk)
     for (int m = 0; m < a.M; ++m) {
         double rate = 0.0;
         if (valid) {
-            if (m + STZ_RATE_DEPTH < a.M) load_mode<R, DIM>(a, v, m + STZ_RATE_DEPTH, s.fresh, md[STZ_RATE_DEPTH]);
+            if (m + STZ_RATE_DEPTH < a.M) load_mode<R, DIM>(a, v, m + STZ_RATE_DEPTH, s.q0src, md[STZ_RATE_DEPTH]);
             if constexpr (XTAL)
                 if (s.t) softm = softm * s.fac;
             R q;
@@ -523,7 +528,7 @@ __device__ bool select_block(const RateArgs<R> &a, const double *gsum, long long
         VoxelState<R> s;
         ModeData<R> md;
         load_voxel<R, DIM, XTAL>(a, v, false, s);
-        load_mode<R, DIM>(a, v, m, s.fresh, md);
+        load_mode<R, DIM>(a, v, m, s.q0src, md);
         R softm = s.soft;
         if constexpr (XTAL)
             if (s.t)
