@@ -84,6 +84,77 @@ struct alignas(2 * sizeof(R)) cpx {
     R x, y;
 };
 
+// Global-memory access through pointers the compiler cannot prove global (those read
+// from per-replica descriptors): generic pointers compile to flat instructions, and a
+// flat store in flight makes every later LDS read wait for it (both count on lgkmcnt).
+#define STZ_GLOBAL __attribute__((address_space(1)))
+template <typename R>
+struct Vec2;
+template <>
+struct Vec2<double> {
+    typedef double type __attribute__((ext_vector_type(2)));
+};
+template <>
+struct Vec2<float> {
+    typedef float type __attribute__((ext_vector_type(2)));
+};
+template <typename T>
+__device__ __forceinline__ T gload(const T *p)
+{
+    return *(const STZ_GLOBAL T *)p;
+}
+template <typename R>
+__device__ __forceinline__ cpx<R> gload(const cpx<R> *p)
+{
+    typedef typename Vec2<R>::type V;
+    const V v = *(const STZ_GLOBAL V *)p;
+    return {v.x, v.y};
+}
+// ld_stream through a global-address-space view of the pointer
+template <int BIT, typename T>
+__device__ __forceinline__ T gld_stream(const T *p)
+{
+    if constexpr ((STZ_NT_LOADS & BIT) != 0) return __builtin_nontemporal_load((const STZ_GLOBAL T *)p);
+    else return *(const STZ_GLOBAL T *)p;
+}
+template <typename T>
+__device__ __forceinline__ void gstore(T *p, T a)
+{
+    *(STZ_GLOBAL T *)p = a;
+}
+template <typename R>
+__device__ __forceinline__ void gstore(cpx<R> *p, cpx<R> a)
+{
+    typedef typename Vec2<R>::type V;
+    const V v = {a.x, a.y};
+    *(STZ_GLOBAL V *)p = v;
+}
+
+// Workgroup-strided loop over TOT elements (element e = tid + i*NT) in two phases per
+// chunk of CH iterations: every load of the chunk is issued before the first use, so a
+// thread keeps up to CH loads in flight instead of waiting on each (a loop whose body
+// loads and then writes LDS is otherwise compiled load -> wait -> write per element).
+template <typename T, int TOT, int NT, int CH = 8, typename LoadF, typename UseF>
+__device__ __forceinline__ void staged_loop(LoadF load, UseF use)
+{
+    constexpr int PER = (TOT + NT - 1) / NT;
+    const int tid = threadIdx.x;
+#pragma unroll
+    for (int i0 = 0; i0 < PER; i0 += CH) {
+        T v[CH];
+#pragma unroll
+        for (int i = 0; i < CH; ++i) {
+            const int e = tid + (i0 + i) * NT;
+            if (i0 + i < PER && (TOT % NT == 0 || e < TOT)) v[i] = load(e);
+        }
+#pragma unroll
+        for (int i = 0; i < CH; ++i) {
+            const int e = tid + (i0 + i) * NT;
+            if (i0 + i < PER && (TOT % NT == 0 || e < TOT)) use(e, v[i]);
+        }
+    }
+}
+
 // Constants derived from stz_params exactly as STZ/init.c:6-75 does.
 struct StzConsts {
     double E, nu, mu, lambda, alpha, kT, tTemp, dt, voxel, ratepre, wfac_v;

@@ -118,16 +118,25 @@ __device__ __forceinline__ double wsum_local(double x)
     return x;
 }
 
-// sigma[v] += uni + ds (FP64 sum, rounded to the storage precision once)
+// the C stress components of voxel v
 template <typename R, int C>
-__device__ __forceinline__ void local_store(const LocalGeom &g, R *sigma, int v, const double *uni, const R *ds,
-                                            double *acc, int &bad)
+__device__ __forceinline__ void local_load(const LocalGeom &g, const R *sigma, int v, R *sv)
+{
+#pragma unroll
+    for (int c = 0; c < C; ++c) sv[c] = gld_stream<4>(sigma + (long long)c * g.N + v);
+}
+
+// sigma[v] = sv + uni + ds (FP64 sum, rounded to the storage precision once); sv holds
+// the components local_load read before any store of the workgroup's rows
+template <typename R, int C>
+__device__ __forceinline__ void local_store(const LocalGeom &g, R *sigma, int v, const R *sv, const double *uni,
+                                            const R *ds, double *acc, int &bad)
 {
 #pragma unroll
     for (int c = 0; c < C; ++c) {
-        const double s = ((double)ld_stream<4>(sigma + (long long)c * g.N + v) + uni[c]) + (double)ds[c];
+        const double s = ((double)sv[c] + uni[c]) + (double)ds[c];
         bad |= (s != s);
-        sigma[(long long)c * g.N + v] = (R)s;
+        gstore(sigma + (long long)c * g.N + v, (R)s);
         acc[c] += s;
     }
 }
@@ -146,11 +155,14 @@ __device__ __forceinline__ void local_sum(const LocalGeom &g, const R *__restric
         if (d1 < 0) d1 += g.n[1];
         if (d2 < 0) d2 += g.n[2];
         const int hj = half_index(g, d0, d1, d2);
+        R kv[S * C];   // the source's response values, all loads issued before the first use
+#pragma unroll
+        for (int i = 0; i < S * C; ++i) kv[i] = gload(K + (long long)i * g.H + hj);
 #pragma unroll
         for (int k = 0; k < S; ++k) {
             const R dk = src[j].d[k];
 #pragma unroll
-            for (int c = 0; c < C; ++c) out[c] = out[c] + dk * K[((long long)k * C + c) * g.H + hj];
+            for (int c = 0; c < C; ++c) out[c] = out[c] + dk * kv[k * C + c];
         }
     }
 }
@@ -192,25 +204,38 @@ __device__ void local_rows(const LocalGeom &g, const R *__restrict__ K, const Lo
         const int b1 = (s0.c[1] - d1 + n1) % n1, b2 = (s0.c[2] - d2 + n2) % n2;
         const int va = (a2 * n1 + a1) * n0 + (s0.c[0] + d0) % n0;
         const int vb = (b2 * n1 + b1) * n0 + (s0.c[0] - d0 + n0) % n0;
+        // va != vb on a pair row, and every voxel belongs to one row of one workgroup:
+        // the stress of both voxels is read before anything is written, so the loads are
+        // all in flight together instead of one load-wait-store round trip per component
+        R sa[C], sb[C];
         R da[C], db[C];
         if (ns == 1) {
+            // the response row first, then the stress: all S*C + 2*C loads are issued
+            // before the first use
             const int hj = (d2 * n1 + d1) * n0 + d0;
+            R kv[S * C];
+#pragma unroll
+            for (int i = 0; i < S * C; ++i) kv[i] = gld_stream<4>(K + (long long)i * g.H + hj);
+            local_load<R, C>(g, sigma, va, sa);
+            if (pair) local_load<R, C>(g, sigma, vb, sb);
 #pragma unroll
             for (int c = 0; c < C; ++c) da[c] = (R)0;
 #pragma unroll
             for (int k = 0; k < S; ++k) {
                 const R dk = s0.d[k];
 #pragma unroll
-                for (int c = 0; c < C; ++c) da[c] = da[c] + dk * ld_stream<4>(K + ((long long)k * C + c) * g.H + hj);
+                for (int c = 0; c < C; ++c) da[c] = da[c] + dk * kv[k * C + c];
             }
 #pragma unroll
             for (int c = 0; c < C; ++c) db[c] = da[c];
         } else {
+            local_load<R, C>(g, sigma, va, sa);
+            if (pair) local_load<R, C>(g, sigma, vb, sb);
             local_sum<R, DIM>(g, K, src, ns, va % n0, a1, a2, da);
             if (pair) local_sum<R, DIM>(g, K, src, ns, vb % n0, b1, b2, db);
         }
-        local_store<R, C>(g, sigma, va, uni, da, acc, bad);
-        if (pair) local_store<R, C>(g, sigma, vb, uni, db, acc, bad);
+        local_store<R, C>(g, sigma, va, sa, uni, da, acc, bad);
+        if (pair) local_store<R, C>(g, sigma, vb, sb, uni, db, acc, bad);
     }
     if (bad) atomicOr(nanflag, 1);
 #pragma unroll

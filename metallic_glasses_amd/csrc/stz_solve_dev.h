@@ -358,19 +358,27 @@ __global__ __launch_bounds__(NT_SOLVE) void k_fwd_x(const double *__restrict__ e
     for (int i = tid; i < L0; i += NT_SOLVE) tw[i] = twg[i];
     const long long row0 = (long long)blockIdx.x * NR;
     R *bufr = reinterpret_cast<R *>(buf);
-    for (int e = tid; e < NR * L0; e += NT_SOLVE) {
-        const int rl = e / L0, i0 = e % L0;
-        const long long v = (row0 + rl) * L0 + i0;
-        double ev[C], rt[C];
+    struct Ev {
+        double v[C];
+    };
+    staged_loop<Ev, NR * L0, NT_SOLVE, 4>(
+        [&](int e) {
+            const long long v = (row0 + e / L0) * L0 + e % L0;
+            Ev r;
 #pragma unroll
-        for (int c = 0; c < C; ++c) ev[c] = eps0[(long long)c * k.N + v];
-        if constexpr (MODE == 0) round_trip<C>(ev, vw.ea, k.lambda, k.mu, rt);
-        else
+            for (int c = 0; c < C; ++c) r.v[c] = gload(eps0 + (long long)c * k.N + v);
+            return r;
+        },
+        [&](int e, const Ev &ev) {
+            const int rl = e / L0, i0 = e % L0;
+            double rt[C];
+            if constexpr (MODE == 0) round_trip<C>(ev.v, vw.ea, k.lambda, k.mu, rt);
+            else
 #pragma unroll
-            for (int c = 0; c < C; ++c) rt[c] = ev[c];
+                for (int c = 0; c < C; ++c) rt[c] = ev.v[c];
 #pragma unroll
-        for (int c = 0; c < C; ++c) bufr[2 * (i0 * LS + c * NP + (rl >> 1)) + (rl & 1)] = (R)rt[c];
-    }
+            for (int c = 0; c < C; ++c) bufr[2 * (i0 * LS + c * NP + (rl >> 1)) + (rl & 1)] = (R)rt[c];
+        });
     __syncthreads();
     fft_lines<R, L0, NL, LS, NT_SOLVE>(buf, tw, (R)-1);
     constexpr int H0 = L0 / 2 + 1;
@@ -384,7 +392,7 @@ __global__ __launch_bounds__(NT_SOLVE) void k_fwd_x(const double *__restrict__ e
         cpx<R> X;
         if ((rl & 1) == 0) X = {(Z.x + Zm.x) * (R)0.5, (Z.y - Zm.y) * (R)0.5};
         else X = {(Z.y + Zm.y) * (R)0.5, -(Z.x - Zm.x) * (R)0.5};
-        spec[((long long)c * k.rows + row0 + rl) * k.hp + kk] = X;
+        gstore(spec + ((long long)c * k.rows + row0 + rl) * k.hp + kk, X);
     }
 }
 
@@ -410,30 +418,39 @@ __global__ __launch_bounds__(NT_SOLVE) void k_inv_x(cpx<R> *__restrict__ spec_a,
     R *__restrict__ sigma = vw.sigma;
     double *__restrict__ spart = vw.spart;
     int *__restrict__ nanflag = vw.nanflag;
+    // the block reduction reuses the FFT tile when it is large enough
+    constexpr int NRED = C + (MODE == 1 ? 1 : 0);
+    constexpr bool RED_IN_BUF = (long long)L0 * LS * (long long)sizeof(cpx<R>) >= (long long)NRED * NT_SOLVE * 8;
     __shared__ cpx<R> buf[L0 * LS];
     __shared__ cpx<R> tw[L0];
-    __shared__ double red[NT_SOLVE];
+    __shared__ double redx[RED_IN_BUF ? 1 : NRED * NT_SOLVE];
     const int tid = threadIdx.x;
     for (int i = tid; i < L0; i += NT_SOLVE) tw[i] = twg[i];
     const long long row0 = (long long)blockIdx.x * NR;
-    for (int e = tid; e < C * NP * L0; e += NT_SOLVE) {
-        const int kk = e % L0;
-        const int p = (e / L0) % NP;
-        const int c = e / (L0 * NP);
-        const cpx<R> *ra = spec + ((long long)c * k.rows + row0 + 2 * p) * k.hp;
-        const cpx<R> *rb = ra + k.hp;
-        cpx<R> Z;
-        if (kk == 0 || kk == H) {
-            Z = {ra[kk].x, rb[kk].x};
-        } else if (kk < H) {
-            const cpx<R> a = ra[kk], b = rb[kk];
-            Z = {a.x - b.y, a.y + b.x};
-        } else {
-            const cpx<R> a = ra[L0 - kk], b = rb[L0 - kk];
-            Z = {a.x + b.y, -a.y + b.x};
-        }
-        buf[kk * LS + c * NP + p] = Z;
-    }
+    struct Pair {
+        cpx<R> a, b;
+    };
+    staged_loop<Pair, C * NP * L0, NT_SOLVE, 6>(
+        [&](int e) {
+            const int kk = e % L0;
+            const int p = (e / L0) % NP;
+            const int c = e / (L0 * NP);
+            const cpx<R> *ra = spec + ((long long)c * k.rows + row0 + 2 * p) * k.hp + (kk <= H ? kk : L0 - kk);
+            Pair q;
+            q.a = gload(ra);
+            q.b = gload(ra + k.hp);
+            return q;
+        },
+        [&](int e, const Pair &q) {
+            const int kk = e % L0;
+            const int p = (e / L0) % NP;
+            const int c = e / (L0 * NP);
+            cpx<R> Z;
+            if (kk == 0 || kk == H) Z = {q.a.x, q.b.x};
+            else if (kk < H) Z = {q.a.x - q.b.y, q.a.y + q.b.x};
+            else Z = {q.a.x + q.b.y, -q.a.y + q.b.x};
+            buf[kk * LS + c * NP + p] = Z;
+        });
     __syncthreads();
     fft_lines<R, L0, NL, LS, NT_SOLVE>(buf, tw, (R)1);
     const R *bufr = reinterpret_cast<const R *>(buf);
@@ -442,63 +459,75 @@ __global__ __launch_bounds__(NT_SOLVE) void k_inv_x(cpx<R> *__restrict__ spec_a,
     for (int c = 0; c < C; ++c) acc[c] = 0.0;
     double cacc = 0.0;
     int bad = 0;
-    for (int e = tid; e < NR * L0; e += NT_SOLVE) {
-        const int rl = e / L0, i0 = e % L0;
-        const long long v = (row0 + rl) * L0 + i0;
-        double ev[C], rt[C], d[C], s[C], tot[C];
+    struct In {
+        double ev[C];
+        double t0[MODE == 1 ? C : 1];
+        double lx, mx;
+    };
+    staged_loop<In, NR * L0, NT_SOLVE, 2>(
+        [&](int e) {
+            const long long v = (row0 + e / L0) * L0 + e % L0;
+            In r;
 #pragma unroll
-        for (int c = 0; c < C; ++c) ev[c] = eps0[(long long)c * k.N + v];
-        if constexpr (MODE == 0) round_trip<C>(ev, vw.ea, k.lambda, k.mu, rt);
-        else
+            for (int c = 0; c < C; ++c) r.ev[c] = gload(eps0 + (long long)c * k.N + v);
+            if constexpr (MODE == 1) {
 #pragma unroll
-            for (int c = 0; c < C; ++c) rt[c] = ev[c];
+                for (int c = 0; c < C; ++c) r.t0[c] = gload(k.tau0 + (long long)c * k.N + v);
+                r.lx = gload(k.lamx + v);
+                r.mx = gload(k.mux + v);
+            }
+            return r;
+        },
+        [&](int e, const In &in) {
+            const int rl = e / L0, i0 = e % L0;
+            const long long v = (row0 + rl) * L0 + i0;
+            double rt[C], d[C], s[C], tot[C];
+            if constexpr (MODE == 0) round_trip<C>(in.ev, vw.ea, k.lambda, k.mu, rt);
+            else
 #pragma unroll
-        for (int c = 0; c < C; ++c) {
-            const double z = (double)bufr[2 * (i0 * LS + c * NP + (rl >> 1)) + (rl & 1)];
-            bad |= (z != z);
-            tot[c] = z / (double)k.N + vw.ea[c];   // STZ/elastic.c:822
-            d[c] = tot[c] - rt[c];
-        }
-        stress_of<C>(d, k.lambda, k.mu, s);
-#pragma unroll
-        for (int c = 0; c < C; ++c) {
-            sigma[(long long)c * k.N + v] = (R)s[c];
-            acc[c] += s[c];
-        }
-        if constexpr (MODE == 1) {
-            double t0[C], nx[C];
-#pragma unroll
-            for (int c = 0; c < C; ++c) t0[c] = k.tau0[(long long)c * k.N + v];
-            inhom_update<C>(tot, rt, t0, k.lamx[v], k.mux[v], k.lambda, k.mu, nx);
+                for (int c = 0; c < C; ++c) rt[c] = in.ev[c];
 #pragma unroll
             for (int c = 0; c < C; ++c) {
-                k.eout[(long long)c * k.N + v] = nx[c];
-                const double dd = nx[c] - rt[c];
-                cacc += dd * dd;
+                const double z = (double)bufr[2 * (i0 * LS + c * NP + (rl >> 1)) + (rl & 1)];
+                bad |= (z != z);
+                tot[c] = z / (double)k.N + vw.ea[c];   // STZ/elastic.c:822
+                d[c] = tot[c] - rt[c];
             }
-        }
-    }
+            stress_of<C>(d, k.lambda, k.mu, s);
+#pragma unroll
+            for (int c = 0; c < C; ++c) {
+                gstore(sigma + (long long)c * k.N + v, (R)s[c]);
+                acc[c] += s[c];
+            }
+            if constexpr (MODE == 1) {
+                double nx[C];
+                inhom_update<C>(tot, rt, in.t0, in.lx, in.mx, k.lambda, k.mu, nx);
+#pragma unroll
+                for (int c = 0; c < C; ++c) {
+                    gstore(k.eout + (long long)c * k.N + v, nx[c]);
+                    const double dd = nx[c] - rt[c];
+                    cacc += dd * dd;
+                }
+            }
+        });
     if (bad) atomicOr(nanflag, 1);
-    // fixed-shape tree reduction of each component
-    for (int c = 0; c < C; ++c) {
-        red[tid] = acc[c];
-        __syncthreads();
-        for (int w = NT_SOLVE / 2; w > 0; w >>= 1) {
-            if (tid < w) red[tid] = red[tid] + red[tid + w];
-            __syncthreads();
-        }
-        if (tid == 0) spart[(long long)blockIdx.x * C + c] = red[0];
+    // fixed-shape tree reduction, all components at once (each component's tree is the
+    // same as a reduction of that component alone)
+    __syncthreads();   // every epilogue read of the tile is done before it is reused
+    double *red = RED_IN_BUF ? reinterpret_cast<double *>(buf) : redx;
+#pragma unroll
+    for (int c = 0; c < C; ++c) red[c * NT_SOLVE + tid] = acc[c];
+    if constexpr (MODE == 1) red[C * NT_SOLVE + tid] = cacc;
+    __syncthreads();
+    for (int w = NT_SOLVE / 2; w > 0; w >>= 1) {
+        if (tid < w)
+#pragma unroll
+            for (int c = 0; c < NRED; ++c) red[c * NT_SOLVE + tid] = red[c * NT_SOLVE + tid] + red[c * NT_SOLVE + tid + w];
         __syncthreads();
     }
-    if constexpr (MODE == 1) {
-        red[tid] = cacc;
-        __syncthreads();
-        for (int w = NT_SOLVE / 2; w > 0; w >>= 1) {
-            if (tid < w) red[tid] = red[tid] + red[tid + w];
-            __syncthreads();
-        }
-        if (tid == 0) k.cpart[blockIdx.x] = red[0];
-    }
+    if (tid < C) spart[(long long)blockIdx.x * C + tid] = red[tid * NT_SOLVE];
+    if constexpr (MODE == 1)
+        if (tid == 0) k.cpart[blockIdx.x] = red[C * NT_SOLVE];
 }
 
 // rows per workgroup of the x passes: the largest power of two (2..32) whose LDS tile,
@@ -535,17 +564,17 @@ __global__ __launch_bounds__(NT_SOLVE) void k_ypass(cpx<R> *__restrict__ spec_a,
     const int i2 = blockIdx.y;
     const int c = blockIdx.z % k.C;
     cpx<R> *base = spec + ((long long)c * k.rows + (long long)i2 * L1) * k.hp + k0base;
-    for (int e = tid; e < L1 * W; e += NT_SOLVE) {
-        const int col = e % W, p = e / W;
-        cpx<R> val = {0, 0};
-        if (k0base + col < k.h0) val = base[(long long)p * k.hp + col];
-        buf[p * W + col] = val;
-    }
+    staged_loop<cpx<R>, L1 * W, NT_SOLVE>(
+        [&](int e) {
+            const int col = e % W, p = e / W;
+            return k0base + col < k.h0 ? gload(base + (long long)p * k.hp + col) : cpx<R>{0, 0};
+        },
+        [&](int e, const cpx<R> &v) { buf[e] = v; });
     __syncthreads();
     fft_lines<R, L1, W, W, NT_SOLVE>(buf, tw, sgn);
     for (int e = tid; e < L1 * W; e += NT_SOLVE) {
         const int col = e % W, p = e / W;
-        if (k0base + col < k.h0) base[(long long)p * k.hp + col] = buf[p * W + col];
+        if (k0base + col < k.h0) gstore(base + (long long)p * k.hp + col, buf[e]);
     }
 }
 
@@ -573,13 +602,15 @@ __global__ __launch_bounds__(NTF) void k_fused(cpx<R> *__restrict__ spec_a, cons
     const int i1 = (DIM == 3) ? blockIdx.y : 0;
     // element (p, col) of component c lives at spec[(c*rows + p*pstride + i1) * hp + k0]
     const long long pstride = (DIM == 3) ? k.n[1] : 1;
-    for (int e = tid; e < NE; e += NTF) {
-        const int line = e % NL, p = e / NL;
-        const int c = line / W, col = line % W;
-        cpx<R> val = {0, 0};
-        if (k0base + col < k.h0) val = spec[((long long)c * k.rows + (long long)p * pstride + i1) * k.hp + k0base + col];
-        buf[e] = val;
-    }
+    staged_loop<cpx<R>, NE, NTF>(
+        [&](int e) {
+            const int line = e % NL, p = e / NL;
+            const int c = line / W, col = line % W;
+            return k0base + col < k.h0
+                       ? gload(spec + ((long long)c * k.rows + (long long)p * pstride + i1) * k.hp + k0base + col)
+                       : cpx<R>{0, 0};
+        },
+        [&](int e, const cpx<R> &v) { buf[e] = v; });
     __syncthreads();
     fft_lines<R, LZ, NL, NL, NTF>(buf, tw, (R)-1);
     // Gamma-hat (OP 0), the displacement operator (OP 1) or the heat multiplier (OP 2) on
@@ -696,7 +727,7 @@ __global__ __launch_bounds__(NTF) void k_fused(cpx<R> *__restrict__ spec_a, cons
     for (int e = tid; e < NE; e += NTF) {
         const int line = e % NL, p = e / NL;
         const int c = line / W, col = line % W;
-        if (k0base + col < k.h0) spec[((long long)c * k.rows + (long long)p * pstride + i1) * k.hp + k0base + col] = buf[e];
+        if (k0base + col < k.h0) gstore(spec + ((long long)c * k.rows + (long long)p * pstride + i1) * k.hp + k0base + col, buf[e]);
     }
 }
 
