@@ -537,6 +537,80 @@ __device__ __forceinline__ void fft_lines(cpx<R> *buf, const cpx<R> *tw, R s)
     fft_plan<R, L, NL, LS, NT, 0, 1>(buf, tw, s);
 }
 
+// fft_pass whose inputs come from ld(line, point) when GIN (the first pass of a transform
+// of data in global memory: the points go straight into the registers of the butterflies,
+// all loads of a thread issued before the first use) and whose outputs go to
+// st(line, point, value) when GOUT (the last pass: no LDS round trip before the store).
+// The LDS tile is only touched by the passes in between.
+template <typename R, int L, int NL, int LS, int NT, int Q, int NS, bool GIN, bool GOUT, typename Ld, typename St>
+__device__ __forceinline__ void fft_pass_io(cpx<R> *buf, const cpx<R> *tw, R s, Ld &ld, St &st)
+{
+    constexpr int NQ = L / Q;
+    constexpr int TASKS = NL * NQ;
+    constexpr int TPT = (TASKS + NT - 1) / NT;
+    constexpr int TSTEP = L / (NS * Q);
+    cpx<R> v[TPT][Q];
+    const int tid = threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < TPT; ++k) {
+        const int t = tid + k * NT;
+        if ((TASKS % NT == 0) || t < TASKS) {
+            const int line = t % NL, j = t / NL;
+#pragma unroll
+            for (int r = 0; r < Q; ++r) {
+                if constexpr (GIN) v[k][r] = ld(line, j + r * NQ);
+                else v[k][r] = buf[(j + r * NQ) * LS + line];
+            }
+        }
+    }
+    if constexpr (!GIN) __syncthreads();   // in place: every read of the tile before a write
+#pragma unroll
+    for (int k = 0; k < TPT; ++k) {
+        const int t = tid + k * NT;
+        if ((TASKS % NT == 0) || t < TASKS) {
+            const int line = t % NL, j = t / NL;
+            const int jm = j % NS;
+            if (NS > 1) {
+#pragma unroll
+                for (int r = 1; r < Q; ++r) {
+                    cpx<R> w = tw[jm * TSTEP * r];
+                    w.y = w.y * (-s);
+                    v[k][r] = cmul(v[k][r], w);
+                }
+            }
+            Dft<R, Q>::run(v[k], s);
+            const int idxD = (j / NS) * NS * Q + jm;
+#pragma unroll
+            for (int r = 0; r < Q; ++r) {
+                if constexpr (GOUT) st(line, idxD + r * NS, v[k][r]);
+                else buf[(idxD + r * NS) * LS + line] = v[k][r];
+            }
+        }
+    }
+    if constexpr (!GOUT) __syncthreads();
+}
+
+template <typename R, int L, int NL, int LS, int NT, int I, int NS, bool GIN, bool GOUT, typename Ld, typename St>
+__device__ __forceinline__ void fft_plan_io(cpx<R> *buf, const cpx<R> *tw, R s, Ld &ld, St &st)
+{
+    if constexpr (I < radix_count(L)) {
+        constexpr int Q = radix_at(L, I);
+        constexpr bool LASTP = I + 1 == radix_count(L);
+        fft_pass_io<R, L, NL, LS, NT, Q, NS, GIN && I == 0, GOUT && LASTP>(buf, tw, s, ld, st);
+        fft_plan_io<R, L, NL, LS, NT, I + 1, NS * Q, GIN, GOUT>(buf, tw, s, ld, st);
+    }
+}
+
+// fft_lines with the input read through ld (GIN) and/or the output written through st
+// (GOUT) instead of the LDS tile; the twiddle table must be in LDS before the second pass
+// (the first pass multiplies by no twiddle)
+template <typename R, int L, int NL, int LS, int NT, bool GIN, bool GOUT, typename Ld, typename St>
+__device__ __forceinline__ void fft_lines_io(cpx<R> *buf, const cpx<R> *tw, R s, Ld ld, St st)
+{
+    static_assert(radix_count(L) > 0, "unsupported FFT length");
+    fft_plan_io<R, L, NL, LS, NT, 0, 1, GIN, GOUT>(buf, tw, s, ld, st);
+}
+
 // <sigma> from the stress-solve block partials spart[b][c] (fixed order: per-thread
 // strided runs, then a tree); a workgroup of exactly STZ_NT_MEAN threads; out[c] for c < C
 #define STZ_NT_MEAN 1024

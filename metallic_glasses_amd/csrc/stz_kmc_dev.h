@@ -596,25 +596,41 @@ struct AthArgs {
     double *socp;
 };
 
+// the athermal flag of voxel v (STZ/evolution.c:52-56): min_m Q < 0 or |min_m Q| < thr
+template <typename R>
+__device__ __forceinline__ int athermal_flag(const AthArgs<R> &a, long long v)
+{
+    const R q = a.minq[v];
+    return q < (R)0 || fabs((double)q) < a.thr;
+}
+
+// the block's work given every thread's flag f (0 beyond N).  `par` alternates between
+// consecutive calls of one workgroup: the per-wave counts are double-buffered, so the
+// next block may start while a slower wave still reads this block's counts (every call
+// has a barrier after its count writes, so no wave is two blocks ahead).  A block without
+// flagged voxels writes +0.0, the value of its tree of zeros.
 template <typename R, int DIM>
-__device__ void athermal_block(const AthArgs<R> &a, long long blk)
+__device__ void athermal_block_f(const AthArgs<R> &a, long long blk, int f, int par)
 {
     constexpr int C = Comp<DIM>::C, S = Comp<DIM>::S;
-    __shared__ int ws[NW];
+    __shared__ int ws2[2][NW];
     __shared__ double red[NT];
+    int *ws = ws2[par];
     const int tid = threadIdx.x, lane = tid % WAVE, w = tid / WAVE;
     const long long v = blk * NT + tid;
-    int f = 0;
-    if (v < a.N) {
-        const R q = a.minq[v];
-        f = (q < (R)0 || fabs((double)q) < a.thr);
-    }
     const unsigned long long bal = __ballot(f);
     const int before = __popcll(bal & ((1ull << lane) - 1ull));
     if (lane == 0) ws[w] = __popcll(bal);
     __syncthreads();
-    int loff = 0;
-    for (int i = 0; i < w; ++i) loff += ws[i];
+    int loff = 0, tot = 0;
+    for (int i = 0; i < NW; ++i) {
+        if (i < w) loff += ws[i];
+        tot += ws[i];
+    }
+    if (tot == 0) {                   // uniform over the workgroup
+        if (tid == 0) a.socp[blk] = 0.0;
+        return;
+    }
     double dw = 0.0;
     if (f) {
         const long long idx = (long long)a.off[blk] + loff + before;
@@ -643,6 +659,13 @@ __device__ void athermal_block(const AthArgs<R> &a, long long blk)
         __syncthreads();
     }
     if (tid == 0) a.socp[blk] = red[0];
+}
+
+template <typename R, int DIM>
+__device__ void athermal_block(const AthArgs<R> &a, long long blk)
+{
+    const long long v = blk * NT + threadIdx.x;
+    athermal_block_f<R, DIM>(a, blk, v < a.N ? athermal_flag(a, v) : 0, 0);
 }
 
 // fixed-order sum of the athermal block partials by one workgroup of NTH threads

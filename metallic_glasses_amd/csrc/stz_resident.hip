@@ -294,9 +294,25 @@ __global__ __launch_bounds__(NT) void kb_athermal(const RepDev *__restrict__ rep
     a.cap = 1ll << 62;
     a.N = N;
     a.socp = d.socp;
-    for (long long blk = blockIdx.x; blk < nblk; blk += gridDim.x) {
-        athermal_block<R, DIM>(a, blk);
-        __syncthreads();              // the block's shared scratch is reused by the next one
+    // the flags of PRE blocks are loaded together, then the blocks run in order
+    constexpr int PRE = 8;
+    int par = 0;
+    for (long long b0 = blockIdx.x; b0 < nblk; b0 += (long long)gridDim.x * PRE) {
+        int f[PRE];
+#pragma unroll
+        for (int i = 0; i < PRE; ++i) {
+            const long long blk = b0 + (long long)i * gridDim.x;
+            const long long v = blk * NT + threadIdx.x;
+            f[i] = (blk < nblk && v < N) ? athermal_flag(a, v) : 0;
+        }
+#pragma unroll
+        for (int i = 0; i < PRE; ++i) {
+            const long long blk = b0 + (long long)i * gridDim.x;
+            if (blk < nblk) {
+                athermal_block_f<R, DIM>(a, blk, f[i], par);
+                par ^= 1;
+            }
+        }
     }
 }
 

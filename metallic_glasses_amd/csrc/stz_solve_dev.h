@@ -29,6 +29,7 @@ struct SolveConsts {
     double lambda, mu, alpha, s0, sh, sa, invN;
     double epsavg[6];
     int n[3];
+    double rn[3];       // 1.0 / n[a] (the reference grid's k-vector scale, computed once)
     long long N;
     int rows;           // n1*n2
     int h0, hp;
@@ -190,14 +191,14 @@ __device__ __forceinline__ void gamma_part(const R *e, const R *g, const R *gg, 
 }
 
 // unit vector of the reference k grid (Grad, STZ/init.c:116-137) for integer indices;
-// returns |k| (g2, set to 1 at k = 0)
+// returns |k| (g2, set to 1 at k = 0); rn[a] = 1.0 / n[a]
 template <int DIM>
-__device__ __forceinline__ double unit_k(const int *idx, const int *n, double *g)
+__device__ __forceinline__ double unit_k(const int *idx, const double *rn, double *g)
 {
     g[0] = g[1] = g[2] = 0.0;
 #pragma unroll
     for (int a = 0; a < DIM; ++a) {
-        double s = (double)(idx[a]) * (double)(1.0 / n[a]);
+        double s = (double)(idx[a]) * rn[a];
         g[a] = (s - round(s)) * (2 * STZ_PI_REF);
     }
     double g2;
@@ -564,18 +565,16 @@ __global__ __launch_bounds__(NT_SOLVE) void k_ypass(cpx<R> *__restrict__ spec_a,
     const int i2 = blockIdx.y;
     const int c = blockIdx.z % k.C;
     cpx<R> *base = spec + ((long long)c * k.rows + (long long)i2 * L1) * k.hp + k0base;
-    staged_loop<cpx<R>, L1 * W, NT_SOLVE>(
-        [&](int e) {
-            const int col = e % W, p = e / W;
+    // line = column of the tile; the first pass reads the columns from the spectrum and
+    // the last pass writes them back
+    fft_lines_io<R, L1, W, W, NT_SOLVE, true, true>(
+        buf, tw, sgn,
+        [&](int col, int p) {
             return k0base + col < k.h0 ? gload(base + (long long)p * k.hp + col) : cpx<R>{0, 0};
         },
-        [&](int e, const cpx<R> &v) { buf[e] = v; });
-    __syncthreads();
-    fft_lines<R, L1, W, W, NT_SOLVE>(buf, tw, sgn);
-    for (int e = tid; e < L1 * W; e += NT_SOLVE) {
-        const int col = e % W, p = e / W;
-        if (k0base + col < k.h0) gstore(base + (long long)p * k.hp + col, buf[e]);
-    }
+        [&](int col, int p, const cpx<R> &v) {
+            if (k0base + col < k.h0) gstore(base + (long long)p * k.hp + col, v);
+        });
 }
 
 // ---------------------------------------------------------------------------------
@@ -602,17 +601,14 @@ __global__ __launch_bounds__(NTF) void k_fused(cpx<R> *__restrict__ spec_a, cons
     const int i1 = (DIM == 3) ? blockIdx.y : 0;
     // element (p, col) of component c lives at spec[(c*rows + p*pstride + i1) * hp + k0]
     const long long pstride = (DIM == 3) ? k.n[1] : 1;
-    staged_loop<cpx<R>, NE, NTF>(
-        [&](int e) {
-            const int line = e % NL, p = e / NL;
-            const int c = line / W, col = line % W;
-            return k0base + col < k.h0
-                       ? gload(spec + ((long long)c * k.rows + (long long)p * pstride + i1) * k.hp + k0base + col)
-                       : cpx<R>{0, 0};
-        },
-        [&](int e, const cpx<R> &v) { buf[e] = v; });
-    __syncthreads();
-    fft_lines<R, LZ, NL, NL, NTF>(buf, tw, (R)-1);
+    auto gaddr = [&](int line, int p) {
+        return spec + ((long long)(line / W) * k.rows + (long long)p * pstride + i1) * k.hp + k0base + line % W;
+    };
+    // forward: the first pass reads the tile from the spectrum, the result stays in LDS
+    fft_lines_io<R, LZ, NL, NL, NTF, true, false>(
+        buf, tw, (R)-1,
+        [&](int line, int p) { return k0base + line % W < k.h0 ? gload(gaddr(line, p)) : cpx<R>{0, 0}; },
+        [](int, int, const cpx<R> &) {});
     // Gamma-hat (OP 0), the displacement operator (OP 1) or the heat multiplier (OP 2) on
 // each wavevector of the tile
     const R lam = (R)k.lambda, mu = (R)k.mu, s0 = (R)k.s0, sh = (R)k.sh, sa = (R)k.sa;
@@ -628,7 +624,7 @@ __global__ __launch_bounds__(NTF) void k_fused(cpx<R> *__restrict__ spec_a, cons
             if constexpr (DIM == 3) { idx[1] = i1; idx[2] = p; }
             else { idx[1] = p; idx[2] = 0; }
             double gd[3];
-            const R kmag = (R)unit_k<DIM>(idx, k.n, gd);
+            const R kmag = (R)unit_k<DIM>(idx, k.rn, gd);
             bool nyq = false;
 #pragma unroll
             for (int a = 0; a < DIM; ++a) nyq |= (2 * idx[a] == k.n[a]);
@@ -669,7 +665,7 @@ __global__ __launch_bounds__(NTF) void k_fused(cpx<R> *__restrict__ spec_a, cons
             if constexpr (DIM == 3) { idx[1] = i1; idx[2] = p; }
             else { idx[1] = p; idx[2] = 0; }
             double gd[3];
-            const double g2 = unit_k<DIM>(idx, k.n, gd);
+            const double g2 = unit_k<DIM>(idx, k.rn, gd);
             const R m = (R)exp(-g2 * k.hdt);
 #pragma unroll
             for (int c = 0; c < C; ++c) {
@@ -685,7 +681,7 @@ __global__ __launch_bounds__(NTF) void k_fused(cpx<R> *__restrict__ spec_a, cons
         if constexpr (DIM == 3) { idx[1] = i1; idx[2] = p; }
         else { idx[1] = p; idx[2] = 0; }
         double gd[3];
-        unit_k<DIM>(idx, k.n, gd);
+        unit_k<DIM>(idx, k.rn, gd);
         bool nyq = false;
 #pragma unroll
         for (int a = 0; a < DIM; ++a) nyq |= (2 * idx[a] == k.n[a]);
@@ -723,12 +719,12 @@ __global__ __launch_bounds__(NTF) void k_fused(cpx<R> *__restrict__ spec_a, cons
     }
     }
     __syncthreads();
-    fft_lines<R, LZ, NL, NL, NTF>(buf, tw, (R)1);
-    for (int e = tid; e < NE; e += NTF) {
-        const int line = e % NL, p = e / NL;
-        const int c = line / W, col = line % W;
-        if (k0base + col < k.h0) gstore(spec + ((long long)c * k.rows + (long long)p * pstride + i1) * k.hp + k0base + col, buf[e]);
-    }
+    // inverse from the tile; the last pass writes the spectrum
+    fft_lines_io<R, LZ, NL, NL, NTF, false, true>(
+        buf, tw, (R)1, [](int, int) { return cpx<R>{0, 0}; },
+        [&](int line, int p, const cpx<R> &v) {
+            if (k0base + line % W < k.h0) gstore(gaddr(line, p), v);
+        });
 }
 
 // k0 columns per tile for the strided passes
@@ -774,7 +770,10 @@ inline SolveConsts solve_consts(const stz_ctx *ctx)
     k.sa = -1.0 * ctx->k.alpha / ctx->k.mu;
     k.invN = 1.0 / (double)ctx->N;
     for (int c = 0; c < 6; ++c) k.epsavg[c] = ctx->epsavg[c];
-    for (int a = 0; a < 3; ++a) k.n[a] = ctx->n[a];
+    for (int a = 0; a < 3; ++a) {
+        k.n[a] = ctx->n[a];
+        k.rn[a] = 1.0 / (double)ctx->n[a];
+    }
     k.N = ctx->N;
     k.rows = ctx->n[1] * ctx->n[2];
     k.h0 = ctx->h0;
