@@ -446,6 +446,22 @@ __device__ void transform_voxel(const ApplyArgs &a, long long v, int mm, const d
         for (int k = 0; k < S; ++k)
             deps[((long long)m * S + k) * a.N + v] = (R)(dscale * dnew[m * S + k]);   // :508-512
 }
+
+// the redraw of transform_voxel (the crystal-like barriers or the M*S strains) by the
+// NT threads of a workgroup, one value per thread: every load of the draws is in flight
+// at once instead of one load-store round trip per value in a single thread
+template <typename R, int DIM>
+__device__ void redraw_voxel_wg(const ApplyArgs &a, long long v, const double *dnew, double dscale)
+{
+    constexpr int S = Comp<DIM>::S;
+    if (a.mro && a.mro[v]) {
+        R *q0 = (R *)a.q0;
+        for (int m = threadIdx.x; m < a.M; m += NT) q0[(long long)m * a.N + v] = (R)(a.mro_q0 * (1 - a.rratio + a.rratio * (dnew[m])));
+        return;
+    }
+    R *deps = (R *)a.deps;
+    for (int i = threadIdx.x; i < a.M * S; i += NT) deps[(long long)i * a.N + v] = (R)(dscale * dnew[i]);
+}
 #pragma clang fp contract(on)
 
 // ---------------------------------------------------------------------------------
@@ -631,10 +647,13 @@ __device__ void athermal_block_f(const AthArgs<R> &a, long long blk, int f, int 
         if (tid == 0) a.socp[blk] = 0.0;
         return;
     }
+    __shared__ long long fv[NT], fidx[NT];   // the block's flagged voxels in list order
     double dw = 0.0;
     if (f) {
         const long long idx = (long long)a.off[blk] + loff + before;
         a.list[idx] = (int)v;
+        fv[loff + before] = v;
+        fidx[loff + before] = idx;
         if (idx < a.cap) {
             const int mm = a.argm[v];
             R sg[C], st[5], d[C];
@@ -649,9 +668,12 @@ __device__ void athermal_block_f(const AthArgs<R> &a, long long blk, int f, int 
             for (int k = 0; k < S; ++k) st[k] = deps[((long long)mm * S + k) * a.N + v];
             expand<R, DIM>(st, d);
             dw = fabs((double)(contract<R, DIM>(sg, d) * (R)a.voxel * (R)1e-18 * (R)STZ_J_IN_EV));   // :66-67
-            transform_voxel<R, DIM>(a.ap, v, mm, a.dsrc + idx * (long long)a.ap.M * S, a.dscale);
+            transform_voxel_state<R, DIM>(a.ap, v, mm);   // reads the chosen strain before the redraw
         }
     }
+    __syncthreads();
+    for (int j = 0; j < tot; ++j)
+        if (fidx[j] < a.cap) redraw_voxel_wg<R, DIM>(a.ap, fv[j], a.dsrc + fidx[j] * (long long)a.ap.M * S, a.dscale);
     red[tid] = dw;
     __syncthreads();
     for (int s2 = NT / 2; s2 > 0; s2 >>= 1) {
