@@ -447,20 +447,45 @@ __device__ void transform_voxel(const ApplyArgs &a, long long v, int mm, const d
             deps[((long long)m * S + k) * a.N + v] = (R)(dscale * dnew[m * S + k]);   // :508-512
 }
 
-// the redraw of transform_voxel (the crystal-like barriers or the M*S strains) by the
-// NT threads of a workgroup, one value per thread: every load of the draws is in flight
-// at once instead of one load-store round trip per value in a single thread
+// The redraws of transform_voxel (the crystal-like barriers or the M*S strains) of nv
+// voxels vox[j] with draws dsrc + didx[j]*M*S (j < nv, didx[j] < cap), by the NT threads
+// of a workgroup: the (voxel, value) pairs are spread over the threads and each thread
+// issues 8 loads before its stores, instead of one load-store round trip per value.
 template <typename R, int DIM>
-__device__ void redraw_voxel_wg(const ApplyArgs &a, long long v, const double *dnew, double dscale)
+__device__ void redraw_wg(const ApplyArgs &a, const long long *vox, const long long *didx, int nv, long long cap,
+                          const double *dsrc, double dscale)
 {
-    constexpr int S = Comp<DIM>::S;
-    if (a.mro && a.mro[v]) {
-        R *q0 = (R *)a.q0;
-        for (int m = threadIdx.x; m < a.M; m += NT) q0[(long long)m * a.N + v] = (R)(a.mro_q0 * (1 - a.rratio + a.rratio * (dnew[m])));
-        return;
-    }
+    constexpr int S = Comp<DIM>::S, U = 8;
+    const int MS = a.M * S;
+    const int items = nv * MS;
     R *deps = (R *)a.deps;
-    for (int i = threadIdx.x; i < a.M * S; i += NT) deps[(long long)i * a.N + v] = (R)(dscale * dnew[i]);
+    R *q0 = (R *)a.q0;
+    for (int b0 = 0; b0 < items; b0 += U * NT) {
+        double dv[U];
+        int xt[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int it = b0 + u * NT + (int)threadIdx.x;
+            xt[u] = 0;
+            if (it < items && didx[it / MS] < cap) {
+                const long long v = vox[it / MS];
+                xt[u] = (a.mro && a.mro[v]) ? 2 : 1;
+                dv[u] = dsrc[didx[it / MS] * MS + it % MS];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int it = b0 + u * NT + (int)threadIdx.x;
+            if (xt[u] == 0) continue;
+            const long long v = vox[it / MS];
+            const int i = it % MS;
+            if (xt[u] == 2) {   // crystal-like site: M barriers from the first M draws
+                if (i < a.M) q0[(long long)i * a.N + v] = (R)(a.mro_q0 * (1 - a.rratio + a.rratio * (dv[u])));
+            } else {
+                deps[(long long)i * a.N + v] = (R)(dscale * dv[u]);   // :508-512
+            }
+        }
+    }
 }
 #pragma clang fp contract(on)
 
@@ -672,8 +697,7 @@ __device__ void athermal_block_f(const AthArgs<R> &a, long long blk, int f, int 
         }
     }
     __syncthreads();
-    for (int j = 0; j < tot; ++j)
-        if (fidx[j] < a.cap) redraw_voxel_wg<R, DIM>(a.ap, fv[j], a.dsrc + fidx[j] * (long long)a.ap.M * S, a.dscale);
+    redraw_wg<R, DIM>(a.ap, fv, fidx, tot, a.cap, a.dsrc, a.dscale);
     red[tid] = dw;
     __syncthreads();
     for (int s2 = NT / 2; s2 > 0; s2 >>= 1) {
