@@ -325,15 +325,26 @@ __device__ __forceinline__ double group_sum(const double *bsum, long long nb, lo
 __device__ double group_total(const double *gsum, long long ng, double *sh)
 {
     __shared__ double last;
+    constexpr int PRE = 8;            // chunks whose loads are issued before the first scan
     double carry = 0.0;
-    for (long long c0 = 0; c0 < ng; c0 += NT) {
-        const long long g = c0 + threadIdx.x;
-        const double x = g < ng ? gsum[g] : 0.0;
-        const double incl = block_scan(x, sh);
-        if (threadIdx.x == NT - 1) last = incl;
-        __syncthreads();
-        carry = (c0 == 0) ? last : carry + last;
-        __syncthreads();
+    for (long long c00 = 0; c00 < ng; c00 += (long long)NT * PRE) {
+        double xs[PRE];
+#pragma unroll
+        for (int i = 0; i < PRE; ++i) {
+            const long long g = c00 + (long long)i * NT + threadIdx.x;
+            xs[i] = g < ng ? gsum[g] : 0.0;
+        }
+#pragma unroll
+        for (int i = 0; i < PRE; ++i) {
+            const long long c0 = c00 + (long long)i * NT;
+            if (c0 < ng) {            // uniform over the workgroup
+                const double incl = block_scan(xs[i], sh);
+                if (threadIdx.x == NT - 1) last = incl;
+                __syncthreads();
+                carry = (c0 == 0) ? last : carry + last;
+                __syncthreads();
+            }
+        }
     }
     return carry;
 }
@@ -391,16 +402,27 @@ struct ApplyArgs {
 };
 
 #pragma clang fp contract(off)
-// everything but the redraw of the M*S new transformation strains
+// everything but the redraw of the M*S new transformation strains.  Every value is read
+// before anything is written (the arrays are distinct and each value is read once), so
+// the loads are in flight together rather than one round trip per read-modify-write.
 template <typename R, int DIM>
 __device__ void transform_voxel_state(const ApplyArgs &a, long long v, int mm)
 {
     constexpr int C = Comp<DIM>::C, S = Comp<DIM>::S;
-    R *deps = (R *)a.deps;
+    const R *deps = (const R *)a.deps;
     R *last = (R *)a.deps_last;
-    R st[5], d[C];
+    const R *sg = (const R *)a.sigma;
+    R st[5], d[C], sr[C];
+    double e0[C];
 #pragma unroll
     for (int k = 0; k < S; ++k) st[k] = deps[((long long)mm * S + k) * a.N + v];
+#pragma unroll
+    for (int c = 0; c < C; ++c) e0[c] = a.eps0[(long long)c * a.N + v];
+#pragma unroll
+    for (int c = 0; c < C; ++c) sr[c] = sg[(long long)c * a.N + v];
+    const double tl = a.tloc[v];
+    const double pm = a.perm[v];
+    const bool six = a.mro && a.mro[v] == 1;
 #pragma unroll
     for (int k = 0; k < S; ++k) last[(long long)k * a.N + v] = st[k];                  // PreDEps0 snapshot :369-374
     expand<R, DIM>(st, d);
@@ -408,19 +430,17 @@ __device__ void transform_voxel_state(const ApplyArgs &a, long long v, int mm)
 #pragma unroll
     for (int c = 0; c < C; ++c) dd[c] = (double)d[c];
 #pragma unroll
-    for (int c = 0; c < C; ++c) a.eps0[(long long)c * a.N + v] += dd[c];             // :377-379
+    for (int c = 0; c < C; ++c) a.eps0[(long long)c * a.N + v] = e0[c] + dd[c];      // :377-379
     {
         // local heating (STZ/elastic.c:381-383): |sigma:deps| * 1e9 J/m^3 * (1/Cp/rho)
-        const R *sg = (const R *)a.sigma;
         double sv[C];
 #pragma unroll
-        for (int c = 0; c < C; ++c) sv[c] = (double)sg[(long long)c * a.N + v] + a.soff[c];
+        for (int c = 0; c < C; ++c) sv[c] = (double)sr[c] + a.soff[c];
         const double dh = contract<double, DIM>(sv, dd) * 1E9;
-        a.tloc[v] += fabs(dh) * STZ_THERMAL_COEFF1;
+        a.tloc[v] = tl + fabs(dh) * STZ_THERMAL_COEFF1;
     }
     const double ds = contract<double, DIM>(dd, dd);                                   // :385-386
-    const bool six = a.mro && a.mro[v] == 1;
-    double pv = a.perm[v] + ds * (six ? STZ_PERM_SOFT_MRO : a.permsoft);               // :387-390
+    double pv = pm + ds * (six ? STZ_PERM_SOFT_MRO : a.permsoft);                      // :387-390
     if (pv > a.permcap) pv = a.permcap;                                               // :391
     a.perm[v] = pv;
     a.temp[v] = ds * (six ? STZ_TEMP_SOFT_MRO : a.tempsoft);                          // :393-396
