@@ -319,59 +319,88 @@ __global__ __launch_bounds__(NT) void kb_athermal(const RepDev *__restrict__ rep
 // --- host-loop arithmetic of the reference, on the device, with the host's operation order
 #pragma clang fp contract(off)
 
+// a[i] and a[i] += v for a runtime component i (the loaded component) as unrolled
+// selects: a private copy of the run state indexed at run time is placed in scratch
+// memory, and every access of the bookkeeping then is a memory round trip
+__device__ __forceinline__ double get6(const double *a, int i)
+{
+    double r = a[0];
+#pragma unroll
+    for (int c = 1; c < 6; ++c) {
+        double x = a[c];
+        asm volatile("" : "+v"(x));   // an opaque value: the select is not folded back into
+        r = (i == c) ? x : r;         // one load through a runtime address
+    }
+    return r;
+}
+__device__ __forceinline__ void add6(double *a, int i, double v)
+{
+#pragma unroll
+    for (int c = 0; c < 6; ++c)
+        if (c == i) a[c] += v;
+}
+
 // isotropicinv (STZ/elastic.c:522-531)
-__device__ void iso_inv(const RepDev &d, int C, const double *st, double *e)
+template <int C>
+__device__ __forceinline__ void iso_inv(const RepDev &d, const double *st, double *e)
 {
     const double lam = d.lambda, mu = d.mu;
-    if (C == 3) {
+    if constexpr (C == 3) {
         double tra = -lam / 2 / mu / (2 * lam + 2 * mu) * (st[0] + st[1]);
         e[0] = tra + st[0] / 2 / mu;
         e[1] = tra + st[1] / 2 / mu;
         e[2] = st[2] / 2 / mu;
     } else {
         double tra = -lam / 2 / mu / (3 * lam + 2 * mu) * ((st[0] + st[1]) + st[2]);
+#pragma unroll
         for (int i = 0; i < 3; ++i) e[i] = tra + st[i] / 2 / mu;
+#pragma unroll
         for (int i = 3; i < 6; ++i) e[i] = st[i] / 2 / mu;
     }
 }
 
 // Sigma[comp][*] += delta: FP64 defers it to the next rate pass, mixed keeps an offset
-__device__ void add_uniform_d(RepState *s, bool fp64, int comp, double delta)
+__device__ __forceinline__ void add_uniform_d(RepState *s, bool fp64, int comp, double delta)
 {
     if (fp64) {
         s->pend_comp = comp;
         s->pend_delta = delta;
-        s->usum[comp] += delta;      // in the field once the next rate pass applies it
+        add6(s->usum, comp, delta);  // in the field once the next rate pass applies it
     } else {
-        s->soff[comp] += delta;
+        add6(s->soff, comp, delta);
     }
 }
 
 // KMC loading after a thermal event (STZ/evolution.c:281-315)
-__device__ void load_thermal_d(const RepDev &d, RepState *s, int C, bool fp64, double t)
+template <int C>
+__device__ __forceinline__ void load_thermal_d(const RepDev &d, RepState *s, bool fp64, double t)
 {
     const double sr = d.strainrate, mu = d.mu;
     const int lc = d.lc;
     double ts[6] = {0, 0, 0, 0, 0, 0}, te[6];
     if (d.loading == STZ_SHEAR) {
+#pragma unroll
         for (int c = 0; c < C; ++c) ts[c] = (c == lc) ? 0 : s->stress[c];
-        iso_inv(d, C, ts, te);
+        iso_inv<C>(d, ts, te);
+#pragma unroll
         for (int c = 0; c < C; ++c)
             if (c != lc) s->epsavg[c] += -te[c];
-        s->epsavg[lc] += sr * t;
-        s->stress[lc] += 2 * mu * sr * t;
+        add6(s->epsavg, lc, sr * t);
+        add6(s->stress, lc, 2 * mu * sr * t);
         add_uniform_d(s, fp64, lc, 2 * mu * sr * t);
         return;
     }
     const double nu = d.nu, E = d.E;
+#pragma unroll
     for (int c = 1; c < C; ++c) ts[c] = s->stress[c];
-    iso_inv(d, C, ts, te);
+    iso_inv<C>(d, ts, te);
     s->epsavg[0] += (sr * t * 1 - te[0]);
     s->epsavg[1] += (sr * t * (-nu) - te[1]);
-    if (C == 3) {
+    if constexpr (C == 3) {
         s->epsavg[2] += (-te[2]);
     } else {
         s->epsavg[2] += (sr * t * (-nu) - te[2]);
+#pragma unroll
         for (int c = 3; c < 6; ++c) s->epsavg[c] += (-te[c]);
     }
     s->stress[0] += E * sr * t;
@@ -379,23 +408,25 @@ __device__ void load_thermal_d(const RepDev &d, RepState *s, int C, bool fp64, d
 }
 
 // pure elastic increment (STZ/evolution.c:321-358)
-__device__ void load_elastic_d(const RepDev &d, RepState *s, int C, bool fp64)
+template <int C>
+__device__ __forceinline__ void load_elastic_d(const RepDev &d, RepState *s, bool fp64)
 {
     const double sr = d.strainrate, inc = d.strainincrement, mu = d.mu;
     const int lc = d.lc;
     if (d.loading == STZ_SHEAR) {
-        s->epsavg[lc] += sr * 1;
-        s->stress[lc] += 2 * mu * inc;
+        add6(s->epsavg, lc, sr * 1);
+        add6(s->stress, lc, 2 * mu * inc);
         add_uniform_d(s, fp64, lc, 2 * mu * inc);
         return;
     }
     const double nu = d.nu, E = d.E;
     s->epsavg[0] += (inc * 1);
     s->epsavg[1] += (inc * (-nu));
-    if (C == 3) {
+    if constexpr (C == 3) {
         s->epsavg[2] += (inc * 0.0);
     } else {
         s->epsavg[2] += (inc * (-nu));
+#pragma unroll
         for (int c = 3; c < 6; ++c) s->epsavg[c] += (inc * 0.0);
     }
     s->stress[0] += E * inc;
@@ -403,23 +434,27 @@ __device__ void load_elastic_d(const RepDev &d, RepState *s, int C, bool fp64)
 }
 
 // athermal error subtraction, STZ/evolution.c:91-97 (reference) or loading-consistent
-__device__ void athermal_correction_d(const RepDev &d, RepState *s, int C)
+template <int C>
+__device__ __forceinline__ void athermal_correction_d(const RepDev &d, RepState *s)
 {
     double ts[6] = {0, 0, 0, 0, 0, 0}, te[6];
     if (d.athermal_correction == 0) {
+#pragma unroll
         for (int c = 1; c < C; ++c) ts[c] = s->stress[c];
     } else {
+#pragma unroll
         for (int c = 0; c < C; ++c) ts[c] = (c == d.lc) ? 0.0 : s->stress[c];
     }
-    iso_inv(d, C, ts, te);
+    iso_inv<C>(d, ts, te);
+#pragma unroll
     for (int c = 0; c < C; ++c) s->epsavg[c] -= te[c];
 }
 
-__device__ bool terminated_d(const RepDev &d, const RepState *s)
+__device__ __forceinline__ bool terminated_d(const RepDev &d, const RepState *s)
 {
-    if (fabs(s->epsavg[d.lc]) > s->endstrain) return true;                        // bmg.c:383,397
+    if (fabs(get6(s->epsavg, d.lc)) > s->endstrain) return true;                    // bmg.c:383,397
     if (d.loading == STZ_TENSION && s->stress[0] < -0.5) return true;               // bmg.c:388
-    if (d.loading == STZ_SHEAR && fabs(s->stress[d.lc]) < STZ_EPSILON) return true; // bmg.c:402
+    if (d.loading == STZ_SHEAR && fabs(get6(s->stress, d.lc)) < STZ_EPSILON) return true; // bmg.c:402
     return false;
 }
 
@@ -510,7 +545,7 @@ __global__ __launch_bounds__(STZ_NT_MEAN) void kb_finish(const RepDev *__restric
         r.site = (int)res[10];
         r.mode = (int)res[11];
         r.q_chosen = res[12];
-        load_thermal_d(d, &s, C, fp64 != 0, t);
+        load_thermal_d<C>(d, &s, fp64 != 0, t);
         s.totime += t;                        // STZ/evolution.c:317
     } else if (kind == KIND_ATHERMAL) {
         s.astep++;                            // athermal_plasticity (STZ/evolution.c:47-100)
@@ -518,7 +553,7 @@ __global__ __launch_bounds__(STZ_NT_MEAN) void kb_finish(const RepDev *__restric
         s.ntrans += n;
         s.avlan += n;
         s.soc_elast += soc;
-        athermal_correction_d(d, &s, C);
+        athermal_correction_d<C>(d, &s);
         r.kind = STZ_STEP_ATHERMAL;
         r.n_transformed = (int)n;
         r.soc_elast = soc;
@@ -528,10 +563,10 @@ __global__ __launch_bounds__(STZ_NT_MEAN) void kb_finish(const RepDev *__restric
         r.dt_kmc = 1 / R;
         s.estep++;
         s.totime += d.dt;                     // STZ/evolution.c:324
-        load_elastic_d(d, &s, C, fp64 != 0);
+        load_elastic_d<C>(d, &s, fp64 != 0);
     }
     if (s.printperiod > 0) {
-        const double cur = s.epsavg[d.lc];
+        const double cur = get6(s.epsavg, d.lc);
         if (fabs(cur - s.strainhist_last) >= (s.printperiod * d.strainincrement * 0.99)) {
             s.strainhist_last = cur;          // strainhist[N_print-1], STZ/bmg.c:193-201
             r.printed = 1;
