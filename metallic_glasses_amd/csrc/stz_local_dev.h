@@ -155,14 +155,14 @@ __device__ __forceinline__ void local_sum(const LocalGeom &g, const R *__restric
         if (d1 < 0) d1 += g.n[1];
         if (d2 < 0) d2 += g.n[2];
         const int hj = half_index(g, d0, d1, d2);
-        R kv[S * C];   // the source's response values, all loads issued before the first use
-#pragma unroll
-        for (int i = 0; i < S * C; ++i) kv[i] = gload(K + (long long)i * g.H + hj);
 #pragma unroll
         for (int k = 0; k < S; ++k) {
+            R kv[C];   // the C response values of strain component k, loaded before their use
+#pragma unroll
+            for (int c = 0; c < C; ++c) kv[c] = gload(K + ((long long)k * C + c) * g.H + hj);
             const R dk = src[j].d[k];
 #pragma unroll
-            for (int c = 0; c < C; ++c) out[c] = out[c] + dk * kv[k * C + c];
+            for (int c = 0; c < C; ++c) out[c] = out[c] + dk * kv[c];
         }
     }
 }
