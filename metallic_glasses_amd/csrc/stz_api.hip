@@ -617,6 +617,8 @@ int init_fields(stz_ctx *ctx, const std::vector<double> &theta)
         if (rc) return rc;
         STZ_HIP(hipStreamSynchronize(ctx->stream));
     }
+    const int rc = invalidate_cache(ctx);   // new Q0 and strains
+    if (rc) return rc;
     for (int c = 0; c < 6; ++c) ctx->epsavg[c] = 0.0;
     return solve(ctx);   // STZ/bmg.c:117-127
 }
@@ -779,7 +781,11 @@ int stz_set_scalars(stz_ctx *ctx, const double epsavg[6], const double stress[6]
         if (epsavg) ctx->epsavg[c] = epsavg[c];
         if (stress) ctx->stress[c] = stress[c];
     }
-    ctx->totime = totime;
+    if (totime != ctx->totime) {
+        ctx->totime = totime;
+        const int rc = invalidate_cache(ctx);   // settled softening entries assume a clock that only advances
+        if (rc) return rc;
+    }
     return STZ_OK;
 }
 

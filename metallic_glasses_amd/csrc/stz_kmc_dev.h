@@ -90,6 +90,7 @@ struct VoxelState {
     R fac;          // crystal-like MRO: factor on the modifier and the softening per mode
     int t;          // MRO type (0 glass)
     bool fresh;     // the cached Q0*exp(-D*S) of this voxel is valid for this step's S
+    bool settle;    // the entry is written as settled (its temporary softening term is 0)
 };
 
 // NO_SF_XTAL_MRO factors (STZ/init.c:645-667): six-fold 0.1, two-fold 0.01, ten-fold 0.9
@@ -104,14 +105,34 @@ __device__ __forceinline__ R mro_factor(int t)
 // voxel) with the S it was computed for; S equals Perm bit for bit except shortly after
 // a transform, so almost every voxel takes the cached value, which is the bit-identical
 // result of the same expression.
+//
+// The entry scache[v] is "settled" when its sign bit is set (and it is not a NaN): then
+// S = -scache[v], the temporary term Temp*exp((tlast - t)/tTemp) was exactly 0 when it
+// was written and stays 0 (the clock only advances; a transform, an upload, a state
+// initialisation, a checkpoint load or a clock reset invalidates the entry), and Perm,
+// Temp and tlast are not read.  Otherwise S is computed and the entry is fresh when it
+// holds that S.  An S with its sign bit set (a negative Perm upload) is never cached.
 template <typename R, int DIM, bool XTAL = false>
 __device__ __forceinline__ void load_voxel(const RateArgs<R> &a, long long v, bool apply_pending, VoxelState<R> &s)
 {
     constexpr int C = Comp<DIM>::C, S = Comp<DIM>::S;
-    double soft = 0.0;
-    if (a.softening)
-        soft = ld_stream<2>(a.perm + v) + ld_stream<2>(a.temp + v) * exp((ld_stream<2>(a.tlast + v) - a.totime) / a.tTemp);   // :731
-    s.soft = (R)soft;
+    const R sc = ld_stream<2>(a.scache + v);
+    s.settle = false;
+    if (__builtin_signbit(sc) && sc == sc) {
+        s.soft = -sc;
+        s.fresh = true;
+    } else {
+        double soft = 0.0;
+        bool tzero = true;
+        if (a.softening) {
+            const double tt = ld_stream<2>(a.temp + v) * exp((ld_stream<2>(a.tlast + v) - a.totime) / a.tTemp);
+            soft = ld_stream<2>(a.perm + v) + tt;                                            // :731
+            tzero = (tt == 0.0);
+        }
+        s.soft = (R)soft;
+        s.fresh = (sc == s.soft);                     // false for the NaN of an invalidated entry
+        s.settle = tzero && !__builtin_signbit(s.soft);
+    }
 #pragma unroll
     for (int c = 0; c < C; ++c) {
         R x = ld_stream<2>(a.sigma + (long long)c * a.N + v);
@@ -132,7 +153,6 @@ __device__ __forceinline__ void load_voxel(const RateArgs<R> &a, long long v, bo
         s.t = 0;
         s.fac = (R)1;
     }
-    s.fresh = (ld_stream<2>(a.scache + v) == s.soft);   // false for the NaN of an invalidated entry
     // one address select per voxel (a select between two struct fields inside the mode
     // loop is compiled through private memory)
     const uintptr_t qa = (uintptr_t)a.q0eff, qb = (uintptr_t)a.q0;
@@ -286,7 +306,9 @@ __device__ __forceinline__ void rates_block(const RateArgs<R> &a, long long blk)
     if (valid) {
         a.minq[v] = minq;
         a.argm[v] = (uint8_t)am;
-        if (!s.fresh) a.scache[v] = s.soft;
+        // a settled entry holds -S; an S with its sign bit set is not cached (NaN)
+        if (!s.fresh || s.settle)
+            a.scache[v] = s.settle ? -s.soft : (__builtin_signbit(s.soft) ? (R)__builtin_nan("") : s.soft);
     }
     __syncthreads();
     if (tid < a.M) {
