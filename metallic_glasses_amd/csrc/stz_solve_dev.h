@@ -688,34 +688,31 @@ __global__ __launch_bounds__(NTF) void k_fused(cpx<R> *__restrict__ spec_a, cons
         R g[3] = {(R)gd[0], (R)gd[1], (R)gd[2]};
         R gg[6];
         products<R, DIM>(g, gg);
-        R er[C], ei[C], orr[C], oi[C];
+        // the real parts, then the imaginary parts (Gamma-hat acts on each alone): half
+        // the live values of both at once, which set the kernel's register count
+        R *bufr = reinterpret_cast<R *>(buf);
+#pragma unroll 1
+        for (int part = 0; part < 2; ++part) {
+            R e[C], o[C];
 #pragma unroll
-        for (int c = 0; c < C; ++c) {
-            const cpx<R> x = buf[p * NL + c * W + col];
-            er[c] = x.x;
-            ei[c] = x.y;
-        }
-        gamma_part<R, C>(er, g, gg, lam, mu, s0, sh, sa, orr);
-        gamma_part<R, C>(ei, g, gg, lam, mu, s0, sh, sa, oi);
-        if (nyq) {
-            // average with the mirror bin's unit vector (Nyquist components negated)
-            R gm[3] = {g[0], g[1], g[2]};
+            for (int c = 0; c < C; ++c) e[c] = bufr[2 * (p * NL + c * W + col) + part];
+            gamma_part<R, C>(e, g, gg, lam, mu, s0, sh, sa, o);
+            if (nyq) {
+                // average with the mirror bin's unit vector (Nyquist components negated)
+                R gm[3] = {g[0], g[1], g[2]};
 #pragma unroll
-            for (int a = 0; a < DIM; ++a)
-                if (2 * idx[a] == k.n[a]) gm[a] = -gm[a];
-            R ggm[6];
-            products<R, DIM>(gm, ggm);
-            R o2r[C], o2i[C];
-            gamma_part<R, C>(er, gm, ggm, lam, mu, s0, sh, sa, o2r);
-            gamma_part<R, C>(ei, gm, ggm, lam, mu, s0, sh, sa, o2i);
+                for (int a = 0; a < DIM; ++a)
+                    if (2 * idx[a] == k.n[a]) gm[a] = -gm[a];
+                R ggm[6];
+                products<R, DIM>(gm, ggm);
+                R o2[C];
+                gamma_part<R, C>(e, gm, ggm, lam, mu, s0, sh, sa, o2);
 #pragma unroll
-            for (int c = 0; c < C; ++c) {
-                orr[c] = (orr[c] + o2r[c]) * (R)0.5;
-                oi[c] = (oi[c] + o2i[c]) * (R)0.5;
+                for (int c = 0; c < C; ++c) o[c] = (o[c] + o2[c]) * (R)0.5;
             }
-        }
 #pragma unroll
-        for (int c = 0; c < C; ++c) buf[p * NL + c * W + col] = {orr[c], oi[c]};
+            for (int c = 0; c < C; ++c) bufr[2 * (p * NL + c * W + col) + part] = o[c];
+        }
     }
     }
     __syncthreads();
