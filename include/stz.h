@@ -146,8 +146,15 @@ int stz_init_state_mro(stz_ctx *ctx, const int *type, const double *theta);
 int stz_init_state_device(stz_ctx *ctx, unsigned long long seed, const double *spheres, int nsph, double inc_mean,
                           double inc_std, long long *inside);
 
+/* Field transfer (host buffers of doubles, read or written only during the call).  An
+   upload of STZ_F_Q0, _DEPS, _DEPS_LAST, _PERM, _TEMP or _TLAST discards the rate pass's cached
+   Q0*exp(-D*S) values, which are recomputed by the next rate pass. */
 int stz_upload(stz_ctx *ctx, int field, const double *host);
 int stz_download(stz_ctx *ctx, int field, double *host);
+/* The host scalars EpsAvg, Stress (epsavg/stress may be NULL to keep them) and totime.
+   A new totime also discards the cached Q0*exp(-D*S) values: the rate pass keeps a
+   voxel's cached softening without re-reading Perm/Temp/tlast once its temporary term
+   is zero, which holds only while the clock advances. */
 int stz_set_scalars(stz_ctx *ctx, const double epsavg[6], const double stress[6], double totime);
 int stz_get_scalars(stz_ctx *ctx, double epsavg[6], double stress[6], double *totime);
 
