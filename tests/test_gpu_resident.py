@@ -136,7 +136,7 @@ def test_resident_termination_and_stop_at_print():
         _same_record(a, b, i)
 
 
-@pytest.mark.parametrize("name", ["shear3d_64", "ref2d_512"])
+@pytest.mark.parametrize("name", ["shear3d_64", "ref2d_512", "shear3d_128"])
 def test_resident_replays_golden_trace(name):
     path = os.path.join(GOLD, f"trace_{name}.npz")
     if not os.path.exists(path):
