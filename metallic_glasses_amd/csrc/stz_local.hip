@@ -71,7 +71,7 @@ KKey key_of(const stz_ctx *ctx)
     return KKey(ctx->dim, ctx->n[0], ctx->n[1], ctx->n[2], ctx->prec, ctx->k.lambda, ctx->k.mu);
 }
 
-LocalGeom geom_of(const stz_ctx *ctx) { return local_geom(ctx->dim, ctx->n); }
+LocalGeom geom_of(const stz_ctx *ctx) { return local_geom(ctx->dim, ctx->n, local_row_voxels(ctx->prec == STZ_MIXED)); }
 
 #define LHIP(call)                                                                 \
     do {                                                                           \
@@ -285,6 +285,6 @@ int stz_local_launch(stz_ctx *ctx, int ns, const double *uni)
 // rows of the half space, at most n1*(n2/2 + 1) <= n1*n2, the rows of the partial array
 long long stz_local_blocks(const stz_ctx *ctx)
 {
-    const LocalGeom g = local_geom(ctx->dim, ctx->n);
+    const LocalGeom g = geom_of(ctx);
     return (g.items + g.ipw - 1) / g.ipw;
 }

@@ -33,10 +33,19 @@
 
 namespace {
 
-// voxels along x per half-space row group of one workgroup (each updates two grid rows)
+// voxels along x per half-space row group of one workgroup (each updates two grid rows),
+// FP64 and FP32 storage (measured at 128^3: 256/512/1024 within 2 % in FP64, 1024 7 %
+// faster than 512 in FP32)
 #ifndef STZ_LOCAL_ROW_VOXELS
 #define STZ_LOCAL_ROW_VOXELS 512
 #endif
+#ifndef STZ_LOCAL_ROW_VOXELS_FP32
+#define STZ_LOCAL_ROW_VOXELS_FP32 1024
+#endif
+__host__ __device__ constexpr int local_row_voxels(bool fp32)
+{
+    return fp32 ? STZ_LOCAL_ROW_VOXELS_FP32 : STZ_LOCAL_ROW_VOXELS;
+}
 
 struct LocalGeom {
     int n[3];          // 3-D: the grid; 2-D: (n0, 1, n1)
@@ -47,7 +56,7 @@ struct LocalGeom {
     int items;         // offset rows n[1]*h2
 };
 
-__host__ __device__ inline LocalGeom local_geom(int dim, const int *n)
+__host__ __device__ inline LocalGeom local_geom(int dim, const int *n, int row_voxels)
 {
     LocalGeom g;
     g.n[0] = n[0];
@@ -56,7 +65,7 @@ __host__ __device__ inline LocalGeom local_geom(int dim, const int *n)
     g.N = g.n[0] * g.n[1] * g.n[2];
     g.h2 = g.n[2] / 2 + 1;
     g.H = g.n[0] * g.n[1] * g.h2;
-    g.ipw = g.n[0] >= STZ_LOCAL_ROW_VOXELS ? 1 : STZ_LOCAL_ROW_VOXELS / g.n[0];
+    g.ipw = g.n[0] >= row_voxels ? 1 : row_voxels / g.n[0];
     g.items = g.n[1] * g.h2;
     return g;
 }

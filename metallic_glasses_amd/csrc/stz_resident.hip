@@ -243,7 +243,7 @@ __global__ __launch_bounds__(NT_LOCAL) void kb_local(const RepDev *__restrict__ 
     const RepState *st = d.st;
     double *res = d.res;
     if (st->halt || reinterpret_cast<const int *>(res + RES_LOCAL)[0] != 1) return;
-    const LocalGeom g = local_geom(DIM, d.n);
+    const LocalGeom g = local_geom(DIM, d.n, local_row_voxels(sizeof(R) == 4));
     const bool thermal = reinterpret_cast<const int *>(res + RES_KIND)[0] == KIND_THERMAL;
     const int ns = thermal ? 1 : (int)res[13];
     for (int s = threadIdx.x; s < ns; s += NT_LOCAL) {
