@@ -5,15 +5,20 @@ Workload (default): 128x128x128 voxels, FP64 plain GPU path, mu = 35 GPa, nu = 0
 20 shear-transformation modes, gamma0 = 0.1, Q0 ~ N(2.0, 0.2) eV from seed 1234,
 T = 300 K, simple shear at 1e-4/s.  A step is one iteration of the reference main loop
 (rates over all modes and voxels, then a thermal event with selection, eigenstrain
-update and a full Green's-function stress solve, or an athermal avalanche, or a pure
-elastic increment).  Warm-up runs the elastic loading until W thermal events have
-happened; the timed region is exactly K steps; `value` = thermal events / wall time.
+update and stress update, or an athermal avalanche, or a pure elastic increment).
+Warm-up runs the elastic loading until W thermal events have happened; the timed region
+is exactly K device-resident steps; `value` = thermal events / wall time with the
+event-local Green's update (full FFT re-solves at print steps, for large avalanches and
+every --resync updates).
 
-Also reported: the mixed-precision path (FP32 FFT/Green/rates, FP64 sums, clock,
+Also reported: the same K steps with the full FFT solve after every transform (the
+reference's algorithm); the mixed-precision path (FP32 FFT/Green/rates, FP64 sums, clock,
 eigenstrain and mean stress) from the same state with its stress-field error against
-FP64; the roofline of the dominant kernel (k_rates, HBM-bound) from HIP events on the
-library's stream; the CPU FP64 oracle timed on one host core on a bounded sample that
-starts from the GPU's timed-region state.
+FP64; the roofline of the dominant kernel (kb_rates, HBM-bound) from HIP events on the
+library's stream; the CPU FP64 oracle on the host's threads on a bounded sample that
+starts from the GPU's timed-region state; the 64-replica ensemble; per-voxel moduli.
+--large 256 | 512 instead measures 256^3 or 512x512x256 with a soft inclusion, mixed
+against FP64.
 """
 import argparse
 import json
@@ -47,6 +52,9 @@ def parse():
     ap.add_argument("--ensemble-replicas", type=int, default=64)
     ap.add_argument("--ensemble-steps", type=int, default=200)
     ap.add_argument("--ensemble-warmup", type=int, default=200)
+    ap.add_argument("--large", choices=sorted(LARGE), default=None,
+                    help="instead of the default workload: 256 = 256^3 mixed vs FP64, Q0 std 0.25; "
+                         "512 = 512x512x256 with a 5%% soft inclusion")
     ap.add_argument("--pmc-target", default=None,
                     help="comma list of rates,copy: run only those kernels (a few launches) for a "
                          "rocprofv3 --pmc pass, then exit")
@@ -283,6 +291,9 @@ def main():
     import metallic_glasses_amd as mg
     if a.pmc_target:
         pmc_target(a, mg)
+        return
+    if a.large:
+        large_leg(a, mg)
         return
 
     dims = tuple(int(x) for x in a.grid.split(","))
@@ -537,6 +548,100 @@ def inhom_leg(a, mg, kw, snap, dims, nsolve=10, nsteps=40):
 
 ENSEMBLE_GRIDS = [(32, 32, 32), (64, 64, 128), (96, 96, 96), (128, 128, 128)]
 ENSEMBLE_GRAPH_STEPS = 4          # ensemble steps per captured hipGraph (shape classes as parallel branches)
+
+
+LARGE = {
+    "256": dict(n=(256, 256, 256), q0_std=0.25, spheres=None),
+    "512": dict(n=(512, 512, 256), q0_std=0.2, spheres=(0.05, 16.0, 1.6, 0.2)),
+}
+
+
+def large_leg(a, mg):
+    """A large single grid, mixed against FP64: 256^3 with Q0 ~ N(2.0, 0.25), or 512x512x256
+    with Q0 ~ N(2.0, 0.2) plus a 5 % soft inclusion (spheres of radius 16 voxels, Q0 ~
+    N(1.6, 0.2) inside).  Both contexts start from the same device-generated state (seed
+    1234), stay resident together and use the event-local update.  Each runs resident
+    warm-up steps until 20 thermal events, then exactly K timed resident steps.  Reported:
+    thermal events/s of each, the mixed/FP64 ratio, the SS-curve error over the steps whose
+    sequence agrees, the single-solve stress error of the FP64 end state, and kb_rates of
+    each precision (HIP events on the library stream) against the HBM peak."""
+    cfg = LARGE[a.large]
+    dims = cfg["n"]
+    kw = mg.config_3d(n=dims, seed=a.seed, q0_std=cfg["q0_std"])
+    di = dict(seed=a.seed)
+    if cfg["spheres"]:
+        frac, rad, imean, istd = cfg["spheres"]
+        di.update(spheres=mg.soft_inclusions(dims, fraction=frac, radius=rad, seed=a.seed),
+                  inc_mean=imean, inc_std=istd)
+    ctx = {}
+    for name, prec in (("fp64", mg.FP64), ("mixed", mg.MIXED)):
+        t0 = time.perf_counter()
+        g = mg.Stz(init_state=False, device_init=di, **dict(kw, precision=prec))
+        g.set_print(5, 1.0)
+        tune = g.autotune(reps=5)
+        g.set_solver(SOLVERS["local"], a.resync, a.local_max)
+        g.run_config(use_graph=0)
+        ctx[name] = g
+        print(f"# {name} {dims}: init + autotune {time.perf_counter() - t0:.1f}s, tiles {tune[0]}, "
+              f"inclusion voxels {g.inclusion_voxels}", file=sys.stderr, flush=True)
+    curves = {}
+    legs = {}
+    for name, g in ctx.items():
+        curve = []
+        nst = nev = 0
+        while nev < 20 and nst < 2000:
+            _, kinds, _, _, _ = run_steps(g, 50, curve)
+            nst += 50
+            nev += int(kinds[1])
+        dt, kinds, _, printed, _ = run_steps(g, a.steps, curve)
+        curves[name] = curve
+        g.timing(True)
+        run_steps(g, a.timing_steps)
+        ms, n, _, _ = g.timing_get()
+        g.timing(False)
+        b = 8 if name == "fp64" else 4
+        k4_ms = ms[0] / max(n[0], 1)
+        k4_bytes = rate_bytes_per_voxel(g.C, g.M, g.S, b) * g.N
+        legs[name] = {"value": round(float(kinds[1] / dt), 3), "ms_per_step": round(dt * 1e3 / a.steps, 4),
+                      "warmup_steps": nst, "steps_by_kind": {"elastic": int(kinds[0]), "thermal": int(kinds[1]),
+                                                            "athermal": int(kinds[2]), "printed": int(printed)},
+                      "kb_rates": {"avg_ms": round(k4_ms, 5), "bytes_per_launch": int(k4_bytes),
+                                   "achieved_GBs": round(k4_bytes / (k4_ms * 1e-3) / 1e9, 1),
+                                   "frac": round(k4_bytes / (k4_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}}
+        print(f"# {name}: {legs[name]}", file=sys.stderr, flush=True)
+    n_same = same_prefix(curves["fp64"], curves["mixed"])
+    f, m = ctx["fp64"], ctx["mixed"]
+    m.upload(mg.F_EPS0, f.download(mg.F_EPS0))
+    e, s, t = f.scalars()
+    m.set_scalars(e, s, t)
+    f.stress_solve()
+    m.stress_solve()
+    sa, sb = f.download(mg.F_SIGMA), m.download(mg.F_SIGMA)
+    err1 = float(np.linalg.norm(sb - sa) / np.linalg.norm(sa))
+    del sa, sb
+    incl = f.inclusion_voxels
+    f.close()
+    m.close()
+    mx, fp = legs["mixed"], legs["fp64"]
+    out = {
+        "metric": "kMC events/s on 1 GPU, mixed vs FP64 at fixed stress err; FFT HBM roofline %",
+        "value": mx["value"], "unit": "events/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": mx["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32/f64 mixed", "data": "synthetic (device-generated seeded initial state)",
+        "config": {"workload": f"STZ kMC {dims[0]}x{dims[1]}x{dims[2]} mixed, M=20, simple shear, seed {a.seed}, "
+                               f"Q0 ~ N(2.0, {cfg['q0_std']})" +
+                               (f", soft inclusion {incl / f.N:.4f} of voxels with Q0 ~ N(1.6, 0.2)" if incl else "") +
+                               "; event-local Green's update",
+                   "grid": list(dims), "modes": 20, "precision": "mixed", "solver": "local",
+                   "parallelism": "single GPU"},
+        "mixed": mx, "fp64": fp, "mixed_over_fp64": round(mx["value"] / fp["value"], 3),
+        "steps_identical": n_same, "ss_curve_rel_err": curve_err(curves["fp64"], curves["mixed"], n_same),
+        "single_solve_stress_rel_err": err1,
+        "roofline": {"bound": "hbm", "kernel": "kb_rates", "achieved": mx["kb_rates"]["achieved_GBs"],
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": mx["kb_rates"]["frac"], "traffic": None,
+                     "bytes_per_launch": mx["kb_rates"]["bytes_per_launch"], "avg_ms": mx["kb_rates"]["avg_ms"]},
+    }
+    print(json.dumps(out))
 
 
 def ensemble_plan(nrep):
