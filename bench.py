@@ -559,12 +559,14 @@ LARGE = {
 def large_leg(a, mg):
     """A large single grid, mixed against FP64: 256^3 with Q0 ~ N(2.0, 0.25), or 512x512x256
     with Q0 ~ N(2.0, 0.2) plus a 5 % soft inclusion (spheres of radius 16 voxels, Q0 ~
-    N(1.6, 0.2) inside).  Both contexts start from the same device-generated state (seed
-    1234), stay resident together and use the event-local update.  Each runs resident
-    warm-up steps until 20 thermal events, then exactly K timed resident steps.  Reported:
-    thermal events/s of each, the mixed/FP64 ratio, the SS-curve error over the steps whose
-    sequence agrees, the single-solve stress error of the FP64 end state, and kb_rates of
-    each precision (HIP events on the library stream) against the HBM peak."""
+    N(1.6, 0.2) inside).  Three contexts start from the same device-generated state (seed
+    1234): the plain FP64 path with the full FFT solve after every transform (run and
+    closed first), then FP64 and mixed with the event-local update, resident together.
+    Each runs resident warm-up steps until 20 thermal events, then exactly K timed resident
+    steps.  Reported: thermal events/s of each, the mixed/FP64 ratios, the SS-curve error
+    over the steps whose sequence agrees, the single-solve stress error of the FP64 end
+    state, and kb_rates of each context (HIP events on the library stream) against the HBM
+    peak."""
     cfg = LARGE[a.large]
     dims = cfg["n"]
     kw = mg.config_3d(n=dims, seed=a.seed, q0_std=cfg["q0_std"])
@@ -573,20 +575,18 @@ def large_leg(a, mg):
         frac, rad, imean, istd = cfg["spheres"]
         di.update(spheres=mg.soft_inclusions(dims, fraction=frac, radius=rad, seed=a.seed),
                   inc_mean=imean, inc_std=istd)
-    ctx = {}
-    for name, prec in (("fp64", mg.FP64), ("mixed", mg.MIXED)):
+    def make(name, prec, solver):
         t0 = time.perf_counter()
         g = mg.Stz(init_state=False, device_init=di, **dict(kw, precision=prec))
         g.set_print(5, 1.0)
         tune = g.autotune(reps=5)
-        g.set_solver(SOLVERS["local"], a.resync, a.local_max)
+        g.set_solver(SOLVERS[solver], a.resync, a.local_max)
         g.run_config(use_graph=0)
-        ctx[name] = g
         print(f"# {name} {dims}: init + autotune {time.perf_counter() - t0:.1f}s, tiles {tune[0]}, "
               f"inclusion voxels {g.inclusion_voxels}", file=sys.stderr, flush=True)
-    curves = {}
-    legs = {}
-    for name, g in ctx.items():
+        return g
+
+    def run(name, g, b):
         curve = []
         nst = nev = 0
         while nev < 20 and nst < 2000:
@@ -599,7 +599,6 @@ def large_leg(a, mg):
         run_steps(g, a.timing_steps)
         ms, n, _, _ = g.timing_get()
         g.timing(False)
-        b = 8 if name == "fp64" else 4
         k4_ms = ms[0] / max(n[0], 1)
         k4_bytes = rate_bytes_per_voxel(g.C, g.M, g.S, b) * g.N
         legs[name] = {"value": round(float(kinds[1] / dt), 3), "ms_per_step": round(dt * 1e3 / a.steps, 4),
@@ -609,6 +608,17 @@ def large_leg(a, mg):
                                    "achieved_GBs": round(k4_bytes / (k4_ms * 1e-3) / 1e9, 1),
                                    "frac": round(k4_bytes / (k4_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}}
         print(f"# {name}: {legs[name]}", file=sys.stderr, flush=True)
+
+    curves = {}
+    legs = {}
+    # the plain FP64 path (full FFT solve after every transform) first, on a context of its own
+    # that is closed before the other two exist
+    g = make("fp64_fft", mg.FP64, "fft")
+    run("fp64_fft", g, 8)
+    g.close()
+    ctx = {"fp64": make("fp64", mg.FP64, "local"), "mixed": make("mixed", mg.MIXED, "local")}
+    for name, g in ctx.items():
+        run(name, g, 8 if name == "fp64" else 4)
     n_same = same_prefix(curves["fp64"], curves["mixed"])
     f, m = ctx["fp64"], ctx["mixed"]
     m.upload(mg.F_EPS0, f.download(mg.F_EPS0))
@@ -634,7 +644,10 @@ def large_leg(a, mg):
                                "; event-local Green's update",
                    "grid": list(dims), "modes": 20, "precision": "mixed", "solver": "local",
                    "parallelism": "single GPU"},
-        "mixed": mx, "fp64": fp, "mixed_over_fp64": round(mx["value"] / fp["value"], 3),
+        "mixed": mx, "fp64": fp, "fp64_fft_every_event": legs["fp64_fft"],
+        "mixed_over_fp64": round(mx["value"] / fp["value"], 3),
+        "mixed_over_fp64_fft": round(mx["value"] / legs["fp64_fft"]["value"], 3),
+        "fp64_fft_steps_identical_to_local": same_prefix(curves["fp64_fft"], curves["fp64"]),
         "steps_identical": n_same, "ss_curve_rel_err": curve_err(curves["fp64"], curves["mixed"], n_same),
         "single_solve_stress_rel_err": err1,
         "roofline": {"bound": "hbm", "kernel": "kb_rates", "achieved": mx["kb_rates"]["achieved_GBs"],
