@@ -562,8 +562,8 @@ def large_leg(a, mg):
     N(1.6, 0.2) inside).  Three contexts start from the same device-generated state (seed
     1234): the plain FP64 path with the full FFT solve after every transform (run and
     closed first), then FP64 and mixed with the event-local update, resident together.
-    Each runs resident warm-up steps until 20 thermal events, then exactly K timed resident
-    steps.  Reported: thermal events/s of each, the mixed/FP64 ratios, the SS-curve error
+    Each runs resident warm-up steps (50 at a time) until W thermal events, then exactly K
+    timed resident steps.  Reported: thermal events/s of each, the mixed/FP64 ratios, the SS-curve error
     over the steps whose sequence agrees, the single-solve stress error of the FP64 end
     state, and kb_rates of each context (HIP events on the library stream) against the HBM
     peak."""
@@ -589,7 +589,7 @@ def large_leg(a, mg):
     def run(name, g, b):
         curve = []
         nst = nev = 0
-        while nev < 20 and nst < 2000:
+        while nev < a.warmup and nst < 2000:
             _, kinds, _, _, _ = run_steps(g, 50, curve)
             nst += 50
             nev += int(kinds[1])
