@@ -565,8 +565,9 @@ def large_leg(a, mg):
     Each runs resident warm-up steps (50 at a time) until W thermal events, then exactly K
     timed resident steps.  Reported: thermal events/s of each, the mixed/FP64 ratios, the SS-curve error
     over the steps whose sequence agrees, the single-solve stress error of the FP64 end
-    state, and kb_rates of each context (HIP events on the library stream) against the HBM
-    peak."""
+    state, kb_rates of each context (HIP events on the library stream) against the HBM
+    peak, and the five FFT solve kernels of the FP64 and mixed contexts (HIP events over a
+    window of host-driven steps) against their algorithmic bytes."""
     cfg = LARGE[a.large]
     dims = cfg["n"]
     kw = mg.config_3d(n=dims, seed=a.seed, q0_std=cfg["q0_std"])
@@ -607,6 +608,16 @@ def large_leg(a, mg):
                       "kb_rates": {"avg_ms": round(k4_ms, 5), "bytes_per_launch": int(k4_bytes),
                                    "achieved_GBs": round(k4_bytes / (k4_ms * 1e-3) / 1e9, 1),
                                    "frac": round(k4_bytes / (k4_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}}
+        if name != "fp64_fft":
+            # host-driven steps (full FFT solve on every solved step), events around each
+            # solve kernel
+            g.timing(True)
+            for _ in range(max(a.timing_steps // 10, 5)):
+                if g.step()[0]:
+                    break
+            _, _, sms, sn = g.timing_get()
+            g.timing(False)
+            legs[name]["fft_roofline"] = fft_table(dims, g.C, g.N, b, sms, sn)
         print(f"# {name}: {legs[name]}", file=sys.stderr, flush=True)
 
     curves = {}
