@@ -148,7 +148,9 @@ int stz_init_state_device(stz_ctx *ctx, unsigned long long seed, const double *s
 
 /* Field transfer (host buffers of doubles, read or written only during the call).  An
    upload of STZ_F_Q0, _DEPS, _DEPS_LAST, _PERM, _TEMP or _TLAST discards the rate pass's cached
-   Q0*exp(-D*S) values, which are recomputed by the next rate pass. */
+   Q0*exp(-D*S) values, which are recomputed by the next rate pass.  An upload of STZ_F_Q0
+   also sets the stress-free barriers Q1 that every print step restores (STZ/init.c:189,
+   STZ/bmg.c:215-223). */
 int stz_upload(stz_ctx *ctx, int field, const double *host);
 int stz_download(stz_ctx *ctx, int field, double *host);
 /* The host scalars EpsAvg, Stress (epsavg/stress may be NULL to keep them) and totime.
@@ -257,7 +259,10 @@ int stz_elastic_energy(stz_ctx *ctx, double *e);
 
 /* One iteration of the reference main loop (STZ/bmg.c:138-411) driven by the context's
    own MT19937 stream: softening, ActEnergy, athermal or thermal/elastic KMC, loading,
-   print-period re-solve.  Returns 1 when the run terminates (end strain / instability). */
+   print-period re-solve.  Returns 1 when the run terminates (end strain / instability).
+   The print block and the termination tests run after every step once stz_set_print has
+   been called (a context without it never prints or terminates); as in the reference, a
+   printperiod <= 0 makes every step a print step (STZ/bmg.c:181-187). */
 int stz_set_print(stz_ctx *ctx, int printperiod, double endstrain);
 int stz_step(stz_ctx *ctx, stz_step_result *res);
 
@@ -265,7 +270,8 @@ int stz_step(stz_ctx *ctx, stz_step_result *res);
    before the print-step re-solve as the reference does (STZ/bmg.c:181-287):
    stz_step_kmc runs the iteration up to the print block and sets res->printed = 1 when
    the print block is due (it then returns 0 and the caller must call stz_step_print);
-   stz_step_print re-solves the stress (STZ/bmg.c:289-295) and checks termination. */
+   stz_step_print resets the barriers Q0 <- Q1 (STZ/bmg.c:215-223: only crystal-like
+   transforms change Q0), re-solves the stress (STZ/bmg.c:289-295) and checks termination. */
 int stz_step_kmc(stz_ctx *ctx, stz_step_result *res);
 int stz_step_print(stz_ctx *ctx, stz_step_result *res);
 

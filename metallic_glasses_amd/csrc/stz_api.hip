@@ -537,7 +537,7 @@ void stz_destroy(stz_ctx *ctx)
     stz_resident_free(ctx);
     stz_local_release(ctx);
     stz_inhom_free(ctx);
-    void *ptrs[] = {ctx->q0eff, ctx->scache, ctx->eps0, ctx->sigma, ctx->q0, ctx->deps, ctx->deps_last, ctx->perm, ctx->temp, ctx->tlast, ctx->tloc,
+    void *ptrs[] = {ctx->q0eff, ctx->scache, ctx->eps0, ctx->sigma, ctx->q0, ctx->q1, ctx->deps, ctx->deps_last, ctx->perm, ctx->temp, ctx->tlast, ctx->tloc,
                     ctx->spec, ctx->minq, ctx->argm, ctx->bsum, ctx->gsum, ctx->res, ctx->spart, ctx->mro,
                     ctx->acnt, ctx->aoff, ctx->alist, ctx->dnew, ctx->socp, ctx->gneg};
     for (void *q : ptrs)
@@ -617,6 +617,8 @@ int init_fields(stz_ctx *ctx, const std::vector<double> &theta)
         if (rc) return rc;
         STZ_HIP(hipStreamSynchronize(ctx->stream));
     }
+    if (ctx->q1)   // Q1 = Q0 (STZ/init.c:189)
+        STZ_HIP(hipMemcpyAsync(ctx->q1, ctx->q0, (size_t)ctx->M * N * ctx->rsz, hipMemcpyDeviceToDevice, ctx->stream));
     const int rc = invalidate_cache(ctx);   // new Q0 and strains
     if (rc) return rc;
     for (int c = 0; c < 6; ++c) ctx->epsavg[c] = 0.0;
@@ -639,7 +641,15 @@ int stz_set_mro_map(stz_ctx *ctx)
     ctx->nmro = n;
     if (!n) {
         ctx->mro_h.clear();
+        if (ctx->q1) STZ_HIP(hipFree(ctx->q1));
+        ctx->q1 = nullptr;
         return STZ_OK;
+    }
+    // only crystal-like transforms change Q0 (STZ/elastic.c:446-506), so only MRO contexts
+    // keep the initial barriers Q1; the caller fills it
+    if (!ctx->q1) {
+        const int rc = dalloc(ctx, &ctx->q1, (size_t)ctx->M * ctx->N * ctx->rsz);
+        if (rc) return rc;
     }
     STZ_HIP(hipMalloc((void **)&ctx->mro, (size_t)ctx->N));
     STZ_HIP(hipMemcpy(ctx->mro, ctx->mro_h.data(), (size_t)ctx->N, hipMemcpyHostToDevice));
@@ -718,7 +728,10 @@ int stz_upload(stz_ctx *ctx, int field, const double *host)
         for (int c = 0; c < 6; ++c) ctx->soff[c] = 0.0;
         rc = put_real(ctx, ctx->sigma, host, n);
         break;
-    case STZ_F_Q0: rc = put_real(ctx, ctx->q0, host, n); break;
+    case STZ_F_Q0:
+        rc = put_real(ctx, ctx->q0, host, n);
+        if (!rc && ctx->q1) rc = put_real(ctx, ctx->q1, host, n);   // the barriers a print step restores
+        break;
     case STZ_F_DEPS: rc = put_real(ctx, ctx->deps, host, n); break;
     case STZ_F_DEPS_LAST: rc = put_real(ctx, ctx->deps_last, host, n); break;
     case STZ_F_PERM: STZ_HIP(hipMemcpyAsync(ctx->perm, host, n * 8, hipMemcpyHostToDevice, ctx->stream)); break;
@@ -1002,6 +1015,7 @@ int stz_elastic_energy(stz_ctx *ctx, double *e)
 int stz_set_print(stz_ctx *ctx, int printperiod, double endstrain)
 {
     if (!ctx) return STZ_E_ARG;
+    ctx->print_on = 1;
     ctx->printperiod = printperiod;
     ctx->endstrain = endstrain;
     return STZ_OK;
@@ -1168,7 +1182,8 @@ int stz_step_kmc(stz_ctx *ctx, stz_step_result *res)
         res->kind = STZ_STEP_ELASTIC;
     }
     int term = 0;
-    if (ctx->printperiod > 0) {
+    if (ctx->print_on) {
+        // STZ/bmg.c:181-187; a period <= 0 makes the test true at every step, as there
         const double cur = ctx->epsavg[ctx->lc];
         if (std::fabs(cur - ctx->strainhist_last) >= (ctx->printperiod * ctx->p.strainincrement * 0.99)) {
             ctx->strainhist_last = cur;           // strainhist[N_print-1], STZ/bmg.c:193-201
@@ -1188,7 +1203,16 @@ int stz_step_kmc(stz_ctx *ctx, stz_step_result *res)
 int stz_step_print(stz_ctx *ctx, stz_step_result *res)
 {
     if (!ctx || !res) return STZ_E_ARG;
-    int rc = solve(ctx);                          // print-step re-solve, STZ/bmg.c:289-295
+    int rc;
+    if (ctx->q1) {
+        // Q0 <- Q1 (STZ/bmg.c:215-223): the barriers crystal-like transforms redrew return to
+        // their initial draws (glass barriers never change, so only MRO contexts keep Q1)
+        STZ_HIP(hipMemcpyAsync(ctx->q0, ctx->q1, (size_t)ctx->M * ctx->N * ctx->rsz, hipMemcpyDeviceToDevice,
+                               ctx->stream));
+        rc = invalidate_cache(ctx);
+        if (rc) return rc;
+    }
+    rc = solve(ctx);                              // print-step re-solve, STZ/bmg.c:289-295
     if (rc) return rc;
     for (int c = 0; c < 6; ++c) res->stress[c] = ctx->stress[c];
     return terminated(ctx) ? 1 : 0;               // STZ/bmg.c:382-410

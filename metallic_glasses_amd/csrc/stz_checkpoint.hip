@@ -7,10 +7,11 @@
 // per-voxel array in its storage precision (bit for bit), EpsAvg, <sigma>, the clock,
 // the pending uniform stress increment, the print/termination state, the counters and
 // the event-local update's bookkeeping, the local temperature and the heat-conduction
-// switch, the crystal-like MRO map, and per-voxel moduli when they are set (the
-// reference keeps them fixed for a run, STZ/init.c:457-465).  Derived arrays (the Q0*exp(-D*S) cache, min Q,
-// block sums) are rebuilt by the next rate pass.  A run restarted from a checkpoint
-// continues exactly as the saved run does.
+// switch, the crystal-like MRO map with the initial barriers a print step restores, and
+// per-voxel moduli when they are set (the reference keeps them fixed for a run,
+// STZ/init.c:457-465).  Derived arrays (the Q0*exp(-D*S) cache, min Q, block sums) are
+// rebuilt by the next rate pass.  A run restarted from a checkpoint continues exactly as
+// the saved run does.
 #include "stz_internal.h"
 
 #include <algorithm>
@@ -33,7 +34,7 @@ struct CkptScalars {
     double epsavg[6], stress[6], soff[6], ebase[6], usum[6];
     double totime, pend_delta, strainhist_last, soc_elast, endstrain;
     int64_t step, astep, kstep, estep, ntrans, nsolve, avlan, nlocal, rng_count, nmro;
-    int32_t pend_comp, printperiod, rng_pos, heat;
+    int32_t pend_comp, printperiod, rng_pos, heat, print_on, pad;
     uint32_t rng_state[624];
 };
 
@@ -58,6 +59,7 @@ std::vector<Field> fields_of(stz_ctx *c)
         {c->tlast, N * 8},
         {c->tloc, N * 8},
     };
+    if (c->q1) f.push_back({c->q1, (size_t)c->M * N * R});   // initial barriers of an MRO context
     if (c->lamx) {
         f.push_back({c->lamx, N * 8});
         f.push_back({c->mux, N * 8});
@@ -70,7 +72,7 @@ CkptHeader header_of(const stz_ctx *c)
     CkptHeader h;
     std::memset(&h, 0, sizeof(h));
     std::memcpy(h.magic, kMagic, 8);
-    h.version = 2;
+    h.version = 3;
     h.dim = c->dim;
     h.n0 = c->n[0];
     h.n1 = c->n[1];
@@ -129,6 +131,7 @@ int stz_checkpoint_save(stz_ctx *ctx, const char *path)
     s.rng_count = ctx->rng.count;
     s.pend_comp = ctx->pend_comp;
     s.printperiod = ctx->printperiod;
+    s.print_on = ctx->print_on;
     s.rng_pos = ctx->rng.pos;
     s.heat = ctx->heat;
     s.nmro = ctx->nmro;
@@ -157,7 +160,7 @@ int stz_checkpoint_load(stz_ctx *ctx, const char *path)
     if (!f) return fail(ctx, nullptr, "cannot open the checkpoint file", STZ_E_ARG);
     CkptHeader h;
     CkptScalars s;
-    if (std::fread(&h, sizeof(h), 1, f) != 1 || std::memcmp(h.magic, kMagic, 8) != 0 || h.version != 2)
+    if (std::fread(&h, sizeof(h), 1, f) != 1 || std::memcmp(h.magic, kMagic, 8) != 0 || h.version != 3)
         return fail(ctx, f, "not a checkpoint file of this library", STZ_E_ARG);
     const CkptHeader mine = header_of(ctx);
     if (std::memcmp(&h, &mine, sizeof(h)) != 0)
@@ -202,6 +205,7 @@ int stz_checkpoint_load(stz_ctx *ctx, const char *path)
     ctx->soc_elast = s.soc_elast;
     ctx->endstrain = s.endstrain;
     ctx->printperiod = s.printperiod;
+    ctx->print_on = s.print_on != 0;
     ctx->step = s.step;
     ctx->astep = s.astep;
     ctx->kstep = s.kstep;

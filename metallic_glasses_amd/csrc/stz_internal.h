@@ -196,7 +196,7 @@ struct RepState {
     long long step, kstep, astep, estep, ntrans, nsolve, avlan;
     long long rpos;
     long long budget;        // steps this batch may record
-    int pend_comp, halt, err, nrec, printperiod, pad;
+    int pend_comp, halt, err, nrec, printperiod, print_on;
     // event-local stress update: mean strain of the last solve, uniform increments put in
     // the field since (FP64), updates since the last full solve
     double ebase[6], usum[6];
@@ -256,6 +256,8 @@ struct stz_ctx {
     double *eps0 = nullptr;       // [C][N]
     void *sigma = nullptr;        // [C][N] R
     void *q0 = nullptr;           // [M][N] R
+    void *q1 = nullptr;           // [M][N] R, the stress-free barriers a print step restores
+                                  // (STZ/init.c:189, STZ/bmg.c:215-223); crystal-like MRO contexts only
     void *deps = nullptr;         // [M][S][N] R
     void *deps_last = nullptr;    // [S][N] R
     double *perm = nullptr, *temp = nullptr, *tlast = nullptr;
@@ -300,7 +302,9 @@ struct stz_ctx {
     double pend_delta = 0.0;       // FP64: applied and written back by the next rate pass
     double soff[6] = {0, 0, 0, 0, 0, 0}; // MIXED: uniform stress offset added on read
 
-    // run control
+    // run control: print block and termination tests after every step once stz_set_print
+    // has been called (print_on)
+    int print_on = 0;
     int printperiod = 0;
     double endstrain = 1e300;
     double strainhist_last = 0.0;

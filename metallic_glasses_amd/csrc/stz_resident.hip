@@ -565,7 +565,8 @@ __global__ __launch_bounds__(STZ_NT_MEAN) void kb_finish(const RepDev *__restric
         s.totime += d.dt;                     // STZ/evolution.c:324
         load_elastic_d<C>(d, &s, fp64 != 0);
     }
-    if (s.printperiod > 0) {
+    if (s.print_on) {
+        // STZ/bmg.c:181-187; a period <= 0 makes the test true at every step, as there
         const double cur = get6(s.epsavg, d.lc);
         if (fabs(cur - s.strainhist_last) >= (s.printperiod * d.strainincrement * 0.99)) {
             s.strainhist_last = cur;          // strainhist[N_print-1], STZ/bmg.c:193-201
@@ -842,6 +843,7 @@ int push_state(stz_ctx *ctx, long long budget)
     s.soc_elast = ctx->soc_elast;
     s.endstrain = ctx->endstrain;
     s.printperiod = ctx->printperiod;
+    s.print_on = ctx->print_on;
     s.step = ctx->step;
     s.kstep = ctx->kstep;
     s.astep = ctx->astep;

@@ -35,7 +35,7 @@ class OraStepInfo(C.Structure):
         ("kind", C.c_int), ("site", C.c_int), ("mode", C.c_int), ("n_transformed", C.c_int),
         ("n_negative", C.c_longlong), ("q_chosen", C.c_double), ("total_rate", C.c_double),
         ("dt_kmc", C.c_double), ("totime", C.c_double), ("epsavg", C.c_double * 6),
-        ("stress", C.c_double * 6), ("printed", C.c_int), ("mro", C.c_int),
+        ("stress", C.c_double * 6), ("printed", C.c_int), ("mro", C.c_int), ("el_energy", C.c_double),
     ]
 
 
@@ -43,7 +43,7 @@ class OraMT(C.Structure):
     _fields_ = [("mt", C.c_uint32 * 624), ("mti", C.c_int)]
 
 
-F_EPS0, F_SIGMA, F_Q0, F_DEPS, F_DEPS_LAST, F_PERM, F_TEMP, F_TLAST, F_Q, F_SOFT, F_EPSO, F_TLOCAL = range(12)
+F_EPS0, F_SIGMA, F_Q0, F_DEPS, F_DEPS_LAST, F_PERM, F_TEMP, F_TLAST, F_Q, F_SOFT, F_EPSO, F_TLOCAL, F_Q1 = range(13)
 
 _lib = None
 
@@ -163,10 +163,16 @@ class Oracle:
 
     def field(self, which):
         n = {F_EPS0: self.C * self.N, F_SIGMA: self.C * self.N, F_EPSO: self.C * self.N,
-             F_Q0: self.M * self.N, F_Q: self.M * self.N, F_DEPS: self.M * self.S * self.N,
+             F_Q0: self.M * self.N, F_Q1: self.M * self.N, F_Q: self.M * self.N, F_DEPS: self.M * self.S * self.N,
              F_DEPS_LAST: self.S * self.N}.get(which, self.N)
         ptr = lib().ora_field(self._h, which)
         return np.ctypeslib.as_array(ptr, shape=(n,))
+
+    def set_barriers(self, q0):
+        """Sets Q0 and the stress-free barriers Q1 a print step restores (as stz_upload of
+        STZ_F_Q0 does on the GPU side)."""
+        self.field(F_Q0)[:] = q0
+        self.field(F_Q1)[:] = q0
 
     def scalars(self):
         e = np.zeros(6); s = np.zeros(6); t = C.c_double()

@@ -50,15 +50,33 @@
  *   ora_softening     SofteningFactor default branch, STZ/init.c:711-739.
  *   ora_act_energy    ActEnergy with SF_SQUARE_FWD, MICRO_MEMORY,
  *                     NO_SF_XTAL_MRO, STZ/init.c:561-690; Q<0 count
- *                     STZ/bmg.c:147-153.  pin: closed-form single voxel.
+ *                     STZ/bmg.c:147-153.  pins: closed-form single voxel; pure shear
+ *                     parallel to the mode strain (W = s|deps|^2 V 1e-18/e) with last
+ *                     strains parallel (D = 1), antiparallel (D = 0) and orthogonal
+ *                     (D = 1/4) (tests/test_oracle_step_pins.py).
  *   ora_total_rate    KMC rate loop, STZ/evolution.c:120-134.
  *   ora_select        KMC normalise / prefix / search, STZ/evolution.c:136-224
  *                     pin: numpy cumsum + searchsorted brute force.
  *   ora_transform     VoxelShearTransform glass branch, STZ/elastic.c:361-520.
  *                     pin: hand-computed 2x2 grid, M = 2 case.
+ *   ora_elastic_energy ElasticEnergy, STZ/elastic.c:533-547.  pin: sum of
+ *                     sigma:C0^-1:sigma/2 with a numerical inverse of the Voigt stiffness.
  *   athermal          athermal_plasticity, STZ/evolution.c:47-100.
+ *                     pin: hand-computed 2x2 grid, M = 2: first-index argmin ties, the
+ *                     0.33 boundary, row-major redraw offsets in the stream, the solve's
+ *                     mean C0:(epsavg - <eps0>) and the correction against a numerical
+ *                     inverse of the Voigt stiffness.
  *   ora_step          main loop body, STZ/bmg.c:138-411, KMC loading
- *                     STZ/evolution.c:262-358.
+ *                     STZ/evolution.c:262-358, print block STZ/bmg.c:181-380 (Q0 <- Q1
+ *                     and the re-solve), termination STZ/bmg.c:382-410.
+ *                     pins: thermal step from a uniform state (R, t = 1/R, the dice's
+ *                     index, mixed strain/stress loading: a fresh solve of the new
+ *                     state keeps the loaded <sigma> and has zero mean stress across
+ *                     the load); elastic step (dt, uniaxial-stress increment, the shear
+ *                     branch's strainrate*1); the print schedule for periods 5, 3, 1,
+ *                     0, -2; end-strain, tension and shear instability exits; a
+ *                     crystal-like redraw undone by the next print.  Each of eight
+ *                     planted mistakes in these branches fails one of the pins.
  *
  * Threads: element-wise loops and FFT lines run under OpenMP; every floating-point
  * reduction (rate total, prefix, <sigma>) stays sequential in the reference order, so
@@ -305,6 +323,7 @@ struct ora_state {
     double *g[3], *gg[6];
     /* fields */
     double *eps0, *sigma, *epso, *q0, *deps, *deps_last, *perm, *temp, *tlast, *q, *soft;
+    double *q1;             /* stress-free barriers, restored at every print (STZ/init.c:189, STZ/bmg.c:215-223) */
     double *rate;
     /* solver scratch */
     double *spec;           /* C complex fields, N each */
@@ -325,7 +344,7 @@ struct ora_state {
     long long step, astep, kstep, estep;
     long long avlan_counter;
     double soc_elast;
-    int printperiod; double endstrain; double strainhist_last;
+    int print_on, printperiod; double endstrain; double strainhist_last;
 };
 
 static const double W3[3] = {1, 1, 2};
@@ -481,6 +500,7 @@ static ora_state *alloc_state(const ora_params *p)
     s->sigma = (double *)calloc((size_t)(C * N), sizeof(double));
     s->epso = (double *)calloc((size_t)(C * N), sizeof(double));
     s->q0 = (double *)calloc((size_t)(M * N), sizeof(double));
+    s->q1 = (double *)calloc((size_t)(M * N), sizeof(double));
     s->deps = (double *)calloc((size_t)(M * S * N), sizeof(double));
     s->deps_last = (double *)calloc((size_t)(S * N), sizeof(double));
     s->perm = (double *)calloc((size_t)N, sizeof(double));
@@ -496,7 +516,7 @@ static ora_state *alloc_state(const ora_params *p)
     s->g2 = (double *)calloc((size_t)N, sizeof(double));
     s->mro = (unsigned char *)calloc((size_t)N, 1);
     s->mroth = (double *)calloc((size_t)N, sizeof(double));
-    if (!s->mro || !s->mroth || !s->eps0 || !s->sigma || !s->epso || !s->q0 || !s->deps || !s->deps_last || !s->perm ||
+    if (!s->mro || !s->mroth || !s->eps0 || !s->sigma || !s->epso || !s->q0 || !s->q1 || !s->deps || !s->deps_last || !s->perm ||
         !s->temp || !s->tlast || !s->q || !s->soft || !s->rate || !s->spec || !s->tau0 || !s->epsolast ||
         !s->tlocal || !s->g2) {
         ora_destroy(s);
@@ -516,7 +536,7 @@ ora_state *ora_create_empty(const ora_params *p) { return alloc_state(p); }
 void ora_destroy(ora_state *s)
 {
     if (!s) return;
-    free(s->eps0); free(s->sigma); free(s->epso); free(s->q0); free(s->deps); free(s->deps_last);
+    free(s->eps0); free(s->sigma); free(s->epso); free(s->q0); free(s->q1); free(s->deps); free(s->deps_last);
     free(s->perm); free(s->temp); free(s->tlast); free(s->q); free(s->soft); free(s->rate);
     free(s->spec); free(s->tau0); free(s->epsolast);
     free(s->lamx); free(s->mux); free(s->estar);
@@ -586,6 +606,7 @@ static ora_state *init_fields(ora_state *s)
     for (int m = 0; m < s->M; m++)
         for (long v = 0; v < N; v++)
             s->q0[(long)m * N + v] = ora_gaussian(&s->rng, s->p.q0_std) + s->p.q0_mean;
+    memcpy(s->q1, s->q0, sizeof(double) * (size_t)s->M * (size_t)N);       /* Q1 = Q0, STZ/init.c:189 */
     for (int m = 0; m < s->M; m++)
         for (long v = 0; v < N; v++) dstrain(s, m, v);
     /* initial solve with eps0 = 0, epsavg = 0 (STZ/bmg.c:117-127) */
@@ -650,6 +671,7 @@ double *ora_field(ora_state *s, int which)
     case ORA_F_SOFTENING: return s->soft;
     case ORA_F_EPSO: return s->epso;
     case ORA_F_TLOCAL: return s->tlocal;
+    case ORA_F_Q1: return s->q1;
     default: return NULL;
     }
 }
@@ -684,6 +706,7 @@ void ora_counters(ora_state *s, long long *o)
 
 void ora_set_print(ora_state *s, int printperiod, double endstrain)
 {
+    s->print_on = 1;
     s->printperiod = printperiod;
     s->endstrain = endstrain;
 }
@@ -1266,10 +1289,13 @@ int ora_step(ora_state *s, ora_step_info *info)
     }
     if (rc) return rc;
     int term = 0;
-    if (s->printperiod > 0) {
+    if (s->print_on) {
+        /* print block, STZ/bmg.c:181-380; with printperiod <= 0 the test holds at every step */
         double cur = s->epsavg[s->lc];
         if (fabs(cur - s->strainhist_last) >= (s->printperiod * s->p.strainincrement * 0.99)) {
-            s->strainhist_last = cur;
+            s->strainhist_last = cur;                                     /* :193-201 */
+            info->el_energy = ora_elastic_energy(s);                      /* :174-178, recorded :225 */
+            memcpy(s->q0, s->q1, sizeof(double) * (size_t)s->M * (size_t)s->N);   /* Q0 <- Q1, :215-223 */
             ora_stress_solve(s);              /* print-step re-solve, STZ/bmg.c:289-295 */
             info->printed = 1;
         }

@@ -62,6 +62,7 @@ typedef struct ora_step_info {
     double stress[6];
     int printed;       /* 1 if the print block ran (and re-solved the stress) */
     int mro;           /* thermal: MRO type of the selected voxel (actQ.out's first column) */
+    double el_energy;  /* printed: ElasticEnergy of the field before the re-solve (STZ/bmg.c:174-178, 225) */
 } ora_step_info;
 
 /* --- RNG (GSL mt19937 + polar Gaussian) ------------------------------- */
@@ -95,7 +96,8 @@ enum {
     ORA_F_Q = 8,          /* [M][N] last ActEnergy output */
     ORA_F_SOFTENING = 9,  /* [N] */
     ORA_F_EPSO = 10,      /* [C][N] total strain of the last solve */
-    ORA_F_TLOCAL = 11     /* [N] local temperature */
+    ORA_F_TLOCAL = 11,    /* [N] local temperature */
+    ORA_F_Q1 = 12         /* [M][N] stress-free barriers restored at every print (STZ/init.c:189) */
 };
 double *ora_field(ora_state *s, int which);
 void    ora_scalars(ora_state *s, double *epsavg, double *stress, double *totime);
@@ -120,6 +122,8 @@ double ora_elastic_energy(ora_state *s);             /* ElasticEnergy (sum) */
 
 /* one iteration of the main loop (STZ/bmg.c:138-411); returns 0, or <0 on error, 1 on termination */
 int ora_step(ora_state *s, ora_step_info *info);
+/* enables the print block and the termination tests (off until called); printperiod <= 0
+   prints at every step, as the reference's test |d strain| >= printperiod*inc*0.99 does */
 void ora_set_print(ora_state *s, int printperiod, double endstrain);
 void ora_set_strainhist(ora_state *s, double last); /* strainhist[N_print-1], STZ/bmg.c:182 */
 int  ora_set_threads(int n);   /* OpenMP threads (n <= 0: query); returns the count in use */

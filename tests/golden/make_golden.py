@@ -7,7 +7,7 @@ trace_<name>.npz     event traces of the CPU FP64 oracle (oracle/liboracle.so) f
                      replay the same seed and compare step by step.
 
 Run: python tests/golden/make_golden.py fft
-     python tests/golden/make_golden.py trace ref2d_512 [events]
+     python tests/golden/make_golden.py trace ref2d_512 [events] [threads]
 """
 import os
 import sys
@@ -42,7 +42,22 @@ def make_fft():
     np.savez_compressed(os.path.join(HERE, "fft_vectors.npz"), **out)
 
 
-def make_trace(name, events, threads=8, printperiod=5, endstrain=1.0):
+# sigma snapshots: the full stress tensor at a fixed seeded sample of voxels, taken after
+# the step that completes the given thermal event (print re-solve included)
+SIGMA_SAMPLE = 4096
+SIGMA_AT = {"shear3d_128": (500, 2000, 10000)}
+
+
+def sample_voxels(N, k=SIGMA_SAMPLE, seed=20261204):
+    rng = np.random.default_rng(seed)
+    return np.sort(rng.choice(N, size=min(k, N), replace=False)).astype(np.int64)
+
+
+def make_trace(name, events, threads=8, printperiod=5, endstrain=1.0, save_every=1000):
+    """Runs the oracle until `events` thermal events and writes trace_<name>.npz.
+
+    The file is rewritten every `save_every` thermal events, so a long run leaves a
+    usable (shorter) trace if it is stopped early."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as po
     po.set_threads(threads)
@@ -53,24 +68,42 @@ def make_trace(name, events, threads=8, printperiod=5, endstrain=1.0):
     rows = []
     nev = 0
     lc = 0 if kw.get("loading", 0) == 0 else (2 if kw["dim"] == 2 else 5)
+    sigma_at = SIGMA_AT.get(name, ())
+    vox = sample_voxels(o.N) if sigma_at else np.zeros(0, np.int64)
+    snap_step, snap_sigma = [], []
+
+    def save():
+        a = np.array(rows, dtype=object)
+        tmp = os.path.join(HERE, f".tmp_trace_{name}.npz")
+        extra = {}
+        if sigma_at:
+            extra = dict(sigma_voxels=vox, sigma_step=np.array(snap_step, dtype=np.int64),
+                         sigma=np.array(snap_sigma, dtype=np.float64).reshape(len(snap_step), o.C, len(vox)))
+        np.savez_compressed(tmp,
+                            kind=a[:, 0].astype(np.int8), site=a[:, 1].astype(np.int32),
+                            mode=a[:, 2].astype(np.int8), ntrans=a[:, 3].astype(np.int32),
+                            totime=a[:, 4].astype(np.float64), stress=a[:, 5].astype(np.float64),
+                            printed=a[:, 6].astype(np.int8), printperiod=printperiod, endstrain=endstrain,
+                            params=np.array(repr(sorted(kw.items()))), **extra)
+        os.replace(tmp, os.path.join(HERE, f"trace_{name}.npz"))
+
     while nev < events:
         rc, info = o.step()
         rows.append((info.kind, info.site, info.mode, info.n_transformed, info.totime, info.stress[lc],
                      info.printed))
-        nev += info.kind == 1
+        if info.kind == 1:
+            nev += 1
+            if nev in sigma_at:
+                sig = o.field(po.F_SIGMA).reshape(o.C, o.N)
+                snap_step.append(len(rows) - 1)
+                snap_sigma.append(sig[:, vox].copy())
+            if nev % save_every == 0:
+                save()
         if len(rows) % 500 == 0:
             print(f"{name}: {len(rows)} steps, {nev} events, {time.time() - t0:.0f}s", flush=True)
         if rc:
             break
-    a = np.array(rows, dtype=object)
-    tmp = os.path.join(HERE, f".tmp_trace_{name}.npz")
-    np.savez_compressed(tmp,
-                        kind=a[:, 0].astype(np.int8), site=a[:, 1].astype(np.int32),
-                        mode=a[:, 2].astype(np.int8), ntrans=a[:, 3].astype(np.int32),
-                        totime=a[:, 4].astype(np.float64), stress=a[:, 5].astype(np.float64),
-                        printed=a[:, 6].astype(np.int8), printperiod=printperiod, endstrain=endstrain,
-                        params=np.array(repr(sorted(kw.items()))))
-    os.replace(tmp, os.path.join(HERE, f"trace_{name}.npz"))
+    save()
     print(f"{name}: done, {len(rows)} steps, {nev} events, {time.time() - t0:.0f}s", flush=True)
 
 
@@ -79,4 +112,5 @@ if __name__ == "__main__":
     if what[0] == "fft":
         make_fft()
     elif what[0] == "trace":
-        make_trace(what[1], int(what[2]) if len(what) > 2 else 10000)
+        make_trace(what[1], int(what[2]) if len(what) > 2 else 10000,
+                   threads=int(what[3]) if len(what) > 3 else 8)

@@ -34,8 +34,9 @@ def _lowered(kw):
     g = mg.Stz(**kw)
     rng = np.random.default_rng(3)
     idx = np.concatenate([np.arange(0, 200), rng.choice(np.arange(256, o.N), 150, replace=False)])
-    q = o.field(po.F_Q0).reshape(o.M, o.N)
+    q = o.field(po.F_Q0).reshape(o.M, o.N).copy()
     q[:, idx] = -1.0
+    o.set_barriers(q.ravel())          # Q0 and the barriers a print step restores
     g.upload(mg.F_Q0, o.field(po.F_Q0))
     o.set_print(5, 1.0)
     g.set_print(5, 1.0)

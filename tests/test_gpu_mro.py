@@ -89,9 +89,10 @@ def test_trajectory_matches_oracle(source, nsteps):
         o = po.Oracle(**kw)
         g = mg.Stz(**kw)
     t = o.mro_map()
+    q_init = o.field(po.F_Q0).copy()
     o.set_print(5, 1.0)
     g.set_print(5, 1.0)
-    worst, hits = 0.0, 0
+    worst, hits, resets, redrawn = 0.0, 0, 0, False
     for i in range(nsteps):
         rc_o, io = o.step()
         rc_g, ig = g.step()
@@ -99,14 +100,23 @@ def test_trajectory_matches_oracle(source, nsteps):
         if io.kind == 1:
             assert (io.site, io.mode) == (ig.site, ig.mode), i
             hits += t[io.site] > 0
+            redrawn |= bool(t[io.site] > 0)
+        if io.kind == 2:
+            redrawn = True
+        if io.printed and redrawn:
+            # the print block restored every barrier to its initial draw (STZ/bmg.c:215-223)
+            assert np.array_equal(o.field(po.F_Q0), q_init), i
+            assert np.array_equal(g.download(mg.F_Q0), q_init), i
+            resets += 1
+            redrawn = False
         so = np.array(io.stress[:6])
         sg = np.array(ig.stress[:6])
         worst = max(worst, np.max(np.abs(so - sg)) / max(np.max(np.abs(so)), 1e-3))
         assert rc_o == rc_g
         if rc_o:
             break
-    print("  crystal events", hits, "worst %.2e" % worst, flush=True)
-    assert hits > 0
+    print("  crystal events", hits, "prints after redraws", resets, "worst %.2e" % worst, flush=True)
+    assert hits > 0 and resets > 0
     assert worst < 1e-10
     assert np.array_equal(g.download(mg.F_Q0), o.field(po.F_Q0))
     assert np.array_equal(g.download(mg.F_DEPS), o.field(po.F_DEPS))
@@ -116,6 +126,7 @@ def test_run_steps_checkpoint_and_refusals(tmp_path):
     kw = _kw(seed=33)
     g1 = mg.Stz(**kw)
     g2 = mg.Stz(**kw)
+    q_init = g1.download(mg.F_Q0)
     for g in (g1, g2):
         g.set_print(5, 1.0)
     seq1 = []
@@ -136,6 +147,13 @@ def test_run_steps_checkpoint_and_refusals(tmp_path):
     b = [h.step()[1] for _ in range(200)]
     assert [(r.kind, r.site, r.mode) for r in a] == [(r.kind, r.site, r.mode) for r in b]
     assert np.array_equal(g1.download(mg.F_Q0), h.download(mg.F_Q0))
+    # the checkpoint keeps the initial barriers: a print block on the restored context
+    # restores them as it does on the saved one
+    r = mg.StepResult()
+    g1.step_print(r)
+    h.step_print(r)
+    assert np.array_equal(g1.download(mg.F_Q0), q_init)
+    assert np.array_equal(h.download(mg.F_Q0), q_init)
     with pytest.raises(mg.StzError) as e:
         mg.Ensemble([g2])
     assert e.value.code == -5

@@ -80,6 +80,7 @@ def test_driver_reproduces_oracle_outputs(tmp_path):
     qs = []
     stress = [0.0]
     strain = [0.0]
+    energy = [0.0]
     u1 = None
     while True:
         rc, info = o.step()
@@ -87,6 +88,7 @@ def test_driver_reproduces_oracle_outputs(tmp_path):
             qs.append(info.q_chosen)
         if info.printed:
             strain.append(info.epsavg[2])
+            energy.append(info.el_energy)
             if u1 is None:
                 u1 = o.displacement()         # the first print block's X1R/X2R (index 001)
         if rc:
@@ -95,6 +97,12 @@ def test_driver_reproduces_oracle_outputs(tmp_path):
     assert np.allclose(q[:, 1], qs, rtol=1e-6, atol=0)     # %.6e text
     assert np.all(q[:, 0] == 0)
     assert np.allclose(ss[:, 0], strain, rtol=1e-6, atol=1e-12)
+    # ElasticEnergy.out: (strain, energy before the print re-solve) per print, %.6e text
+    el = np.loadtxt(tmp_path / "ElasticEnergy.out", ndmin=2)
+    assert el.shape == (len(energy), 2)
+    assert np.allclose(el[:, 0], strain, rtol=1e-6, atol=1e-12)
+    assert np.allclose(el[:, 1], energy, rtol=1e-6, atol=0)
+    assert np.all(el[1:, 1] > 0)
     for name in ("SOC.out", "ElastSOC.out", "ElasticEnergy.out", "EA.out", "EpsAvg.out", "avg.out",
                  "Strain001_P00.out", "Stress001_P00.out"):
         assert (tmp_path / name).exists(), name

@@ -20,7 +20,7 @@ def _fill_random(o, rng, sigma_scale=1.0):
     o.field(po.F_SIGMA)[:] = rng.standard_normal(C * N) * sigma_scale
     o.field(po.F_DEPS)[:] = rng.standard_normal(M * S * N) * 0.07
     o.field(po.F_DEPS_LAST)[:] = rng.standard_normal(S * N) * 0.07
-    o.field(po.F_Q0)[:] = 2.0 + 0.25 * rng.standard_normal(M * N)
+    o.set_barriers(2.0 + 0.25 * rng.standard_normal(M * N))
     o.field(po.F_PERM)[:] = rng.uniform(0, 0.5, N)
     o.field(po.F_TEMP)[:] = rng.uniform(0, 2.0, N)
     o.field(po.F_TLAST)[:] = -rng.uniform(0, 5e-7, N)
