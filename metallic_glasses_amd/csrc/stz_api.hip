@@ -494,6 +494,8 @@ int stz_create(const stz_params *p, stz_ctx **out)
     rc = rc ? rc : dalloc(ctx, (void **)&ctx->argm, N);
     rc = rc ? rc : dalloc(ctx, (void **)&ctx->bsum, (size_t)ctx->ngrp * STZ_GRP * 8);
     rc = rc ? rc : dalloc(ctx, (void **)&ctx->gsum, ctx->ngrp * 8);
+    rc = rc ? rc : dalloc(ctx, (void **)&ctx->csum, (size_t)((ctx->ngrp + STZ_CHUNK - 1) / STZ_CHUNK) * 8);
+    rc = rc ? rc : dalloc(ctx, (void **)&ctx->ccnt, (size_t)((ctx->nblk + STZ_CHUNK - 1) / STZ_CHUNK) * sizeof(int));
     rc = rc ? rc : dalloc(ctx, (void **)&ctx->res, 64 * 8);
     rc = rc ? rc : dalloc(ctx, (void **)&ctx->spart, rows * ctx->C * 8);
     rc = rc ? rc : dalloc(ctx, (void **)&ctx->acnt, ctx->nblk * sizeof(int));
@@ -538,7 +540,7 @@ void stz_destroy(stz_ctx *ctx)
     stz_local_release(ctx);
     stz_inhom_free(ctx);
     void *ptrs[] = {ctx->q0eff, ctx->scache, ctx->eps0, ctx->sigma, ctx->q0, ctx->q1, ctx->deps, ctx->deps_last, ctx->perm, ctx->temp, ctx->tlast, ctx->tloc,
-                    ctx->spec, ctx->minq, ctx->argm, ctx->bsum, ctx->gsum, ctx->res, ctx->spart, ctx->mro,
+                    ctx->spec, ctx->minq, ctx->argm, ctx->bsum, ctx->gsum, ctx->csum, ctx->ccnt, ctx->res, ctx->spart, ctx->mro,
                     ctx->acnt, ctx->aoff, ctx->alist, ctx->dnew, ctx->socp, ctx->gneg};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);

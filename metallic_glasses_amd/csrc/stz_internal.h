@@ -15,6 +15,7 @@
 #define STZ_MAXM 64
 #define STZ_BLK 256          // voxels per rate block (one per thread)
 #define STZ_GRP 256          // rate blocks per selection group
+#define STZ_CHUNK 256        // group sums per chunk of the running scan; voxel blocks per athermal chunk
 #define STZ_LOCAL_MAXSRC 64  // sites one event-local stress update takes (larger avalanches: full solve)
 
 // control block (ctx->res, doubles unless noted), shared by kernels and host:
@@ -210,8 +211,8 @@ struct RepDev {
     double *perm, *temp, *tlast;
     double *tloc;
     uint8_t *argm;
-    double *bsum, *gsum, *socp, *spart, *res;
-    int *acnt, *aoff, *alist;
+    double *bsum, *gsum, *csum, *socp, *spart, *res;
+    int *acnt, *ccnt, *aoff, *alist;
     RepState *st;
     stz_step_result *rec;
     int *hflag;              // host-visible copy of the halt state (mapped pinned memory)
@@ -280,6 +281,8 @@ struct stz_ctx {
     long long acap = 16;          // voxels of an avalanche transformed without a host round trip
     long long acap_max = 256;
     double *gsum = nullptr;       // [ngrp]
+    double *csum = nullptr;       // [nchr] chunk totals: NT consecutive group sums each
+    int *ccnt = nullptr;          // [ncha] athermal counts per chunk of NT voxel blocks
     long long *gneg = nullptr;    // [ngrp]
     double *res = nullptr;        // small result area (device)
     double *hres = nullptr;       // pinned host mirror

@@ -16,6 +16,7 @@ namespace {
 #endif
 
 constexpr int NT = 256;
+static_assert(NT == STZ_CHUNK, "a chunk is one workgroup's scan");
 constexpr int WAVE = 64;
 constexpr int NW = NT / WAVE;
 
@@ -112,13 +113,16 @@ __device__ __forceinline__ R mro_factor(int t)
 // initialisation, a checkpoint load or a clock reset invalidates the entry), and Perm,
 // Temp and tlast are not read.  Otherwise S is computed and the entry is fresh when it
 // holds that S.  An S with its sign bit set (a negative Perm upload) is never cached.
-template <typename R, int DIM, bool XTAL = false>
+//
+// LEAN: no cache at all (every voxel evaluates Q0*exp(-D*S) afresh, which gives the same
+// bits as a cached entry); a.q0eff and a.scache are not touched.
+template <typename R, int DIM, bool XTAL = false, bool LEAN = false>
 __device__ __forceinline__ void load_voxel(const RateArgs<R> &a, long long v, bool apply_pending, VoxelState<R> &s)
 {
     constexpr int C = Comp<DIM>::C, S = Comp<DIM>::S;
-    const R sc = ld_stream<2>(a.scache + v);
+    const R sc = LEAN ? (R)__builtin_nan("") : ld_stream<2>(a.scache + v);
     s.settle = false;
-    if (__builtin_signbit(sc) && sc == sc) {
+    if (!LEAN && __builtin_signbit(sc) && sc == sc) {
         s.soft = -sc;
         s.fresh = true;
     } else {
@@ -155,6 +159,7 @@ __device__ __forceinline__ void load_voxel(const RateArgs<R> &a, long long v, bo
     }
     // one address select per voxel (a select between two struct fields inside the mode
     // loop is compiled through private memory)
+    if constexpr (LEAN) s.fresh = false;
     const uintptr_t qa = (uintptr_t)a.q0eff, qb = (uintptr_t)a.q0;
     s.q0src = (const R *)(s.fresh ? qa : qb);
     if (!s.fresh) {
@@ -257,9 +262,11 @@ __device__ __forceinline__ double block_scan(double x, double *sh)
 // rate pass over one block of NT voxels (all modes): block rate sums per mode, the
 // per-voxel minimum Q and its first argmin, the athermal flag count of the block, the
 // Q<0 count, and the Q0*exp(-D*S) cache refresh.  XTAL: the context has crystal-like
-// MRO voxels (a.mro set); glass contexts compile without the per-voxel type.
+// MRO voxels (a.mro set); glass contexts compile without the per-voxel type.  LEAN:
+// block rate sums and the Q<0 count only (no cache, no per-voxel minimum, no athermal
+// count): the stateless pass of stz_rates_noalloc.
 // ---------------------------------------------------------------------------------
-template <typename R, int DIM, bool XTAL = false>
+template <typename R, int DIM, bool XTAL = false, bool LEAN = false>
 __device__ __forceinline__ void rates_block(const RateArgs<R> &a, long long blk)
 {
     __shared__ double wsum[STZ_MAXM][NW];
@@ -271,7 +278,7 @@ __device__ __forceinline__ void rates_block(const RateArgs<R> &a, long long blk)
     // modes are loaded STZ_RATE_DEPTH ahead of the one being evaluated
     ModeData<R> md[STZ_RATE_DEPTH + 1];
     if (valid) {
-        load_voxel<R, DIM, XTAL>(a, v, true, s);
+        load_voxel<R, DIM, XTAL, LEAN>(a, v, !LEAN, s);
 #pragma unroll
         for (int j = 0; j < STZ_RATE_DEPTH; ++j)
             if (j < a.M) load_mode<R, DIM>(a, v, j, s.q0src, md[j]);
@@ -287,7 +294,7 @@ __device__ __forceinline__ void rates_block(const RateArgs<R> &a, long long blk)
                 if (s.t) softm = softm * s.fac;
             R q;
             const R qe = mode_q<R, DIM, XTAL>(a, md[0], s, softm, q, rate);
-            if (!s.fresh) a.q0eff[(long long)m * a.N + v] = qe;
+            if (!LEAN && !s.fresh) a.q0eff[(long long)m * a.N + v] = qe;
             if (m == 0 || minq > q) { minq = q; am = m; }     // first argmin (STZ/evolution.c:56-63)
             nneg += (q < (R)0);
 #pragma unroll
@@ -303,7 +310,7 @@ __device__ __forceinline__ void rates_block(const RateArgs<R> &a, long long blk)
         wcnt[0][w] = wn;
         wcnt[1][w] = wa;
     }
-    if (valid) {
+    if (!LEAN && valid) {
         a.minq[v] = minq;
         a.argm[v] = (uint8_t)am;
         // a settled entry holds -S; an S with its sign bit set is not cached (NaN)
@@ -323,7 +330,7 @@ __device__ __forceinline__ void rates_block(const RateArgs<R> &a, long long blk)
             ta += wcnt[1][i];
         }
         if (tn) atomicAdd(a.nneg, (unsigned long long)tn);
-        a.acnt[blk] = ta;
+        if (!LEAN) a.acnt[blk] = ta;
     }
 }
 
@@ -368,6 +375,24 @@ __device__ double group_total(const double *gsum, long long ng, double *sh)
             }
         }
     }
+    return carry;
+}
+
+// The total of chunk c (NT consecutive group sums) exactly as group_total's running scan
+// takes it: the last element of the chunk's block scan.  All threads call it.
+__device__ __forceinline__ double chunk_total(const double *gsum, long long ng, long long c, double *sh)
+{
+    const long long g = c * NT + threadIdx.x;
+    const double x = g < ng ? gsum[g] : 0.0;
+    return block_scan(x, sh);       // thread NT - 1 holds the chunk total
+}
+
+// R from the chunk totals: the running carry of group_total, by one thread (bit for bit
+// the same value; the chunk totals are computed in parallel beforehand)
+__device__ __forceinline__ double chunks_total(const double *csum, long long nch)
+{
+    double carry = 0.0;
+    for (long long c = 0; c < nch; ++c) carry = (c == 0) ? csum[0] : carry + csum[c];
     return carry;
 }
 
@@ -552,39 +577,61 @@ __device__ __forceinline__ int first_true(bool f, int *sh_min)
 // site, mode, Q to res[10..12] and, when dnew is given, transforms the chosen voxel.
 // Returns false (res[10] = -1) when tau lies beyond the total.
 template <typename R, int DIM, bool XTAL = false>
-__device__ bool select_block(const RateArgs<R> &a, const double *gsum, long long ng, double tau, double *res,
-                             const ApplyArgs &ap, const double *dnew, double dscale)
+__device__ bool select_block(const RateArgs<R> &a, const double *gsum, const double *csum, long long ng, double tau,
+                             double *res, const ApplyArgs &ap, const double *dnew, double dscale)
 {
     __shared__ double sh[NW];
     __shared__ int smin;
     __shared__ double vals[NT];
+    __shared__ long long s_cstar;
+    __shared__ double s_carry;
     const int tid = threadIdx.x;
     const long long nb = (long long)a.M * a.nblk;
-    // level 2: groups
-    double carry = 0.0;
-    long long gstar = -1;
-    double base2 = 0.0;
-    for (long long c0 = 0; c0 < ng && gstar < 0; c0 += NT) {
-        const long long g = c0 + tid;
-        const double x = g < ng ? gsum[g] : 0.0;
-        const double incl = block_scan(x, sh);
-        const double P = (c0 == 0) ? incl : carry + incl;
-        vals[tid] = P;
-        const int f = first_true(g < ng && P >= tau, &smin);
-        if (f < NT) {
-            gstar = c0 + f;
-            base2 = f == 0 ? (c0 == 0 ? 0.0 : carry) : vals[f - 1];
-        } else {
-            carry = vals[NT - 1];
+    // level 3: chunks of NT groups, by the running carry of the chunk totals (the values
+    // R was summed with); the first chunk whose carry reaches tau
+    if (tid == 0) {
+        const long long nch = (ng + NT - 1) / NT;
+        double carry = 0.0, prev = 0.0;
+        long long cs = -1;
+        for (long long c = 0; c < nch; ++c) {
+            prev = carry;
+            carry = (c == 0) ? csum[0] : carry + csum[c];
+            if (carry >= tau) {
+                cs = c;
+                break;
+            }
         }
-        __syncthreads();
+        s_cstar = cs;
+        s_carry = prev;
     }
-    if (gstar < 0) {                    // tau beyond the total: "failed to locate"
+    __syncthreads();
+    const long long cstar = s_cstar;
+    if (cstar < 0) {                    // tau beyond the total: "failed to locate"
         if (tid == 0) {
             res[10] = -1;
             res[11] = -1;
         }
         return false;
+    }
+    // level 2: groups of chunk cstar, prefixes carry + block scan as the running scan has them
+    long long gstar;
+    double base2;
+    {
+        const long long c0 = cstar * NT;
+        const double carry = s_carry;
+        const long long g = c0 + tid;
+        const double x = g < ng ? gsum[g] : 0.0;
+        const double incl = block_scan(x, sh);
+        const double P = (c0 == 0) ? incl : carry + incl;
+        vals[tid] = P;
+        int f = first_true(g < ng && P >= tau, &smin);
+        if (f == NT) {                  // rounding put the chunk's last prefix alone at tau
+            const long long lastg = (ng - 1 < c0 + NT - 1) ? ng - 1 : c0 + NT - 1;
+            f = (int)(lastg - c0);
+        }
+        gstar = c0 + f;
+        base2 = f == 0 ? (c0 == 0 ? 0.0 : carry) : vals[f - 1];
+        __syncthreads();
     }
     // level 1: blocks of group gstar
     const long long b = gstar * STZ_GRP + tid;
@@ -693,7 +740,7 @@ __device__ __forceinline__ int athermal_flag(const AthArgs<R> &a, long long v)
 // has a barrier after its count writes, so no wave is two blocks ahead).  A block without
 // flagged voxels writes +0.0, the value of its tree of zeros.
 template <typename R, int DIM>
-__device__ void athermal_block_f(const AthArgs<R> &a, long long blk, int f, int par)
+__device__ void athermal_block_f(const AthArgs<R> &a, long long blk, int f, int par, long long boff)
 {
     constexpr int C = Comp<DIM>::C, S = Comp<DIM>::S;
     __shared__ int ws2[2][NW];
@@ -717,7 +764,7 @@ __device__ void athermal_block_f(const AthArgs<R> &a, long long blk, int f, int 
     __shared__ long long fv[NT], fidx[NT];   // the block's flagged voxels in list order
     double dw = 0.0;
     if (f) {
-        const long long idx = (long long)a.off[blk] + loff + before;
+        const long long idx = boff + loff + before;
         a.list[idx] = (int)v;
         fv[loff + before] = v;
         fidx[loff + before] = idx;
@@ -753,7 +800,7 @@ template <typename R, int DIM>
 __device__ void athermal_block(const AthArgs<R> &a, long long blk)
 {
     const long long v = blk * NT + threadIdx.x;
-    athermal_block_f<R, DIM>(a, blk, v < a.N ? athermal_flag(a, v) : 0, 0);
+    athermal_block_f<R, DIM>(a, blk, v < a.N ? athermal_flag(a, v) : 0, 0, (long long)a.off[blk]);
 }
 
 // fixed-order sum of the athermal block partials by one workgroup of NTH threads
@@ -764,6 +811,29 @@ __device__ double ordered_sum(const double *p, long long n)
     const int tid = threadIdx.x;
     double a = 0.0;
     for (long long i = tid; i < n; i += NTH) a = a + p[i];
+    red[tid] = a;
+    __syncthreads();
+    for (int s2 = NTH / 2; s2 > 0; s2 >>= 1) {
+        if (tid < s2) red[tid] = red[tid] + red[tid + s2];
+        __syncthreads();
+    }
+    const double r = red[0];
+    __syncthreads();
+    return r;
+}
+
+// ordered_sum of the athermal block partials over the blocks with flagged voxels only:
+// the others' partials are +0.0 (their tree of zeros), which leaves every running sum as
+// it is, so the value equals ordered_sum over all n partials bit for bit.  cnt: flagged
+// voxels per block; ccnt: per chunk of NT blocks (whole empty chunks are skipped).
+template <int NTH>
+__device__ double ordered_sum_flagged(const double *p, const int *cnt, const int *ccnt, long long n)
+{
+    __shared__ double red[NTH];
+    const int tid = threadIdx.x;
+    double a = 0.0;
+    for (long long i = tid; i < n; i += NTH)
+        if (ccnt[i / NT] != 0 && cnt[i] != 0) a = a + p[i];
     red[tid] = a;
     __syncthreads();
     for (int s2 = NTH / 2; s2 > 0; s2 >>= 1) {

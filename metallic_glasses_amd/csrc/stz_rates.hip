@@ -33,25 +33,35 @@ __global__ __launch_bounds__(NT) void k_group(const double *__restrict__ bsum, l
     group_sums(bsum, nb, gsum, ng, (long long)blockIdx.x * STZ_GRP_PER_BLOCK, ws);
 }
 
-// One workgroup: R from the group sums, the Q<0 count and, when `decide` is set, the
+// chunk totals of the group sums (one workgroup per chunk of NT groups)
+__global__ __launch_bounds__(NT) void k_chunks(const double *__restrict__ gsum, long long ng, double *__restrict__ csum)
+{
+    __shared__ double sh[NW];
+    const double t = chunk_total(gsum, ng, blockIdx.x, sh);
+    if (threadIdx.x == NT - 1) csum[blockIdx.x] = t;
+}
+
+// One workgroup: R from the chunk totals, the Q<0 count and, when `decide` is set, the
 // step kind (STZ/bmg.c:161, STZ/evolution.c:134-136), the selection target and, for an
 // athermal step, the list offsets of the avalanche (more than `cap` voxels: overflow).
-__global__ __launch_bounds__(NT) void k_decide(const double *__restrict__ gsum, long long ng, double *__restrict__ res,
+__global__ __launch_bounds__(NT) void k_decide(const double *__restrict__ csum, long long ng, double *__restrict__ res,
                                                int decide, double dt, const double *dicep, const int *acnt, int *aoff,
                                                long long nblk, long long cap)
 {
-    __shared__ double ws[NW];
     __shared__ unsigned long long s_nn;
+    __shared__ double s_R;
     const int tid = threadIdx.x;
-    const double R = group_total(gsum, ng, ws);
     if (tid == 0) {
+        const double Rt = chunks_total(csum, (ng + NT - 1) / NT);
         unsigned long long *cnt = reinterpret_cast<unsigned long long *>(res + RES_NNEG);
         s_nn = *cnt;
+        s_R = Rt;
         *cnt = 0ull;
-        res[8] = R;
+        res[8] = Rt;
         res[9] = (double)s_nn;
     }
     __syncthreads();
+    const double R = s_R;
     if (!decide) return;
     const unsigned long long nn = s_nn;
     int kind;
@@ -74,7 +84,8 @@ __global__ __launch_bounds__(NT) void k_decide(const double *__restrict__ gsum, 
 }
 
 template <typename R, int DIM, bool XTAL>
-__global__ __launch_bounds__(NT) void k_select(RateArgs<R> a, const double *__restrict__ gsum, long long ng, double tau,
+__global__ __launch_bounds__(NT) void k_select(RateArgs<R> a, const double *__restrict__ gsum,
+                                               const double *__restrict__ csum, long long ng, double tau,
                                                double *__restrict__ res, int gated, ApplyArgs ap,
                                                const double *__restrict__ dnew)
 {
@@ -82,7 +93,7 @@ __global__ __launch_bounds__(NT) void k_select(RateArgs<R> a, const double *__re
         if (reinterpret_cast<const int *>(res + RES_KIND)[0] != KIND_THERMAL) return;
         tau = res[RES_TAU];
     }
-    const bool ok = select_block<R, DIM, XTAL>(a, gsum, ng, tau, res, ap, dnew, 1.0);
+    const bool ok = select_block<R, DIM, XTAL>(a, gsum, csum, ng, tau, res, ap, dnew, 1.0);
     if (!ok && threadIdx.x == 0) reinterpret_cast<int *>(res + RES_KIND)[0] = KIND_FAIL;
 }
 
@@ -321,7 +332,9 @@ int rates_impl(stz_ctx *ctx, int decide)
     const long long nb = (long long)ctx->M * ctx->nblk;
     const unsigned gb = (unsigned)((ctx->ngrp + STZ_GRP_PER_BLOCK - 1) / STZ_GRP_PER_BLOCK);
     hipLaunchKernelGGL(k_group, dim3(gb), dim3(NT), 0, ctx->stream, ctx->bsum, nb, ctx->gsum, ctx->ngrp);
-    hipLaunchKernelGGL(k_decide, dim3(1), dim3(NT), 0, ctx->stream, ctx->gsum, ctx->ngrp, ctx->res, decide, ctx->k.dt,
+    hipLaunchKernelGGL(k_chunks, dim3((unsigned)((ctx->ngrp + NT - 1) / NT)), dim3(NT), 0, ctx->stream, ctx->gsum,
+                       ctx->ngrp, ctx->csum);
+    hipLaunchKernelGGL(k_decide, dim3(1), dim3(NT), 0, ctx->stream, ctx->csum, ctx->ngrp, ctx->res, decide, ctx->k.dt,
                        ctx->dspec_dev, ctx->acnt, ctx->aoff, ctx->nblk, ctx->acap);
     STZ_CHECK_LAUNCH();
     tmark(ctx, 10);
@@ -336,10 +349,10 @@ int select_impl(stz_ctx *ctx, double tau, int gated)
     tmark(ctx, 11);
     const double *dnew = gated ? ctx->dspec_dev + 1 : nullptr;
     if (ctx->mro)
-        hipLaunchKernelGGL((k_select<R, DIM, true>), dim3(1), dim3(NT), 0, ctx->stream, a, ctx->gsum, ctx->ngrp, tau,
+        hipLaunchKernelGGL((k_select<R, DIM, true>), dim3(1), dim3(NT), 0, ctx->stream, a, ctx->gsum, ctx->csum, ctx->ngrp, tau,
                            ctx->res, gated, apply_args(ctx), dnew);
     else
-        hipLaunchKernelGGL((k_select<R, DIM, false>), dim3(1), dim3(NT), 0, ctx->stream, a, ctx->gsum, ctx->ngrp, tau,
+        hipLaunchKernelGGL((k_select<R, DIM, false>), dim3(1), dim3(NT), 0, ctx->stream, a, ctx->gsum, ctx->csum, ctx->ngrp, tau,
                            ctx->res, gated, apply_args(ctx), dnew);
     STZ_CHECK_LAUNCH();
     tmark(ctx, 12);

@@ -121,6 +121,33 @@ __global__ __launch_bounds__(NT) void kb_group(const RepDev *__restrict__ reps, 
     group_sums(d.bsum, nb, d.gsum, ngrp, (long long)blockIdx.x * STZ_GRP_PER_BLOCK, ws);
 }
 
+// Chunk totals per replica, one workgroup per chunk: the running scan's total of NT
+// group sums (chunk c < nchr), and the athermal count of NT voxel blocks (c < ncha).
+// Both levels let the single-workgroup decision read a few thousand values at most.
+__global__ __launch_bounds__(NT) void kb_chunks(const RepDev *__restrict__ reps, long long ngrp, long long nblk)
+{
+    __shared__ double sh[NW];
+    __shared__ int wsi[NW];
+    const RepDev &d = reps[blockIdx.y];
+    if (d.st->halt) return;
+    const long long c = blockIdx.x;
+    if (c < (ngrp + NT - 1) / NT) {
+        const double t = chunk_total(d.gsum, ngrp, c, sh);
+        if (threadIdx.x == NT - 1) d.csum[c] = t;
+    }
+    if (c < (nblk + NT - 1) / NT) {
+        const long long b = c * NT + threadIdx.x;
+        const int x = wave_sum_int(b < nblk ? d.acnt[b] : 0);
+        if (threadIdx.x % WAVE == 0) wsi[threadIdx.x / WAVE] = x;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int t = 0;
+            for (int i = 0; i < NW; ++i) t += wsi[i];
+            d.ccnt[c] = t;
+        }
+    }
+}
+
 // One workgroup per replica: R, the Q<0 count, the step kind, and the stream positions
 // of the step's random numbers; then the residence-time selection and transform of a
 // thermal event, or the list offsets of an athermal avalanche.  A step whose draws are
@@ -130,17 +157,19 @@ __global__ __launch_bounds__(NT) void kb_decide(const RepDev *__restrict__ reps,
                                                 long long ngrp, int M)
 {
     constexpr int S = Comp<DIM>::S;
-    __shared__ double ws[NW];
     __shared__ int s_kind;
     __shared__ double s_tau;
     __shared__ long long s_k0;
     __shared__ int s_par;
+    __shared__ double s_R;
     const RepDev &d = reps[blockIdx.x];
     RepState *st = d.st;
     if (st->halt) return;
     double *res = d.res;
     const int tid = threadIdx.x;
-    const double Rtot = group_total(d.gsum, ngrp, ws);
+    if (tid == 0) s_R = chunks_total(d.csum, (ngrp + NT - 1) / NT);
+    __syncthreads();
+    const double Rtot = s_R;
     // rate args for the selection: the pending stress increment is in the field now
     RateArgs<R> a = rate_args_of<R>(d, st, N, nblk, M);
     a.pend_comp = -1;
@@ -193,7 +222,9 @@ __global__ __launch_bounds__(NT) void kb_decide(const RepDev *__restrict__ reps,
     __syncthreads();
     int kind = s_kind;
     if (kind == KIND_ATHERMAL) {
-        const long long n = block_offsets(d.acnt, nblk, d.aoff);
+        // exclusive offsets of the athermal chunks (aoff[c]); kb_athermal derives the
+        // block offsets inside each chunk
+        const long long n = block_offsets(d.ccnt, (nblk + NT - 1) / NT, d.aoff);
         if (tid == 0) {
             const long long rpos = st->rpos;
             int p = 0;
@@ -216,7 +247,7 @@ __global__ __launch_bounds__(NT) void kb_decide(const RepDev *__restrict__ reps,
     } else if (kind == KIND_THERMAL) {
         const ApplyArgs ap = apply_args_of(d, st, N, M);
         const double *src = mt.g[s_par] + s_k0;
-        const bool ok = select_block<R, DIM>(a, d.gsum, ngrp, s_tau, res, ap, src, d.half_gamma0);
+        const bool ok = select_block<R, DIM>(a, d.gsum, d.csum, ngrp, s_tau, res, ap, src, d.half_gamma0);
         if (!ok) kind = KIND_FAIL;
     }
     if (tid == 0) {
@@ -264,26 +295,63 @@ __global__ __launch_bounds__(NT_LOCAL) void kb_local(const RepDev *__restrict__ 
                        reinterpret_cast<int *>(res + 15), wred);
 }
 
-// Workgroups per replica of kb_athermal: most steps have no avalanche and every
-// workgroup returns at once, so the grid is capped and each workgroup strides over the
-// voxel blocks (an empty launch over 16 replicas of 128^3 costs ~20 us at one workgroup
-// per block, ~3 us capped).
-constexpr int ATH_GRID = 512;
-
+// Athermal compaction and transforms: one workgroup per chunk of NT voxel blocks.  The
+// rate pass counted the flagged voxels of every block (acnt) and kb_chunks those of every
+// chunk (ccnt); kb_decide turned the chunk counts into offsets (aoff[c]).  A workgroup
+// scans its chunk's block counts for the block offsets and works through the blocks that
+// have flagged voxels -- usually none, so most workgroups read NT counts and return, and
+// the min-Q field is read only where an avalanche is.  The partials socp[blk] are written
+// for those blocks only; kb_finish sums the others as the +0.0 they are.
 template <typename R, int DIM>
 __global__ __launch_bounds__(NT) void kb_athermal(const RepDev *__restrict__ reps, long long N, int M,
                                                   long long nblk)
 {
+    __shared__ int sw[NW];
+    __shared__ int s_nb;
+    __shared__ long long s_blk[NT], s_off[NT];
     const RepDev &d = reps[blockIdx.y];
     const RepState *st = d.st;
     const double *res = d.res;
     if (st->halt || reinterpret_cast<const int *>(res + RES_KIND)[0] != KIND_ATHERMAL) return;
+    const long long c = blockIdx.x;
+    if (d.ccnt[c] == 0) return;                 // uniform over the workgroup
+    const int tid = threadIdx.x, lane = tid % WAVE, w = tid / WAVE;
+    const long long blk = c * NT + tid;
+    const int cnt = blk < nblk ? d.acnt[blk] : 0;
+    // exclusive scan of the block counts (exact integers) and the list of non-empty blocks
+    int x = cnt;
+#pragma unroll
+    for (int dd = 1; dd < WAVE; dd <<= 1) {
+        const int y = __shfl_up(x, dd, WAVE);
+        if (lane >= dd) x += y;
+    }
+    if (lane == WAVE - 1) sw[w] = x;
+    const unsigned long long ne = __ballot(cnt != 0);
+    __syncthreads();
+    int woff = 0;
+    for (int i = 0; i < w; ++i) woff += sw[i];
+    const long long boff = (long long)d.aoff[c] + woff + (x - cnt);
+    __shared__ int sne[NW];
+    if (lane == 0) sne[w] = __popcll(ne);
+    __syncthreads();
+    int nbefore = 0, ntot = 0;
+    for (int i = 0; i < NW; ++i) {
+        if (i < w) nbefore += sne[i];
+        ntot += sne[i];
+    }
+    if (cnt != 0) {
+        const int slot = nbefore + __popcll(ne & ((1ull << lane) - 1ull));
+        s_blk[slot] = blk;
+        s_off[slot] = boff;
+    }
+    if (tid == 0) s_nb = ntot;
+    __syncthreads();
     AthArgs<R> a;
     a.ap = apply_args_of(d, st, N, M);
     a.minq = (const R *)d.minq;
     a.argm = d.argm;
     a.sigma = (const R *)d.sigma;
-    for (int c = 0; c < 6; ++c) a.soff[c] = st->soff[c];
+    for (int cc = 0; cc < 6; ++cc) a.soff[cc] = st->soff[cc];
     a.voxel = d.voxel;
     a.thr = d.thr;
     a.list = d.alist;
@@ -294,25 +362,13 @@ __global__ __launch_bounds__(NT) void kb_athermal(const RepDev *__restrict__ rep
     a.cap = 1ll << 62;
     a.N = N;
     a.socp = d.socp;
-    // the flags of PRE blocks are loaded together, then the blocks run in order
-    constexpr int PRE = 8;
+    const int nb = s_nb;
     int par = 0;
-    for (long long b0 = blockIdx.x; b0 < nblk; b0 += (long long)gridDim.x * PRE) {
-        int f[PRE];
-#pragma unroll
-        for (int i = 0; i < PRE; ++i) {
-            const long long blk = b0 + (long long)i * gridDim.x;
-            const long long v = blk * NT + threadIdx.x;
-            f[i] = (blk < nblk && v < N) ? athermal_flag(a, v) : 0;
-        }
-#pragma unroll
-        for (int i = 0; i < PRE; ++i) {
-            const long long blk = b0 + (long long)i * gridDim.x;
-            if (blk < nblk) {
-                athermal_block_f<R, DIM>(a, blk, f[i], par);
-                par ^= 1;
-            }
-        }
+    for (int i = 0; i < nb; ++i) {
+        const long long b = s_blk[i];
+        const long long v = b * NT + tid;
+        athermal_block_f<R, DIM>(a, b, v < N ? athermal_flag(a, v) : 0, par, s_off[i]);
+        par ^= 1;
     }
 }
 
@@ -483,7 +539,7 @@ __global__ __launch_bounds__(STZ_NT_MEAN) void kb_finish(const RepDev *__restric
     const int local = reinterpret_cast<const int *>(res + RES_LOCAL)[0];
     if (solved) mean_partials<C>(d.spart, local ? nparts_local : nparts, invN, mean);
     double soc = 0.0;
-    if (kind == KIND_ATHERMAL) soc = ordered_sum<STZ_NT_MEAN>(d.socp, nblk);
+    if (kind == KIND_ATHERMAL) soc = ordered_sum_flagged<STZ_NT_MEAN>(d.socp, d.acnt, d.ccnt, nblk);
     if (threadIdx.x != 0) return;
     // the solve gates of this step are spent (every thread has read them: the reductions
     // above end in barriers, and without a reduction no other thread uses them)
@@ -766,6 +822,8 @@ void fill_desc(stz_ctx *ctx)
     d.argm = ctx->argm;
     d.bsum = ctx->bsum;
     d.gsum = ctx->gsum;
+    d.csum = ctx->csum;
+    d.ccnt = ctx->ccnt;
     d.socp = ctx->socp;
     d.spart = ctx->spart;
     d.res = ctx->res;
@@ -918,8 +976,10 @@ int launch_class(StzPlan *pl, StzClass &k, hipEvent_t ea, hipEvent_t eb)
     if (eb) (void)hipEventRecord(eb, s);
     const unsigned gb = (unsigned)((p->ngrp + STZ_GRP_PER_BLOCK - 1) / STZ_GRP_PER_BLOCK);
     hipLaunchKernelGGL(kb_group, dim3(gb, nrep), dim3(NT), 0, s, k.d_reps, (long long)p->M * p->nblk, p->ngrp);
+    const long long nch = std::max((p->ngrp + NT - 1) / NT, (p->nblk + NT - 1) / NT);
+    hipLaunchKernelGGL(kb_chunks, dim3((unsigned)nch, nrep), dim3(NT), 0, s, k.d_reps, p->ngrp, p->nblk);
     hipLaunchKernelGGL((kb_decide<R, DIM>), dim3(nrep), dim3(NT), 0, s, k.d_reps, p->N, p->nblk, p->ngrp, p->M);
-    hipLaunchKernelGGL((kb_athermal<R, DIM>), dim3((unsigned)std::min<long long>(p->nblk, ATH_GRID), nrep), dim3(NT), 0,
+    hipLaunchKernelGGL((kb_athermal<R, DIM>), dim3((unsigned)((p->nblk + NT - 1) / NT), nrep), dim3(NT), 0,
                        s, k.d_reps, p->N, p->M, p->nblk);
     if (hipGetLastError() != hipSuccess) return STZ_E_HIP;
     int nparts = 0;
@@ -1223,8 +1283,11 @@ int stz_plan_run(StzPlan *pl, int nsteps, stz_step_result *const *results, int *
         }
         RS_HIP(o, hipStreamSynchronize(pl->stream));
         if (pl->timed > 0) {
+            // only the steps that recorded ran the rate pass: the launches after a halt
+            // (print, budget, window, error) return at once and are left out of the mean
             stz_ctx *p = pl->all[0];
-            for (int i = 0; i < pl->timed; ++i) {
+            const int worked = std::min(pl->timed, p->rs->hst.nrec);
+            for (int i = 0; i < worked; ++i) {
                 float ms = 0.f;
                 if (hipEventElapsedTime(&ms, pl->eva[(size_t)i], pl->evb[(size_t)i]) == hipSuccess) {
                     p->tms[0] += ms;

@@ -4,7 +4,8 @@ Each fixture holds, per main-loop step of the CPU FP64 oracle, the step kind, th
 selected (site, mode) of a thermal event, the number of voxels of an athermal
 avalanche, the clock and the loaded component of the mean stress.  The FP64 GPU path
 replays the same seed and must reproduce the event sequence exactly and the
-stress-strain curve within 1e-10 relative.
+stress-strain curve within 1e-10 relative; where the trace holds stress-field snapshots
+(a fixed sample of voxels, all components), the field within 1e-10 relative.
 """
 import ast
 import os
@@ -13,6 +14,7 @@ import numpy as np
 import pytest
 
 import metallic_glasses_amd as mg
+from trace_check import check_sigma, sigma_steps
 
 pytestmark = pytest.mark.gpu
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
@@ -29,8 +31,11 @@ def test_event_trace_matches_oracle(name):
     lc = 0 if kw.get("loading", 0) == 0 else (2 if kw["dim"] == 2 else 5)
     worst = 0.0
     n = len(d["kind"])
+    snaps = {s: k for k, s in enumerate(sigma_steps(d))}
     for i in range(n):
         rc, r = g.step()
+        if i in snaps:
+            check_sigma(g, d, snaps[i])
         k = int(d["kind"][i])
         assert r.kind == k, (i, r.kind, k)
         if k == 1:

@@ -16,6 +16,7 @@ import pytest
 
 import pyoracle as po
 import metallic_glasses_amd as mg
+from trace_check import run_resident_with_snapshots
 
 pytestmark = pytest.mark.gpu
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
@@ -249,7 +250,7 @@ def test_resident_local_mixed_tracks_fp64():
 
 
 @pytest.mark.parametrize("resync,local_max", [(1024, 16), (100, 1)], ids=["device", "host_fallbacks"])
-@pytest.mark.parametrize("name", ["shear3d_64", "ref2d_512"])
+@pytest.mark.parametrize("name", ["shear3d_64", "ref2d_512", "shear3d_128"])
 def test_resident_local_replays_golden_trace(name, resync, local_max):
     """(100, 1): every 2-voxel avalanche and every 100th update run on the host path."""
     path = os.path.join(GOLD, f"trace_{name}.npz")
@@ -262,7 +263,7 @@ def test_resident_local_replays_golden_trace(name, resync, local_max):
     g = mg.Stz(**kw)
     g.set_print(int(d["printperiod"]), float(d["endstrain"]))
     g.set_solver(mg.SOLVER_LOCAL, resync, local_max)
-    rc, recs = g.run_steps(n)
+    recs, errs = run_resident_with_snapshots(g, d, n)
     assert len(recs) == n
     lc = 0 if kw.get("loading", 0) == 0 else (2 if kw["dim"] == 2 else 5)
     kinds = np.array([r.kind for r in recs])
@@ -275,7 +276,8 @@ def test_resident_local_replays_golden_trace(name, resync, local_max):
     st = np.array([r.stress[lc] for r in recs])
     worst = float(np.max(np.abs(st - d["stress"]) / np.maximum(np.abs(d["stress"]), 1e-3)))
     assert worst < 1e-10, worst
-    print(f"{name}: {int(th.sum())} events identical with the local update, worst stress {worst:.2e}", flush=True)
+    print(f"{name}: {int(th.sum())} events identical with the local update, worst stress {worst:.2e}, "
+          f"field snapshots {errs}", flush=True)
 
 
 def test_ensemble_local_equals_single_runs():

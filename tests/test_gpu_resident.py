@@ -16,6 +16,7 @@ import numpy as np
 import pytest
 
 import metallic_glasses_amd as mg
+from trace_check import run_resident_with_snapshots
 
 pytestmark = pytest.mark.gpu
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
@@ -147,7 +148,7 @@ def test_resident_replays_golden_trace(name):
     print("resident trace", name, n, "steps", flush=True)
     g = mg.Stz(**kw)
     g.set_print(int(d["printperiod"]), float(d["endstrain"]))
-    rc, recs = g.run_steps(n)
+    recs, errs = run_resident_with_snapshots(g, d, n)
     assert len(recs) == n
     lc = 0 if kw.get("loading", 0) == 0 else (2 if kw["dim"] == 2 else 5)
     kinds = np.array([r.kind for r in recs])
@@ -162,7 +163,8 @@ def test_resident_replays_golden_trace(name):
     assert worst < 1e-10, worst
     tt = np.array([r.totime for r in recs])
     assert np.max(np.abs(tt - d["totime"]) / np.maximum(np.abs(d["totime"]), 1e-30)) <= 1e-10
-    print(f"{name}: {int(th.sum())} events identical through the resident step, worst stress {worst:.2e}", flush=True)
+    print(f"{name}: {int(th.sum())} events identical through the resident step, worst stress {worst:.2e}, "
+          f"field snapshots {errs}", flush=True)
 
 
 ENSEMBLE = [
