@@ -34,6 +34,7 @@ extern "C" {
 #define STZ_E_ARG (-3)         /* bad argument / unsupported configuration */
 #define STZ_E_NAN (-4)         /* NaN in the stress solve (STZ/elastic.c:212-215) */
 #define STZ_E_UNSUPPORTED (-5) /* crystal-like MRO voxel drawn, or an unsupported combination */
+#define STZ_E_TOO_LARGE (-6)   /* stz_rates_noalloc: modes * ceil(voxels / 256) above 2^21 */
 
 #define STZ_FP64 0
 #define STZ_MIXED 1
@@ -376,6 +377,27 @@ int stz_timing_enable(stz_ctx *ctx, int on);
 int stz_timing_get(stz_ctx *ctx, double ms[8], long long n[8]);
 /* per-kernel timing inside the stress solve: k_fwd_x, k_ypass(fwd), k_fused, k_ypass(inv), k_inv_x */
 int stz_timing_solve_get(stz_ctx *ctx, double ms[8], long long n[8]);
+
+/* Device pointers of the fields stz_rates_noalloc reads (FP64, layouts as stz_upload:
+   SIGMA [C][N], Q0 [M][N], DEPS [M][S][N], DEPS_LAST [S][N], PERM/TEMP/TLAST [N]). */
+typedef struct stz_rate_fields {
+    const double *sigma, *q0, *deps, *deps_last, *perm, *temp, *tlast;
+} stz_rate_fields;
+
+/* stz_rates without a context and without allocating: SofteningFactor + ActEnergy + the
+   Q<0 count + the KMC rate total over the caller's device fields at clock `totime`, with
+   the constants of *p (FP64 glass: p->precision must be STZ_FP64; no crystal-like MRO).
+   No counterpart in the reference: it is for callers that cannot call hipMalloc (code
+   under HIP graph capture, serving loops that own their buffers).  Kernels are enqueued
+   on `stream` (a hipStream_t; NULL = the null stream) and the results are written to
+   the caller's device memory (*total_rate, *n_negative) when they run; the call neither
+   allocates nor synchronises.  Its block and group rate sums live in one fixed device
+   array of 16 MiB per GPU, held until the process ends: modes * ceil(voxels / 256) must
+   not exceed 2^21 (e.g. 20 modes up to 6.7 M voxels), else STZ_E_TOO_LARGE.  Calls that
+   overlap on one GPU share that array and race; R and the count equal stz_rates' bit for
+   bit on the same fields. */
+int stz_rates_noalloc(const stz_params *p, const stz_rate_fields *f, double totime, void *stream, double *total_rate,
+                      long long *n_negative);
 
 #ifdef __cplusplus
 }

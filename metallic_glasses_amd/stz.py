@@ -136,6 +136,7 @@ def lib():
         L.stz_mt_window_probe.argtypes = [C.c_ulong, C.c_longlong, C.c_longlong, C.c_longlong, C.c_int, C.c_int,
                                           dp, dp, P(C.c_longlong), P(C.c_longlong)]
         L.stz_copy_probe.argtypes = [C.c_longlong, C.c_int, dp, dp]
+        L.stz_rates_noalloc.argtypes = [P(Params), P(RateFields), C.c_double, vp, C.c_void_p, C.c_void_p]
         L.stz_stress_update_local.argtypes = [vp, C.c_int, P(C.c_int)]
         L.stz_set_solver.argtypes = [vp, C.c_int, C.c_longlong, C.c_int]
         L.stz_checkpoint_save.argtypes = [vp, C.c_char_p]
@@ -494,6 +495,22 @@ def mt_window_probe(seed, skip, window_words, offset, with_uniform, ngauss):
     if rc:
         raise StzError(rc, "stz_mt_window_probe")
     return w[:n], s[:n], we.value, se.value
+
+
+class RateFields(C.Structure):
+    """Device pointers of stz_rates_noalloc's fields (include/stz.h stz_rate_fields)."""
+    _fields_ = [(k, C.c_void_p) for k in ("sigma", "q0", "deps", "deps_last", "perm", "temp", "tlast")]
+
+
+def rates_noalloc(kw, ptrs, totime, total_rate_ptr, n_negative_ptr, stream=None):
+    """stz_rates_noalloc: R and the Q<0 count of FP64 fields in device memory (ptrs: a dict
+    of device addresses by RateFields name), written to device memory at total_rate_ptr
+    (double) and n_negative_ptr (int64) when the kernels run on `stream` (None: the null
+    stream).  Returns the C return code."""
+    p = make_params(**kw)
+    f = RateFields(**{k: C.c_void_p(int(v)) for k, v in ptrs.items()})
+    return lib().stz_rates_noalloc(C.byref(p), C.byref(f), float(totime), stream,
+                                   C.c_void_p(int(total_rate_ptr)), C.c_void_p(int(n_negative_ptr)))
 
 
 def copy_probe(nbytes, reps=10):
