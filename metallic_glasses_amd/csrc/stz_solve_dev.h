@@ -547,6 +547,22 @@ constexpr bool x_variant(int L0, int NR)
            (long long)L0 * (C * NR / 2 + 1) * (long long)sizeof(cpx<R>) <= 65536;
 }
 
+// The workgroups of a launch are dispatched to the chip's 8 XCDs in turn (linear id
+// mod 8), and each XCD has its own L2.  The column tiles of the strided passes are
+// narrower than a 128-byte line (W complex values of a row), so the neighbouring tiles
+// that share a line must run on one XCD: the linear id is remapped so that XCD x works
+// through one contiguous eighth of the tiles, in order.  A bijection of the grid.
+constexpr unsigned STZ_NXCD = 8;
+__device__ __forceinline__ uint3 xcd_block()
+{
+    const unsigned gx = gridDim.x, gy = gridDim.y;
+    const unsigned T = gx * gy * gridDim.z;
+    const unsigned L = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+    const unsigned x = L % STZ_NXCD, i = L / STZ_NXCD, q = T / STZ_NXCD, r = T % STZ_NXCD;
+    const unsigned Lp = x * q + (x < r ? x : r) + i;
+    return make_uint3(Lp % gx, (Lp / gx) % gy, Lp / (gx * gy));
+}
+
 // ---------------------------------------------------------------------------------
 // k_ypass: complex FFT along y (stride hp) for a tile of W k0-columns, in place
 // ---------------------------------------------------------------------------------
@@ -554,16 +570,17 @@ template <typename R, int L1, int W>
 __global__ __launch_bounds__(NT_SOLVE) void k_ypass(cpx<R> *__restrict__ spec_a, const cpx<R> *__restrict__ twg,
                                                     SolveConsts k, R sgn, const SolveRep *reps)
 {
-    const View<R> vw = view_of<R>(reps, blockIdx.z / k.C, k, nullptr, spec_a, nullptr, nullptr, nullptr);
+    const uint3 bk = xcd_block();
+    const View<R> vw = view_of<R>(reps, bk.z / k.C, k, nullptr, spec_a, nullptr, nullptr, nullptr);
     if (gated_off_p(vw.gate)) return;
     cpx<R> *__restrict__ spec = vw.spec;
     __shared__ cpx<R> buf[L1 * W];
     __shared__ cpx<R> tw[L1];
     const int tid = threadIdx.x;
     for (int i = tid; i < L1; i += NT_SOLVE) tw[i] = twg[i];
-    const int k0base = blockIdx.x * W;
-    const int i2 = blockIdx.y;
-    const int c = blockIdx.z % k.C;
+    const int k0base = bk.x * W;
+    const int i2 = bk.y;
+    const int c = bk.z % k.C;
     cpx<R> *base = spec + ((long long)c * k.rows + (long long)i2 * L1) * k.hp + k0base;
     // line = column of the tile; the first pass reads the columns from the spectrum and
     // the last pass writes them back
@@ -579,7 +596,7 @@ __global__ __launch_bounds__(NT_SOLVE) void k_ypass(cpx<R> *__restrict__ spec_a,
 
 // ---------------------------------------------------------------------------------
 // k_fused: last axis forward FFT, Gamma-hat, inverse FFT for one tile of W k0-columns
-// (3-D: at fixed y index blockIdx.y; 2-D: the y axis itself is the last axis).  The
+// (3-D: at a fixed y index; 2-D: the y axis itself is the last axis).  The
 // tile of all C components (LZ x C*W complex values) stays in LDS from load to store:
 // the C*W lines are transformed together, Gamma-hat mixes the C components of each of
 // the LZ*W wavevectors in place, and the inverse transform follows.
@@ -590,15 +607,16 @@ __global__ __launch_bounds__(NTF) void k_fused(cpx<R> *__restrict__ spec_a, cons
 {
     constexpr int NL = C * W;             // lines, component-major: line = c*W + col
     constexpr int NE = LZ * NL;
-    const View<R> vw = view_of<R>(reps, blockIdx.z, k, nullptr, spec_a, nullptr, nullptr, nullptr);
+    const uint3 bk = xcd_block();
+    const View<R> vw = view_of<R>(reps, bk.z, k, nullptr, spec_a, nullptr, nullptr, nullptr);
     if (gated_off_p(vw.gate)) return;
     cpx<R> *__restrict__ spec = vw.spec;
     __shared__ cpx<R> buf[NE];
     __shared__ cpx<R> tw[LZ];
     const int tid = threadIdx.x;
     for (int i = tid; i < LZ; i += NTF) tw[i] = twg[i];
-    const int k0base = blockIdx.x * W;
-    const int i1 = (DIM == 3) ? blockIdx.y : 0;
+    const int k0base = bk.x * W;
+    const int i1 = (DIM == 3) ? bk.y : 0;
     // element (p, col) of component c lives at spec[(c*rows + p*pstride + i1) * hp + k0]
     const long long pstride = (DIM == 3) ? k.n[1] : 1;
     auto gaddr = [&](int line, int p) {
