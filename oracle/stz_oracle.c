@@ -105,6 +105,10 @@
 #include <omp.h>
 #endif
 
+/* loops over fewer elements than this run on one thread: on the tiny grids of the pin
+   tests a parallel region costs more than it saves (results are the same either way) */
+#define ORA_PAR_MIN 32768L
+
 /* ------------------------------------------------------------------ */
 /* constants of STZ/bmg.h:42,68-74                                       */
 /* ------------------------------------------------------------------ */
@@ -276,7 +280,7 @@ int ora_fft3d(double *data, int n0, int n1, int n2, int sign)
         const long st = strides[ax];
         const long nlines = ntot / n;
         int bad = 0;
-#pragma omp parallel
+#pragma omp parallel if(ntot >= ORA_PAR_MIN)
         {
             double *line = (double *)malloc(sizeof(double) * 4 * (size_t)n);
             if (!line) {
@@ -738,12 +742,12 @@ static void hom_solve(ora_state *s)
     double *sp = s->spec;
     for (int c = 0; c < C; c++) {
         double *f = sp + 2 * (long)c * N;
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if(N >= ORA_PAR_MIN)
         for (long v = 0; v < N; v++) { f[2 * v] = s->epsolast[(long)c * N + v]; f[2 * v + 1] = 0.; }
         ora_fft3d(f, s->p.n[0], s->p.n[1], s->p.n[2], -1);
     }
     const double s0 = 1.0 / mu, sh = 0.5 / mu, sa = -1.0 * alpha / mu;
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if(N >= ORA_PAR_MIN)
     for (long v = 0; v < N; v++) {
         for (int ri = 0; ri < 2; ri++) {
             double e[6] = {0}, t[6];
@@ -778,11 +782,11 @@ static void hom_solve(ora_state *s)
     for (int c = 0; c < C; c++) {
         double *f = sp + 2 * (long)c * N;
         ora_fft3d(f, s->p.n[0], s->p.n[1], s->p.n[2], +1);
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if(N >= ORA_PAR_MIN)
         for (long v = 0; v < N; v++) s->epso[(long)c * N + v] = f[2 * v] / (double)N + s->epsavg[c]; /* :822 */
     }
     /* IsotropicMul_R2, STZ/elastic.c:686-701 */
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if(N >= ORA_PAR_MIN)
     for (long v = 0; v < N; v++) {
         double d[6] = {0};
         for (int c = 0; c < C; c++) d[c] = s->epso[(long)c * N + v] - s->epsolast[(long)c * N + v];
@@ -811,7 +815,7 @@ static void stress_solve_uniform(ora_state *s)
     const long N = s->N;
     const int C = s->C;
     const double lambda = s->lambda, mu = s->mu;
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if(N >= ORA_PAR_MIN)
     for (long v = 0; v < N; v++) {
         double e[6] = {0}, t0[6], t1[6], tau[6], z[6] = {0}, r[6];
         for (int c = 0; c < C; c++) { e[c] = s->eps0[(long)c * N + v]; z[c] = e[c] + s->epsavg[c]; }
@@ -850,7 +854,7 @@ static void stress_solve_inhom(ora_state *s)
     const long N = s->N;
     const int C = s->C;
     const double lambda = s->lambda, mu = s->mu;
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if(N >= ORA_PAR_MIN)
     for (long v = 0; v < N; v++) {
         double e[6] = {0}, z[6] = {0}, t0[6], t1[6], tau[6], r[6];
         for (int c = 0; c < C; c++) { e[c] = s->eps0[(long)c * N + v]; z[c] = e[c] + s->epsavg[c]; }  /* :993-999 */
@@ -868,7 +872,7 @@ static void stress_solve_inhom(ora_state *s)
         step++;
         memcpy(s->epsolast, s->estar, sizeof(double) * (size_t)C * (size_t)N);   /* :1017-1022 */
         hom_solve(s);                                                     /* epso = total strain :1025 */
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if(N >= ORA_PAR_MIN)
         for (long v = 0; v < N; v++) {
             double z[6] = {0}, t1[6], tau[6], r[6];
             for (int c = 0; c < C; c++) z[c] = s->epso[(long)c * N + v];
@@ -934,12 +938,12 @@ void ora_displacement(ora_state *s, double *u)
     double *sp = s->spec;
     for (int c = 0; c < C; c++) {
         double *f = sp + 2 * (long)c * N;
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if(N >= ORA_PAR_MIN)
         for (long v = 0; v < N; v++) { f[2 * v] = s->epsolast[(long)c * N + v]; f[2 * v + 1] = 0.; }
         ora_fft3d(f, n[0], n[1], n[2], -1);
     }
     const double sdiv = 1.0 / mu;
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if(N >= ORA_PAR_MIN)
     for (long v = 0; v < N; v++) {
         const int idx[3] = {(int)(v % n[0]), (int)((v / n[0]) % n[1]), (int)(v / ((long)n[0] * n[1]))};
         double kk[3] = {0, 0, 0};
@@ -977,7 +981,7 @@ void ora_displacement(ora_state *s, double *u)
     for (int a = 0; a < D; a++) {
         double *f = sp + 2 * (long)a * N;
         ora_fft3d(f, n[0], n[1], n[2], +1);
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if(N >= ORA_PAR_MIN)
         for (long v = 0; v < N; v++) {
             const int idx[3] = {(int)(v % n[0]), (int)((v / n[0]) % n[1]), (int)(v / ((long)n[0] * n[1]))};
             const double *e = s->epsavg;
@@ -1004,7 +1008,7 @@ void ora_displacement(ora_state *s, double *u)
 /* ------------------------------------------------------------------ */
 void ora_softening(ora_state *s)
 {
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if(s->N >= ORA_PAR_MIN)
     for (long v = 0; v < s->N; v++)
         s->soft[v] = s->perm[v] + s->temp[v] * exp((s->tlast[v] - s->totime) / s->tTemp); /* :731 */
 }
@@ -1015,7 +1019,7 @@ long long ora_act_energy(ora_state *s)
     const int C = s->C;
     long long nneg = 0;
     for (int m = 0; m < s->M; m++) {
-#pragma omp parallel for schedule(static) reduction(+ : nneg)
+#pragma omp parallel for schedule(static) reduction(+ : nneg) if(N >= ORA_PAR_MIN)
         for (long v = 0; v < N; v++) {
             double sg[6] = {0}, d[6] = {0}, pd[6] = {0};
             for (int c = 0; c < C; c++) sg[c] = s->sigma[(long)c * N + v];
@@ -1047,7 +1051,7 @@ double ora_total_rate(ora_state *s)
 {
     const long MN = (long)s->M * s->N;
     double t = 0;
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if(MN >= ORA_PAR_MIN)
     for (long i = 0; i < MN; i++) s->rate[i] = s->ratepre * exp(-s->q[i] / s->kT);   /* STZ/evolution.c:125 */
     for (long i = 0; i < MN; i++) t += s->rate[i];                                  /* :126, sequential */
     return t;
@@ -1060,7 +1064,7 @@ int ora_select(ora_state *s, double dice, int *site, int *mode)
     double R = 0;
     for (long i = 0; i < MN; i++) R += s->rate[i];
     double tk = 1 / R;
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if(MN >= ORA_PAR_MIN)
     for (long i = 0; i < MN; i++) s->rate[i] *= tk;
     for (long i = 1; i < MN; i++) s->rate[i] += s->rate[i - 1];
     long ev;
@@ -1144,11 +1148,11 @@ int ora_heat_conduct(ora_state *s, double dt)
 {
     const long N = s->N;
     double *f = s->spec;                      /* scratch: one complex field */
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if(N >= ORA_PAR_MIN)
     for (long v = 0; v < N; v++) { f[2 * v] = s->tlocal[v]; f[2 * v + 1] = 0.0; }
     ora_fft3d(f, s->p.n[0], s->p.n[1], s->p.n[2], -1);
     int bad = 0;
-#pragma omp parallel for schedule(static) reduction(|:bad)
+#pragma omp parallel for schedule(static) reduction(|:bad) if(N >= ORA_PAR_MIN)
     for (long v = 0; v < N; v++) {
         const double m = exp(-s->g2[v] * dt);
         f[2 * v] *= m;
@@ -1157,7 +1161,7 @@ int ora_heat_conduct(ora_state *s, double dt)
     }
     if (bad) return -4;
     ora_fft3d(f, s->p.n[0], s->p.n[1], s->p.n[2], +1);
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if(N >= ORA_PAR_MIN)
     for (long v = 0; v < N; v++) s->tlocal[v] = f[2 * v] / (double)N;
     return 0;
 }
@@ -1170,7 +1174,7 @@ static int athermal(ora_state *s, ora_step_info *info)
     int ntr = 0;
     s->astep++;
     int *pick = (int *)malloc(sizeof(int) * (size_t)N);
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if(N >= ORA_PAR_MIN)
     for (long v = 0; v < N; v++) {
         int mm = 0;
         double minq = s->q[v];
@@ -1225,7 +1229,7 @@ static int kmc(ora_state *s, ora_step_info *info)
                 if (c != lc) s->epsavg[c] += -te[c];
             s->epsavg[lc] += sr * t_kmc;
             s->stress[lc] += 2 * s->mu * sr * t_kmc;
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if(N >= ORA_PAR_MIN)
             for (long v = 0; v < N; v++) s->sigma[(long)lc * N + v] += 2 * s->mu * sr * t_kmc;
         } else {                              /* TENSION, STZ/evolution.c:281-297 */
             ts[0] = 0.0;
@@ -1239,7 +1243,7 @@ static int kmc(ora_state *s, ora_step_info *info)
                 for (int c = 3; c < 6; c++) s->epsavg[c] += (-te[c]);
             }
             s->stress[0] += s->E * sr * t_kmc;
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if(N >= ORA_PAR_MIN)
             for (long v = 0; v < N; v++) s->sigma[v] += s->E * sr * t_kmc;
         }
         s->totime += t_kmc;
@@ -1256,7 +1260,7 @@ static int kmc(ora_state *s, ora_step_info *info)
         if (s->p.loading == 1) {              /* STZ/evolution.c:349-356 */
             s->epsavg[lc] += sr * 1;
             s->stress[lc] += 2 * s->mu * inc;
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if(N >= ORA_PAR_MIN)
             for (long v = 0; v < N; v++) s->sigma[(long)lc * N + v] += 2 * s->mu * inc;
         } else {                              /* STZ/evolution.c:335-347 */
             s->epsavg[0] += (inc * 1);
@@ -1264,7 +1268,7 @@ static int kmc(ora_state *s, ora_step_info *info)
             if (C == 3) s->epsavg[2] += (inc * 0.0);
             else { s->epsavg[2] += (inc * (-s->nu)); for (int c = 3; c < 6; c++) s->epsavg[c] += (inc * 0.0); }
             s->stress[0] += s->E * inc;
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if(N >= ORA_PAR_MIN)
             for (long v = 0; v < N; v++) s->sigma[v] += s->E * inc;
         }
         info->kind = ORA_ELASTIC;
