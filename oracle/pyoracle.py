@@ -105,6 +105,10 @@ def lib():
         L.ora_inhom_iterations.restype = C.c_int
         L.ora_set_threads.argtypes = [C.c_int]
         L.ora_set_threads.restype = C.c_int
+        L.ora_save.argtypes = [C.c_void_p, C.c_char_p]
+        L.ora_save.restype = C.c_int
+        L.ora_load.argtypes = [C.c_void_p, C.c_char_p]
+        L.ora_load.restype = C.c_int
         _lib = L
     return _lib
 
@@ -252,6 +256,18 @@ class Oracle:
         info = OraStepInfo()
         rc = lib().ora_step(self._h, C.byref(info))
         return rc, info
+
+    def save(self, path):
+        """Writes the whole state (fields, scalars, RNG, counters, print schedule)."""
+        rc = lib().ora_save(self._h, os.fsencode(path))
+        if rc:
+            raise RuntimeError(f"ora_save({path}) failed: {rc}")
+
+    def load(self, path):
+        """Replaces the whole state by an image of ora_save (same parameters)."""
+        rc = lib().ora_load(self._h, os.fsencode(path))
+        if rc:
+            raise RuntimeError(f"ora_load({path}) failed: {rc}")
 
 
 def set_threads(n):

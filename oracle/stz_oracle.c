@@ -99,6 +99,7 @@
 #include "stz_oracle.h"
 
 #include <math.h>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 #ifdef _OPENMP
@@ -549,6 +550,75 @@ void ora_destroy(ora_state *s)
     for (int a = 0; a < 3; a++) free(s->g[a]);
     for (int a = 0; a < 6; a++) free(s->gg[a]);
     free(s);
+}
+
+/* ------------------------------------------------------------------ */
+/* state image (test infrastructure: lets a long golden run resume)     */
+/* ------------------------------------------------------------------ */
+#define ORA_IMAGE_MAGIC 0x4f524131u   /* "ORA1" */
+
+/* every array of the state with its length in bytes (NULL arrays have length 0) */
+static int state_arrays(ora_state *s, void **ptr[32], size_t len[32])
+{
+    const size_t N = (size_t)s->N, C = (size_t)s->C, S = (size_t)s->S, M = (size_t)s->M, d = sizeof(double);
+    int k = 0;
+#define ORA_ARR(member, bytes) do { ptr[k] = (void **)&s->member; len[k] = s->member ? (bytes) : 0; k++; } while (0)
+    ORA_ARR(eps0, C * N * d); ORA_ARR(sigma, C * N * d); ORA_ARR(epso, C * N * d);
+    ORA_ARR(q0, M * N * d); ORA_ARR(q1, M * N * d); ORA_ARR(deps, M * S * N * d); ORA_ARR(deps_last, S * N * d);
+    ORA_ARR(perm, N * d); ORA_ARR(temp, N * d); ORA_ARR(tlast, N * d); ORA_ARR(q, M * N * d);
+    ORA_ARR(soft, N * d); ORA_ARR(rate, M * N * d); ORA_ARR(spec, 2 * C * N * d); ORA_ARR(tau0, C * N * d);
+    ORA_ARR(epsolast, C * N * d); ORA_ARR(tlocal, N * d); ORA_ARR(g2, N * d); ORA_ARR(mro, N);
+    ORA_ARR(mroth, N * d); ORA_ARR(lamx, N * d); ORA_ARR(mux, N * d); ORA_ARR(estar, C * N * d);
+    for (int a = 0; a < 3; a++) ORA_ARR(g[a], N * d);
+    for (int a = 0; a < 6; a++) ORA_ARR(gg[a], N * d);
+#undef ORA_ARR
+    return k;
+}
+
+int ora_save(ora_state *s, const char *path)
+{
+    if (!s || !path) return -1;
+    FILE *f = fopen(path, "wb");
+    if (!f) return -2;
+    void **ptr[32];
+    size_t len[32];
+    const int na = state_arrays(s, ptr, len);
+    const unsigned magic = ORA_IMAGE_MAGIC;
+    int ok = fwrite(&magic, sizeof magic, 1, f) == 1 && fwrite(s, sizeof *s, 1, f) == 1 &&
+             fwrite(len, sizeof(size_t), (size_t)na, f) == (size_t)na;
+    for (int i = 0; ok && i < na; i++)
+        if (len[i]) ok = fwrite(*ptr[i], 1, len[i], f) == len[i];
+    ok = (fclose(f) == 0) && ok;
+    return ok ? 0 : -3;
+}
+
+/* s must have been made with the same parameters (and set_moduli if the image has
+   per-voxel moduli); every scalar and array of the image replaces s's own */
+int ora_load(ora_state *s, const char *path)
+{
+    if (!s || !path) return -1;
+    FILE *f = fopen(path, "rb");
+    if (!f) return -2;
+    unsigned magic = 0;
+    ora_state img;
+    void **ptr[32];
+    size_t len[32], want[32];
+    const int na = state_arrays(s, ptr, want);
+    int ok = fread(&magic, sizeof magic, 1, f) == 1 && magic == ORA_IMAGE_MAGIC && fread(&img, sizeof img, 1, f) == 1 &&
+             fread(len, sizeof(size_t), (size_t)na, f) == (size_t)na;
+    ok = ok && img.N == s->N && img.C == s->C && img.M == s->M && img.S == s->S;
+    for (int i = 0; ok && i < na; i++) ok = len[i] == want[i];
+    if (ok) {
+        /* the image's scalars with s's own arrays */
+        void *keep[32];
+        for (int i = 0; i < na; i++) keep[i] = *ptr[i];
+        *s = img;
+        for (int i = 0; i < na; i++) *ptr[i] = keep[i];
+        for (int i = 0; ok && i < na; i++)
+            if (len[i]) ok = fread(*ptr[i], 1, len[i], f) == len[i];
+    }
+    fclose(f);
+    return ok ? 0 : -3;
 }
 
 static void draw_deps(ora_state *s, int m, long v)

@@ -126,6 +126,9 @@ int ora_step(ora_state *s, ora_step_info *info);
    prints at every step, as the reference's test |d strain| >= printperiod*inc*0.99 does */
 void ora_set_print(ora_state *s, int printperiod, double endstrain);
 void ora_set_strainhist(ora_state *s, double last); /* strainhist[N_print-1], STZ/bmg.c:182 */
+/* whole-state image to a file and back (same parameters); 0 on success */
+int ora_save(ora_state *s, const char *path);
+int ora_load(ora_state *s, const char *path);
 int  ora_set_threads(int n);   /* OpenMP threads (n <= 0: query); returns the count in use */
 
 #ifdef __cplusplus

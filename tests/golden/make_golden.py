@@ -7,7 +7,7 @@ trace_<name>.npz     event traces of the CPU FP64 oracle (oracle/liboracle.so) f
                      replay the same seed and compare step by step.
 
 Run: python tests/golden/make_golden.py fft
-     python tests/golden/make_golden.py trace ref2d_512 [events] [threads]
+     python tests/golden/make_golden.py trace ref2d_512 [events] [threads] [state_dir]
 """
 import os
 import sys
@@ -53,11 +53,13 @@ def sample_voxels(N, k=SIGMA_SAMPLE, seed=20261204):
     return np.sort(rng.choice(N, size=min(k, N), replace=False)).astype(np.int64)
 
 
-def make_trace(name, events, threads=8, printperiod=5, endstrain=1.0, save_every=1000):
+def make_trace(name, events, threads=8, printperiod=5, endstrain=1.0, save_every=1000, state_dir=None):
     """Runs the oracle until `events` thermal events and writes trace_<name>.npz.
 
     The file is rewritten every `save_every` thermal events, so a long run leaves a
-    usable (shorter) trace if it is stopped early."""
+    usable (shorter) trace if it is stopped early.  With `state_dir`, the oracle's whole
+    state is written there at the same points, and a later call resumes from the last
+    one whose step count matches the trace file."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as po
     po.set_threads(threads)
@@ -71,6 +73,24 @@ def make_trace(name, events, threads=8, printperiod=5, endstrain=1.0, save_every
     sigma_at = SIGMA_AT.get(name, ())
     vox = sample_voxels(o.N) if sigma_at else np.zeros(0, np.int64)
     snap_step, snap_sigma = [], []
+    out = os.path.join(HERE, f"trace_{name}.npz")
+
+    def image(nsteps):
+        return os.path.join(state_dir, f"state_{name}_{nsteps}.img")
+
+    if state_dir and os.path.exists(out):
+        with np.load(out) as z:
+            n = len(z["kind"])
+            if os.path.exists(image(n)) and str(z["params"]) == repr(sorted(kw.items())):
+                o.load(image(n))
+                rows = [tuple(r) for r in zip(z["kind"].tolist(), z["site"].tolist(), z["mode"].tolist(),
+                                              z["ntrans"].tolist(), z["totime"].tolist(), z["stress"].tolist(),
+                                              z["printed"].tolist())]
+                nev = int(np.sum(z["kind"] == 1))
+                if sigma_at:
+                    snap_step = z["sigma_step"].tolist()
+                    snap_sigma = [a for a in z["sigma"]]
+                print(f"{name}: resumed at {n} steps, {nev} events", flush=True)
 
     def save():
         a = np.array(rows, dtype=object)
@@ -79,13 +99,20 @@ def make_trace(name, events, threads=8, printperiod=5, endstrain=1.0, save_every
         if sigma_at:
             extra = dict(sigma_voxels=vox, sigma_step=np.array(snap_step, dtype=np.int64),
                          sigma=np.array(snap_sigma, dtype=np.float64).reshape(len(snap_step), o.C, len(vox)))
+        if state_dir:
+            os.makedirs(state_dir, exist_ok=True)
+            o.save(image(len(rows)))
         np.savez_compressed(tmp,
                             kind=a[:, 0].astype(np.int8), site=a[:, 1].astype(np.int32),
                             mode=a[:, 2].astype(np.int8), ntrans=a[:, 3].astype(np.int32),
                             totime=a[:, 4].astype(np.float64), stress=a[:, 5].astype(np.float64),
                             printed=a[:, 6].astype(np.int8), printperiod=printperiod, endstrain=endstrain,
                             params=np.array(repr(sorted(kw.items()))), **extra)
-        os.replace(tmp, os.path.join(HERE, f"trace_{name}.npz"))
+        os.replace(tmp, out)
+        if state_dir:
+            for f in os.listdir(state_dir):
+                if f.startswith(f"state_{name}_") and f != os.path.basename(image(len(rows))):
+                    os.remove(os.path.join(state_dir, f))
 
     while nev < events:
         rc, info = o.step()
@@ -113,4 +140,5 @@ if __name__ == "__main__":
         make_fft()
     elif what[0] == "trace":
         make_trace(what[1], int(what[2]) if len(what) > 2 else 10000,
-                   threads=int(what[3]) if len(what) > 3 else 8)
+                   threads=int(what[3]) if len(what) > 3 else 8,
+                   state_dir=what[4] if len(what) > 4 else None)
