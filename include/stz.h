@@ -310,18 +310,22 @@ int stz_ensemble_run(stz_ensemble *e, int nsteps, stz_step_result *const *result
 void stz_ensemble_destroy(stz_ensemble *e);
 
 /* Autotuner of the stress-solve schedule (no counterpart in the reference): times every
-   compiled tile variant of the four solve kernel slots -- x-pass rows (forward, inverse),
-   y-pass columns, fused z-pass columns -- with HIP events on the context's own fields
-   (the stress field is saved and restored), keeps the fastest and caches the choice per
-   grid shape and precision (flags & 1: measure again).  best_ms (optional) receives the
-   chosen variants' mean times (-1 when taken from the cache).  get/set read or pin the
-   choices (0 = built-in default). */
+   compiled variant of the four solve kernel slots -- x-pass rows (forward, inverse),
+   y-pass columns, fused z-pass columns, each with the length's radix plans -- with HIP
+   events on the context's own fields (the stress field is saved and restored), keeps
+   the fastest and caches the choice per grid shape and precision (flags & 1: measure
+   again).  best_ms (optional) receives the chosen variants' mean times (-1 when taken
+   from the cache).  get/set read or pin the choices (0 = built-in default).  A choice is
+   tile | plan << 8: the tile (rows or columns) in the low byte, the radix plan (0 = the
+   length's default factorisation, 1..3 alternatives such as 128 = 16*8 or
+   256 = 4*8*8) above it. */
+#define STZ_TUNE_MAXCAND 16
 int stz_autotune(stz_ctx *ctx, int reps, int flags, double best_ms[4]);
 int stz_get_tuning(stz_ctx *ctx, int out[4]);
 int stz_set_tuning(stz_ctx *ctx, const int in[4]);
 /* compiled variants of one slot (0 x forward rows, 1 y columns, 2 fused columns, 3 x
    inverse rows) for this context's shape; returns their count */
-int stz_tune_candidates(stz_ctx *ctx, int slot, int out[8]);
+int stz_tune_candidates(stz_ctx *ctx, int slot, int out[STZ_TUNE_MAXCAND]);
 
 /* Host evaluation of the Philox4x32-10 rounds used by stz_init_state_device (known-answer
    checks; no GPU needed). */

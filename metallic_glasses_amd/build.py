@@ -21,7 +21,7 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-I", INCLUDE, "-I", CSRC, "-Wall", "-Wno-unused-function",
 ]
-LIB_SOURCES = ["stz_api.hip", "stz_solve.hip", "stz_rates.hip", "stz_resident.hip", "stz_init.hip", "stz_local.hip", "stz_checkpoint.hip", "stz_inhom.hip", "stz_heat.hip", "stz_noalloc.hip", "stz_rng.cpp"]
+LIB_SOURCES = ["stz_solve_f64.hip", "stz_solve_f32.hip", "stz_api.hip", "stz_solve.hip", "stz_rates.hip", "stz_resident.hip", "stz_init.hip", "stz_local.hip", "stz_checkpoint.hip", "stz_inhom.hip", "stz_heat.hip", "stz_noalloc.hip", "stz_rng.cpp"]
 
 
 def _hipcc():

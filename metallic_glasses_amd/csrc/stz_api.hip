@@ -1341,7 +1341,7 @@ int stz_autotune(stz_ctx *ctx, int reps, int flags, double best_ms[4])
             return STZ_OK;
         }
     }
-    int cand[4][8], nc[4], maxc = 0;
+    int cand[4][STZ_TUNE_MAXCAND], nc[4], maxc = 0;
     for (int s = 0; s < 4; ++s) {
         nc[s] = tune_candidates(ctx, s, cand[s]);
         maxc = std::max(maxc, nc[s]);
@@ -1354,7 +1354,7 @@ int stz_autotune(stz_ctx *ctx, int reps, int flags, double best_ms[4])
     STZ_HIP(hipMemcpyAsync(save, ctx->sigma, sbytes, hipMemcpyDeviceToDevice, ctx->stream));
     const int saved_timing = ctx->timing;
     const int saved_tune[4] = {ctx->tune[0], ctx->tune[1], ctx->tune[2], ctx->tune[3]};
-    double t[4][8];
+    double t[4][STZ_TUNE_MAXCAND];
     int rc = STZ_OK;
     for (int i = 0; i < maxc && rc == STZ_OK; ++i) {
         for (int s = 0; s < 4; ++s) ctx->tune[s] = nc[s] ? cand[s][std::min(i, nc[s] - 1)] : 0;
@@ -1399,7 +1399,7 @@ int stz_autotune(stz_ctx *ctx, int reps, int flags, double best_ms[4])
     return STZ_OK;
 }
 
-int stz_tune_candidates(stz_ctx *ctx, int slot, int out[8])
+int stz_tune_candidates(stz_ctx *ctx, int slot, int out[STZ_TUNE_MAXCAND])
 {
     if (!ctx || !out || slot < 0 || slot > 3) return STZ_E_ARG;
     return tune_candidates(ctx, slot, out);

@@ -461,24 +461,110 @@ struct Dft<R, 3> {
     }
 };
 
-// constexpr radix plan: prefer 8, then 4, 2, 3
-__host__ __device__ constexpr int radix_count(int L)
+// Radix-16 DFT as 4 x 4 (n = 4 n1 + n2, k = k1 + 4 k2): DFT-4 over n1 for each n2,
+// multiply by w16^(n2 k1), DFT-4 over n2 for each k1; w16 = e^{s*2*pi*i/16}.
+template <typename R>
+struct Dft<R, 16> {
+    // x * w16^m for m = n2*k1 in {1, 2, 3, 4, 6, 9}
+    template <int M>
+    static __device__ __forceinline__ cpx<R> tw16(cpx<R> x, R s)
+    {
+        const R c1 = (R)0.92387953251128675612818318939678828682;   // cos(pi/8)
+        const R s1 = (R)0.38268343236508977172845998403039886676;   // sin(pi/8)
+        const R h = (R)0.70710678118654752440084436210484903928;
+        if constexpr (M == 4) {
+            return mul_si(x, s);
+        } else if constexpr (M == 2 || M == 6) {
+            // (+-h, s h) for angle pi/4 and 3 pi/4
+            const R c = M == 2 ? h : -h;
+            return {c * x.x - s * h * x.y, c * x.y + s * h * x.x};
+        } else {
+            // cos, sin of 2 pi m/16 for m = 1, 3, 9
+            const R c = M == 1 ? c1 : (M == 3 ? s1 : -c1);
+            const R sn = M == 1 ? s1 : (M == 3 ? c1 : -s1);
+            return {c * x.x - s * sn * x.y, c * x.y + s * sn * x.x};
+        }
+    }
+    static __device__ __forceinline__ void run(cpx<R> *v, R s)
+    {
+        cpx<R> a[4][4];
+#pragma unroll
+        for (int n2 = 0; n2 < 4; ++n2) {
+            cpx<R> t[4] = {v[n2], v[4 + n2], v[8 + n2], v[12 + n2]};
+            Dft<R, 4>::run(t, s);
+#pragma unroll
+            for (int k1 = 0; k1 < 4; ++k1) a[n2][k1] = t[k1];
+        }
+        a[1][1] = tw16<1>(a[1][1], s);
+        a[1][2] = tw16<2>(a[1][2], s);
+        a[1][3] = tw16<3>(a[1][3], s);
+        a[2][1] = tw16<2>(a[2][1], s);
+        a[2][2] = tw16<4>(a[2][2], s);
+        a[2][3] = tw16<6>(a[2][3], s);
+        a[3][1] = tw16<3>(a[3][1], s);
+        a[3][2] = tw16<6>(a[3][2], s);
+        a[3][3] = tw16<9>(a[3][3], s);
+#pragma unroll
+        for (int k1 = 0; k1 < 4; ++k1) {
+            cpx<R> t[4] = {a[0][k1], a[1][k1], a[2][k1], a[3][k1]};
+            Dft<R, 4>::run(t, s);
+#pragma unroll
+            for (int k2 = 0; k2 < 4; ++k2) v[k1 + 4 * k2] = t[k2];
+        }
+    }
+};
+
+// Radix plans.  A plan code lists the radices of a transform's passes, one per byte,
+// first pass in the lowest byte (at most four passes).  Plan 0 of a length is the default
+// (radix 8 first, then 4, 2, 3); plans 1..STZ_NPLAN-1 are alternative factorisations the
+// autotuner may time (code 0: none for that length): fewer passes (radix 16), or the
+// radices in another order.
+#define STZ_NPLAN 4
+__host__ __device__ constexpr unsigned plan_pack(int a, int b = 0, int c = 0, int d = 0)
 {
-    int c = 0;
-    while (L % 8 == 0 && L > 4) { L /= 8; ++c; }
-    while (L % 4 == 0) { L /= 4; ++c; }
-    while (L % 2 == 0) { L /= 2; ++c; }
-    while (L % 3 == 0) { L /= 3; ++c; }
-    return L == 1 ? c : -1;
+    return (unsigned)a | ((unsigned)b << 8) | ((unsigned)c << 16) | ((unsigned)d << 24);
 }
-__host__ __device__ constexpr int radix_at(int L, int i)
+__host__ __device__ constexpr unsigned default_plan(int L)
+{
+    unsigned code = 0;
+    int sh = 0;
+    while (L % 8 == 0 && L > 4) { code |= 8u << sh; sh += 8; L /= 8; }
+    while (L % 4 == 0) { code |= 4u << sh; sh += 8; L /= 4; }
+    while (L % 2 == 0) { code |= 2u << sh; sh += 8; L /= 2; }
+    while (L % 3 == 0) { code |= 3u << sh; sh += 8; L /= 3; }
+    return (L == 1 && sh <= 32) ? code : 0u;
+}
+__host__ __device__ constexpr unsigned plan_code(int L, int j)
+{
+    if (j == 0) return default_plan(L);
+    switch (L) {
+    case 16: return j == 1 ? plan_pack(16) : (j == 2 ? plan_pack(4, 4) : 0u);
+    case 32: return j == 1 ? plan_pack(16, 2) : (j == 2 ? plan_pack(4, 8) : 0u);
+    case 64: return j == 1 ? plan_pack(16, 4) : (j == 2 ? plan_pack(4, 16) : (j == 3 ? plan_pack(8, 4, 2) : 0u));
+    case 96: return j == 1 ? plan_pack(4, 8, 3) : (j == 2 ? plan_pack(16, 2, 3) : 0u);
+    case 128: return j == 1 ? plan_pack(16, 8) : (j == 2 ? plan_pack(8, 16) : (j == 3 ? plan_pack(8, 4, 4) : 0u));
+    case 256: return j == 1 ? plan_pack(16, 16) : (j == 2 ? plan_pack(4, 8, 8) : (j == 3 ? plan_pack(8, 16, 2) : 0u));
+    case 512: return j == 1 ? plan_pack(16, 8, 4) : (j == 2 ? plan_pack(8, 16, 4) : (j == 3 ? plan_pack(16, 16, 2) : 0u));
+    default: return 0u;
+    }
+}
+__host__ __device__ constexpr int plan_passes(unsigned code)
 {
     int c = 0;
-    while (L % 8 == 0 && L > 4) { if (c == i) return 8; L /= 8; ++c; }
-    while (L % 4 == 0) { if (c == i) return 4; L /= 4; ++c; }
-    while (L % 2 == 0) { if (c == i) return 2; L /= 2; ++c; }
-    while (L % 3 == 0) { if (c == i) return 3; L /= 3; ++c; }
-    return 0;
+    while (c < 4 && ((code >> (8 * c)) & 255u)) ++c;
+    return c;
+}
+__host__ __device__ constexpr int plan_radix(unsigned code, int i) { return (int)((code >> (8 * i)) & 255u); }
+__host__ __device__ constexpr long long plan_product(unsigned code)
+{
+    long long p = 1;
+    for (int i = 0; i < plan_passes(code); ++i) p *= plan_radix(code, i);
+    return p;
+}
+// plan j is compiled for length L (plan 0 always, when L is supported)
+__host__ __device__ constexpr bool plan_ok(int L, int j)
+{
+    return j >= 0 && j < STZ_NPLAN && plan_code(L, j) != 0u && plan_product(plan_code(L, j)) == L;
 }
 
 // One Stockham pass of radix Q (stride parameter NS = product of the earlier radices)
@@ -526,22 +612,23 @@ __device__ __forceinline__ void fft_pass(cpx<R> *buf, const cpx<R> *tw, R s)
     __syncthreads();
 }
 
-template <typename R, int L, int NL, int LS, int NT, int I, int NS>
+template <typename R, int L, int NL, int LS, int NT, unsigned PC, int I, int NS>
 __device__ __forceinline__ void fft_plan(cpx<R> *buf, const cpx<R> *tw, R s)
 {
-    if constexpr (I < radix_count(L)) {
-        constexpr int Q = radix_at(L, I);
+    if constexpr (I < plan_passes(PC)) {
+        constexpr int Q = plan_radix(PC, I);
         fft_pass<R, L, NL, LS, NT, Q, NS>(buf, tw, s);
-        fft_plan<R, L, NL, LS, NT, I + 1, NS * Q>(buf, tw, s);
+        fft_plan<R, L, NL, LS, NT, PC, I + 1, NS * Q>(buf, tw, s);
     }
 }
 
-// Full FFT (natural order in and out) of NL lines of length L, sign s (-1 fwd, +1 bwd).
-template <typename R, int L, int NL, int LS, int NT>
+// Full FFT (natural order in and out) of NL lines of length L, sign s (-1 fwd, +1 bwd),
+// by radix plan PL of the length.
+template <typename R, int L, int NL, int LS, int NT, int PL = 0>
 __device__ __forceinline__ void fft_lines(cpx<R> *buf, const cpx<R> *tw, R s)
 {
-    static_assert(radix_count(L) > 0, "unsupported FFT length");
-    fft_plan<R, L, NL, LS, NT, 0, 1>(buf, tw, s);
+    static_assert(plan_ok(L, PL), "unsupported FFT length or radix plan");
+    fft_plan<R, L, NL, LS, NT, plan_code(L, PL), 0, 1>(buf, tw, s);
 }
 
 // fft_pass whose inputs come from ld(line, point) when GIN (the first pass of a transform
@@ -597,25 +684,35 @@ __device__ __forceinline__ void fft_pass_io(cpx<R> *buf, const cpx<R> *tw, R s, 
     if constexpr (!GOUT) __syncthreads();
 }
 
-template <typename R, int L, int NL, int LS, int NT, int I, int NS, bool GIN, bool GOUT, typename Ld, typename St>
+// passes I..IEND-1 of plan PC (the stride parameter NS of pass I given); GIN applies to
+// pass 0 and GOUT to the plan's last pass
+template <typename R, int L, int NL, int LS, int NT, unsigned PC, int I, int IEND, int NS, bool GIN, bool GOUT,
+          typename Ld, typename St>
+__device__ __forceinline__ void fft_plan_range(cpx<R> *buf, const cpx<R> *tw, R s, Ld &ld, St &st)
+{
+    if constexpr (I < IEND) {
+        constexpr int Q = plan_radix(PC, I);
+        constexpr bool LASTP = I + 1 == plan_passes(PC);
+        fft_pass_io<R, L, NL, LS, NT, Q, NS, GIN && I == 0, GOUT && LASTP>(buf, tw, s, ld, st);
+        fft_plan_range<R, L, NL, LS, NT, PC, I + 1, IEND, NS * Q, GIN, GOUT>(buf, tw, s, ld, st);
+    }
+}
+
+template <typename R, int L, int NL, int LS, int NT, unsigned PC, int I, int NS, bool GIN, bool GOUT, typename Ld,
+          typename St>
 __device__ __forceinline__ void fft_plan_io(cpx<R> *buf, const cpx<R> *tw, R s, Ld &ld, St &st)
 {
-    if constexpr (I < radix_count(L)) {
-        constexpr int Q = radix_at(L, I);
-        constexpr bool LASTP = I + 1 == radix_count(L);
-        fft_pass_io<R, L, NL, LS, NT, Q, NS, GIN && I == 0, GOUT && LASTP>(buf, tw, s, ld, st);
-        fft_plan_io<R, L, NL, LS, NT, I + 1, NS * Q, GIN, GOUT>(buf, tw, s, ld, st);
-    }
+    fft_plan_range<R, L, NL, LS, NT, PC, I, plan_passes(PC), NS, GIN, GOUT>(buf, tw, s, ld, st);
 }
 
 // fft_lines with the input read through ld (GIN) and/or the output written through st
 // (GOUT) instead of the LDS tile; the twiddle table must be in LDS before the second pass
 // (the first pass multiplies by no twiddle)
-template <typename R, int L, int NL, int LS, int NT, bool GIN, bool GOUT, typename Ld, typename St>
+template <typename R, int L, int NL, int LS, int NT, bool GIN, bool GOUT, int PL = 0, typename Ld, typename St>
 __device__ __forceinline__ void fft_lines_io(cpx<R> *buf, const cpx<R> *tw, R s, Ld ld, St st)
 {
-    static_assert(radix_count(L) > 0, "unsupported FFT length");
-    fft_plan_io<R, L, NL, LS, NT, 0, 1, GIN, GOUT>(buf, tw, s, ld, st);
+    static_assert(plan_ok(L, PL), "unsupported FFT length or radix plan");
+    fft_plan_io<R, L, NL, LS, NT, plan_code(L, PL), 0, 1, GIN, GOUT>(buf, tw, s, ld, st);
 }
 
 // <sigma> from the stress-solve block partials spart[b][c] (fixed order: per-thread
@@ -670,7 +767,7 @@ int stz_athermal_fused_launch(stz_ctx *ctx, int gated);   // + the |DWork| sum
 int stz_athermal_apply_range(stz_ctx *ctx, long long first, long long count);
 int stz_energy_launch(stz_ctx *ctx);
 bool stz_solve_supported(int dim, const int *n, int prec);
-int tune_candidates(const stz_ctx *ctx, int slot, int out[8]);
+int tune_candidates(const stz_ctx *ctx, int slot, int out[STZ_TUNE_MAXCAND]);
 int stz_init_device_launch(stz_ctx *ctx, unsigned long long seed, const double *spheres_host, int nsph, double inc_mean,
                            double inc_std, long long *inside);
 // batched solve without the mean kernel (the caller reduces the *nparts partials)

@@ -342,7 +342,7 @@ __device__ __forceinline__ bool gated_off_p(const int *gate)
 // k_fwd_x: eps0 -> round trip -> R2C along x for NR rows and all C components
 // (MODE 1: the input is the reference-medium eigenstrain itself, no round trip)
 // ---------------------------------------------------------------------------------
-template <typename R, int L0, int NR, int C, int MODE = 0>
+template <typename R, int L0, int NR, int C, int MODE = 0, int PL = 0>
 __global__ __launch_bounds__(NT_SOLVE) void k_fwd_x(const double *__restrict__ eps0_a, cpx<R> *__restrict__ spec_a,
                                                     const cpx<R> *__restrict__ twg, SolveConsts k, const SolveRep *reps)
 {
@@ -381,7 +381,7 @@ __global__ __launch_bounds__(NT_SOLVE) void k_fwd_x(const double *__restrict__ e
             for (int c = 0; c < C; ++c) bufr[2 * (i0 * LS + c * NP + (rl >> 1)) + (rl & 1)] = (R)rt[c];
         });
     __syncthreads();
-    fft_lines<R, L0, NL, LS, NT_SOLVE>(buf, tw, (R)-1);
+    fft_lines<R, L0, NL, LS, NT_SOLVE, PL>(buf, tw, (R)-1);
     constexpr int H0 = L0 / 2 + 1;
     for (int e = tid; e < C * NR * H0; e += NT_SOLVE) {
         const int kk = e % H0;
@@ -402,7 +402,7 @@ __global__ __launch_bounds__(NT_SOLVE) void k_fwd_x(const double *__restrict__ e
 // (MODE 1: the input is the reference-medium eigenstrain; the epilogue also takes the
 // fixed-point update of the inhomogeneous solve into k.eout and sums its squared change)
 // ---------------------------------------------------------------------------------
-template <typename R, int L0, int NR, int C, int MODE = 0>
+template <typename R, int L0, int NR, int C, int MODE = 0, int PL = 0>
 __global__ __launch_bounds__(NT_SOLVE) void k_inv_x(cpx<R> *__restrict__ spec_a, const double *__restrict__ eps0_a,
                                                     R *__restrict__ sigma_a, double *__restrict__ spart_a,
                                                     int *__restrict__ nanflag_a, const cpx<R> *__restrict__ twg,
@@ -453,7 +453,7 @@ __global__ __launch_bounds__(NT_SOLVE) void k_inv_x(cpx<R> *__restrict__ spec_a,
             buf[kk * LS + c * NP + p] = Z;
         });
     __syncthreads();
-    fft_lines<R, L0, NL, LS, NT_SOLVE>(buf, tw, (R)1);
+    fft_lines<R, L0, NL, LS, NT_SOLVE, PL>(buf, tw, (R)1);
     const R *bufr = reinterpret_cast<const R *>(buf);
     double acc[C];
 #pragma unroll
@@ -566,7 +566,7 @@ __device__ __forceinline__ uint3 xcd_block()
 // ---------------------------------------------------------------------------------
 // k_ypass: complex FFT along y (stride hp) for a tile of W k0-columns, in place
 // ---------------------------------------------------------------------------------
-template <typename R, int L1, int W>
+template <typename R, int L1, int W, int PL = 0>
 __global__ __launch_bounds__(NT_SOLVE) void k_ypass(cpx<R> *__restrict__ spec_a, const cpx<R> *__restrict__ twg,
                                                     SolveConsts k, R sgn, const SolveRep *reps)
 {
@@ -584,7 +584,7 @@ __global__ __launch_bounds__(NT_SOLVE) void k_ypass(cpx<R> *__restrict__ spec_a,
     cpx<R> *base = spec + ((long long)c * k.rows + (long long)i2 * L1) * k.hp + k0base;
     // line = column of the tile; the first pass reads the columns from the spectrum and
     // the last pass writes them back
-    fft_lines_io<R, L1, W, W, NT_SOLVE, true, true>(
+    fft_lines_io<R, L1, W, W, NT_SOLVE, true, true, PL>(
         buf, tw, sgn,
         [&](int col, int p) {
             return k0base + col < k.h0 ? gload(base + (long long)p * k.hp + col) : cpx<R>{0, 0};
@@ -594,6 +594,57 @@ __global__ __launch_bounds__(NT_SOLVE) void k_ypass(cpx<R> *__restrict__ spec_a,
         });
 }
 
+// Gamma-hat on every wavevector of a fused tile (LZ points x C*W lines in LDS, line =
+// c*W + col; 3-D: at y index i1): IsotropicMul_C, HadamardProdnSum_C x3,
+// HadamardProdnSSum_C x2, HadamardProd2nSSum_C (STZ/elastic.c:802-809), with the
+// average over the mirror bin's unit vector on Nyquist bins
+template <typename R, int LZ, int W, int C, int DIM, int NTF>
+__device__ __forceinline__ void gamma_tile(cpx<R> *buf, int k0base, int i1, const SolveConsts &k)
+{
+    constexpr int NL = C * W;
+    const R lam = (R)k.lambda, mu = (R)k.mu, s0 = (R)k.s0, sh = (R)k.sh, sa = (R)k.sa;
+    for (int kp = threadIdx.x; kp < LZ * W; kp += NTF) {
+        const int col = kp % W, p = kp / W;
+        int idx[3];
+        idx[0] = k0base + col;
+        if constexpr (DIM == 3) { idx[1] = i1; idx[2] = p; }
+        else { idx[1] = p; idx[2] = 0; }
+        double gd[3];
+        unit_k<DIM>(idx, k.rn, gd);
+        bool nyq = false;
+#pragma unroll
+        for (int a = 0; a < DIM; ++a) nyq |= (2 * idx[a] == k.n[a]);
+        R g[3] = {(R)gd[0], (R)gd[1], (R)gd[2]};
+        R gg[6];
+        products<R, DIM>(g, gg);
+        // the real parts, then the imaginary parts (Gamma-hat acts on each alone): half
+        // the live values of both at once, which set the kernel's register count
+        R *bufr = reinterpret_cast<R *>(buf);
+#pragma unroll 1
+        for (int part = 0; part < 2; ++part) {
+            R e[C], o[C];
+#pragma unroll
+            for (int c = 0; c < C; ++c) e[c] = bufr[2 * (p * NL + c * W + col) + part];
+            gamma_part<R, C>(e, g, gg, lam, mu, s0, sh, sa, o);
+            if (nyq) {
+                // average with the mirror bin's unit vector (Nyquist components negated)
+                R gm[3] = {g[0], g[1], g[2]};
+#pragma unroll
+                for (int a = 0; a < DIM; ++a)
+                    if (2 * idx[a] == k.n[a]) gm[a] = -gm[a];
+                R ggm[6];
+                products<R, DIM>(gm, ggm);
+                R o2[C];
+                gamma_part<R, C>(e, gm, ggm, lam, mu, s0, sh, sa, o2);
+#pragma unroll
+                for (int c = 0; c < C; ++c) o[c] = (o[c] + o2[c]) * (R)0.5;
+            }
+#pragma unroll
+            for (int c = 0; c < C; ++c) bufr[2 * (p * NL + c * W + col) + part] = o[c];
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------------
 // k_fused: last axis forward FFT, Gamma-hat, inverse FFT for one tile of W k0-columns
 // (3-D: at a fixed y index; 2-D: the y axis itself is the last axis).  The
@@ -601,7 +652,7 @@ __global__ __launch_bounds__(NT_SOLVE) void k_ypass(cpx<R> *__restrict__ spec_a,
 // the C*W lines are transformed together, Gamma-hat mixes the C components of each of
 // the LZ*W wavevectors in place, and the inverse transform follows.
 // ---------------------------------------------------------------------------------
-template <typename R, int LZ, int W, int C, int DIM, int NTF, int OP = 0>
+template <typename R, int LZ, int W, int C, int DIM, int NTF, int OP = 0, int PL = 0>
 __global__ __launch_bounds__(NTF) void k_fused(cpx<R> *__restrict__ spec_a, const cpx<R> *__restrict__ twg,
                                                SolveConsts k, const SolveRep *reps)
 {
@@ -623,12 +674,12 @@ __global__ __launch_bounds__(NTF) void k_fused(cpx<R> *__restrict__ spec_a, cons
         return spec + ((long long)(line / W) * k.rows + (long long)p * pstride + i1) * k.hp + k0base + line % W;
     };
     // forward: the first pass reads the tile from the spectrum, the result stays in LDS
-    fft_lines_io<R, LZ, NL, NL, NTF, true, false>(
+    fft_lines_io<R, LZ, NL, NL, NTF, true, false, PL>(
         buf, tw, (R)-1,
         [&](int line, int p) { return k0base + line % W < k.h0 ? gload(gaddr(line, p)) : cpx<R>{0, 0}; },
         [](int, int, const cpx<R> &) {});
     // Gamma-hat (OP 0), the displacement operator (OP 1) or the heat multiplier (OP 2) on
-// each wavevector of the tile
+    // each wavevector of the tile
     const R lam = (R)k.lambda, mu = (R)k.mu, s0 = (R)k.s0, sh = (R)k.sh, sa = (R)k.sa;
     if constexpr (OP == 1) {
         // u_hat_a = (tmp_a(im)*(1/mu)/|k|, -1.0*tmp_a(re)*(1/mu)/|k|) (HadamardSiDivd,
@@ -692,50 +743,11 @@ __global__ __launch_bounds__(NTF) void k_fused(cpx<R> *__restrict__ spec_a, cons
             }
         }
     } else {
-    for (int kp = tid; kp < LZ * W; kp += NTF) {
-        const int col = kp % W, p = kp / W;
-        int idx[3];
-        idx[0] = k0base + col;
-        if constexpr (DIM == 3) { idx[1] = i1; idx[2] = p; }
-        else { idx[1] = p; idx[2] = 0; }
-        double gd[3];
-        unit_k<DIM>(idx, k.rn, gd);
-        bool nyq = false;
-#pragma unroll
-        for (int a = 0; a < DIM; ++a) nyq |= (2 * idx[a] == k.n[a]);
-        R g[3] = {(R)gd[0], (R)gd[1], (R)gd[2]};
-        R gg[6];
-        products<R, DIM>(g, gg);
-        // the real parts, then the imaginary parts (Gamma-hat acts on each alone): half
-        // the live values of both at once, which set the kernel's register count
-        R *bufr = reinterpret_cast<R *>(buf);
-#pragma unroll 1
-        for (int part = 0; part < 2; ++part) {
-            R e[C], o[C];
-#pragma unroll
-            for (int c = 0; c < C; ++c) e[c] = bufr[2 * (p * NL + c * W + col) + part];
-            gamma_part<R, C>(e, g, gg, lam, mu, s0, sh, sa, o);
-            if (nyq) {
-                // average with the mirror bin's unit vector (Nyquist components negated)
-                R gm[3] = {g[0], g[1], g[2]};
-#pragma unroll
-                for (int a = 0; a < DIM; ++a)
-                    if (2 * idx[a] == k.n[a]) gm[a] = -gm[a];
-                R ggm[6];
-                products<R, DIM>(gm, ggm);
-                R o2[C];
-                gamma_part<R, C>(e, gm, ggm, lam, mu, s0, sh, sa, o2);
-#pragma unroll
-                for (int c = 0; c < C; ++c) o[c] = (o[c] + o2[c]) * (R)0.5;
-            }
-#pragma unroll
-            for (int c = 0; c < C; ++c) bufr[2 * (p * NL + c * W + col) + part] = o[c];
-        }
-    }
+        gamma_tile<R, LZ, W, C, DIM, NTF>(buf, k0base, i1, k);
     }
     __syncthreads();
     // inverse from the tile; the last pass writes the spectrum
-    fft_lines_io<R, LZ, NL, NL, NTF, false, true>(
+    fft_lines_io<R, LZ, NL, NL, NTF, false, true, PL>(
         buf, tw, (R)1, [](int, int) { return cpx<R>{0, 0}; },
         [&](int line, int p, const cpx<R> &v) {
             if (k0base + line % W < k.h0) gstore(gaddr(line, p), v);
@@ -772,6 +784,10 @@ constexpr bool f_variant(int L, int W)
 }
 
 
+// a tuning value is a tile (rows or columns) in its low byte and a radix plan above it
+constexpr int tune_tile(int v) { return v & 255; }
+constexpr int tune_plan(int v) { return (v >> 8) & 15; }
+
 // launch constants of a context (the batched path overrides the per-replica fields)
 inline SolveConsts solve_consts(const stz_ctx *ctx)
 {
@@ -801,3 +817,12 @@ inline SolveConsts solve_consts(const stz_ctx *ctx)
 }
 
 }  // namespace
+
+// the launch sequence of each storage precision (stz_solve_f64.hip, stz_solve_f32.hip);
+// the first argument only selects the precision
+int solve_entry(double, stz_ctx *ctx, int gated, int nrep, const SolveRep *reps, hipStream_t stream, int *nparts);
+int solve_entry(float, stz_ctx *ctx, int gated, int nrep, const SolveRep *reps, hipStream_t stream, int *nparts);
+int spectral_entry(double, stz_ctx *ctx);
+int spectral_entry(float, stz_ctx *ctx);
+int disp_entry(double, stz_ctx *ctx, double *u);
+int disp_entry(float, stz_ctx *ctx, double *u);

@@ -379,7 +379,7 @@ class Stz:
         return list(o)
 
     def tune_candidates(self, slot):
-        o = (C.c_int * 8)()
+        o = (C.c_int * 16)()
         n = self._check(lib().stz_tune_candidates(self._h, int(slot), o))
         return list(o)[:n]
 

@@ -73,9 +73,11 @@ def test_stress_solve_fp64(dims):
 
 
 @pytest.mark.parametrize("prec", [mg.FP64, mg.MIXED])
-@pytest.mark.parametrize("dims", [(64, 32), (512, 512), (32, 32, 32), (64, 64, 128), (96, 96, 96), (128, 128, 128)])
+@pytest.mark.parametrize("dims", [(64, 32), (512, 512), (256, 128), (32, 32, 32), (64, 64, 128), (96, 96, 96),
+                                  (128, 128, 128), (16, 256, 256)])
 def test_stress_solve_every_tile_variant(dims, prec):
-    """Every compiled tile variant the autotuner can pick gives the oracle's stress."""
+    """Every compiled tile variant and radix plan the autotuner can pick gives the oracle's
+    stress (tuning values: tile | plan << 8)."""
     print("tile variants", dims, prec, flush=True)
     o, g = _pair(**_kw(dims, modes=2, precision=prec))
     _random_state(o, 9)
