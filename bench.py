@@ -97,14 +97,18 @@ def pmc_target(a, mg):
             print(f"# rates_c5 target: {nst} warm-up and {len(res)} resident steps", file=sys.stderr, flush=True)
             g.close()
         elif t == "solve":
-            # stress solves of a device-generated state of --grid, FP64 then mixed
+            # stress solves of a device-generated state of --grid, FP64 then mixed, with the
+            # autotuned tiles (the autotuner's own trial launches are separate kernel
+            # instances unless a trial used the chosen variant)
             dims = tuple(int(x) for x in a.grid.split(","))
             for prec in (mg.FP64, mg.MIXED):
                 g = mg.Stz(init_state=False, device_init=dict(seed=a.seed),
                            **mg.config_3d(n=dims, seed=a.seed, q0_std=0.2, precision=prec))
+                tiles = g.autotune(reps=3)[0]
                 for _ in range(a.steps):
                     g.stress_solve()
-                print(f"# solve target: {a.steps} solves of {dims} precision {prec}", file=sys.stderr, flush=True)
+                print(f"# solve target: {a.steps} solves of {dims} precision {prec}, tiles {tiles}",
+                      file=sys.stderr, flush=True)
                 g.close()
         elif t == "rates":
             dims = tuple(int(x) for x in a.grid.split(","))
