@@ -11,7 +11,8 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "liboracle.so")
+# STZ_ORACLE_LIB: another build of the oracle (the mutation tests load planted-mistake copies)
+LIB_PATH = os.environ.get("STZ_ORACLE_LIB") or os.path.join(HERE, "liboracle.so")
 
 
 class OraParams(C.Structure):
