@@ -738,6 +738,15 @@ def large_leg(a, mg):
                                    "achieved_GBs": round(k4_bytes / (k4_ms * 1e-3) / 1e9, 1),
                                    "frac": round(k4_bytes / (k4_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                                    "window_avg_ms": round(kw_ms, 5), "window_launches": int(nw[0])}}
+        if name == "mixed":
+            # the same run continued as one persistent launch per step (kp_step), for the
+            # record: same results bit for bit, per-workgroup L2 write-backs make it slower
+            g.persistent(True)
+            dtp, kp, _, _, _ = run_steps(g, a.steps)
+            g.persistent(False)
+            legs[name]["persistent_step"] = {
+                "value": round(float(kp[1] / dtp), 3), "ms_per_step": round(dtp * 1e3 / a.steps, 4),
+                "over_kernel_sequence_ms": round(dtp / a.steps / (dt / a.steps), 3)}
         if name != "fp64_fft":
             # host-driven steps (full FFT solve on every solved step), events around each
             # solve kernel
@@ -887,6 +896,20 @@ def ensemble_leg(a, mg):
     ens_p.close()
     for m in mem_p:
         m.close()
+    # the mixed ensemble continued (after the precision check read its state) as one
+    # persistent launch per step (all 64 replicas, all shape classes): bit-identical steps,
+    # measured for the record
+    ens_m.persistent(True)
+    t0 = time.perf_counter()
+    _, outs_p = ens_m.run(a.ensemble_steps)
+    dtp = time.perf_counter() - t0
+    ens_m.persistent(False)
+    evp = sum(count_events(o) for o in outs_p)
+    legs["mixed_local_persistent"] = {
+        "value": round(evp / dtp, 2), "unit": "replica-events/s", "events": int(evp), "seconds": round(dtp, 4),
+        "ms_per_step": round(dtp * 1e3 / a.ensemble_steps, 3),
+        "kernel_sequence_ms_per_step": round(legs["mixed_local"]["seconds"] * 1e3 / a.ensemble_steps, 3)}
+    print(f"# ensemble mixed_local_persistent: {legs['mixed_local_persistent']}", file=sys.stderr, flush=True)
     ens_m.close()
     for m in mem_m:
         m.close()

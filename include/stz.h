@@ -34,7 +34,8 @@ extern "C" {
 #define STZ_E_ARG (-3)         /* bad argument / unsupported configuration */
 #define STZ_E_NAN (-4)         /* NaN in the stress solve (STZ/elastic.c:212-215) */
 #define STZ_E_UNSUPPORTED (-5) /* crystal-like MRO voxel drawn, or an unsupported combination */
-#define STZ_E_TOO_LARGE (-6)   /* stz_rates_noalloc: modes * ceil(voxels / 256) above 2^21 */
+#define STZ_E_TOO_LARGE (-6)   /* stz_rates_noalloc: modes * ceil(voxels / 256) above 2^21;
+                                  stz_persist_probe: a table beyond its 31-bit counters */
 
 #define STZ_FP64 0
 #define STZ_MIXED 1
@@ -308,6 +309,27 @@ int stz_ensemble_create(stz_ctx *const *members, int n, stz_ensemble **out);
 int stz_ensemble_config(stz_ensemble *e, int batch, int use_graph);
 int stz_ensemble_run(stz_ensemble *e, int nsteps, stz_step_result *const *results, int *done, int *rcs);
 void stz_ensemble_destroy(stz_ensemble *e);
+
+/* Persistent step (on = 1): each step of every replica of every shape class is ONE kernel
+   launch instead of a sequence of launches per class.  Its workgroups take tickets of the
+   step's work in dependency order (rates, group sums, chunk totals, decision, athermal
+   pass, event-local update, bookkeeping) and wait for their replica's earlier phases on
+   device counters; replayed from a hipGraph with use_graph as the kernel sequence is.
+   Results equal those of the kernel sequence bit for bit.  Takes replicas of one precision
+   and dimension on the event-local update (stz_set_solver STZ_SOLVER_LOCAL), at most 8
+   shape classes: STZ_E_UNSUPPORTED otherwise (a step that needs the full solve still halts
+   to the host path, as without it).  on = 0 returns to the kernel sequence.  No
+   counterpart in the reference (its step is a sequence of MPI-parallel function calls,
+   STZ/bmg.c:138-411). */
+int stz_ensemble_persistent(stz_ensemble *e, int on);
+int stz_run_persistent(stz_ctx *ctx, int on);
+/* CPU check of the persistent step's ticket table (no device call; for tests): builds the
+   table of ncls shape classes in launch order, shape[7*c ..] = {dim, n0, n1, n2, modes,
+   precision, replicas}, decodes every ticket as the kernel does and counts violations (an
+   index out of its range, an item covered other than exactly once, a ticket preceding a
+   ticket of an earlier phase of its replica).  out = {tickets, items, violations};
+   STZ_E_TOO_LARGE when the table does not fit the 31-bit counters. */
+int stz_persist_probe(int ncls, const int *shape, long long out[3]);
 
 /* Autotuner of the stress-solve schedule (no counterpart in the reference): times every
    compiled variant of the four solve kernel slots -- x-pass rows (forward, inverse),

@@ -742,6 +742,46 @@ __device__ void mean_partials(const double *spart, int nblocks, double invN, dou
     __syncthreads();
 }
 
+// mean_partials on a workgroup of NTH threads, with the very operations of the
+// STZ_NT_MEAN-thread tree: thread t keeps the strided runs of the virtual threads
+// t + j*NTH, the tree levels whose two operands belong to one thread add in registers
+// (v and v + w with w >= NTH), the remaining levels in LDS.  Same value bit for bit.
+template <int C, int NTH>
+__device__ void mean_partials_n(const double *spart, int nblocks, double invN, double *out)
+{
+    static_assert(NTH <= STZ_NT_MEAN && STZ_NT_MEAN % NTH == 0, "NTH divides the tree width");
+    constexpr int J = STZ_NT_MEAN / NTH;
+    __shared__ double red[C][NTH];
+    const int tid = threadIdx.x;
+    double a[J][C];
+#pragma unroll
+    for (int j = 0; j < J; ++j)
+#pragma unroll
+        for (int c = 0; c < C; ++c) a[j][c] = 0.0;
+#pragma unroll
+    for (int j = 0; j < J; ++j)
+        for (int b = tid + j * NTH; b < nblocks; b += STZ_NT_MEAN)
+#pragma unroll
+            for (int c = 0; c < C; ++c) a[j][c] += spart[(long long)b * C + c];
+#pragma unroll
+    for (int w = STZ_NT_MEAN / 2; w >= NTH; w >>= 1)
+#pragma unroll
+        for (int j = 0; j < w / NTH; ++j)
+#pragma unroll
+            for (int c = 0; c < C; ++c) a[j][c] = a[j][c] + a[j + w / NTH][c];
+#pragma unroll
+    for (int c = 0; c < C; ++c) red[c][tid] = a[0][c];
+    __syncthreads();
+    for (int w = NTH / 2; w > 0; w >>= 1) {
+        if (tid < w)
+#pragma unroll
+            for (int c = 0; c < C; ++c) red[c][tid] = red[c][tid] + red[c][tid + w];
+        __syncthreads();
+    }
+    if (tid < C) out[tid] = red[tid][0] * invN;
+    __syncthreads();
+}
+
 // ---------------------------------------------------------------------------
 // host-side launchers implemented in the .hip files
 // ---------------------------------------------------------------------------

@@ -855,6 +855,38 @@ __device__ double ordered_sum_flagged(const double *p, const int *cnt, const int
     return r;
 }
 
+// ordered_sum_flagged<NTV> on a workgroup of NTH threads with the NTV-thread tree's own
+// operations: thread t keeps the strided runs of the virtual threads t + j*NTH, the levels
+// whose operands belong to one thread add in registers, the others in LDS (bit for bit)
+template <int NTV, int NTH>
+__device__ double ordered_sum_flagged_n(const double *p, const int *cnt, const int *ccnt, long long n)
+{
+    static_assert(NTH <= NTV && NTV % NTH == 0, "NTH divides the tree width");
+    constexpr int J = NTV / NTH;
+    __shared__ double red[NTH];
+    const int tid = threadIdx.x;
+    double a[J];
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+        a[j] = 0.0;
+        for (long long i = tid + j * NTH; i < n; i += NTV)
+            if (ccnt[i / NT] != 0 && cnt[i] != 0) a[j] = a[j] + p[i];
+    }
+#pragma unroll
+    for (int w = NTV / 2; w >= NTH; w >>= 1)
+#pragma unroll
+        for (int j = 0; j < w / NTH; ++j) a[j] = a[j] + a[j + w / NTH];
+    red[tid] = a[0];
+    __syncthreads();
+    for (int s2 = NTH / 2; s2 > 0; s2 >>= 1) {
+        if (tid < s2) red[tid] = red[tid] + red[tid + s2];
+        __syncthreads();
+    }
+    const double r = red[0];
+    __syncthreads();
+    return r;
+}
+
 // group sums gsum[g] for the STZ_GRP_PER_BLOCK groups g0.. of one workgroup
 #define STZ_GRP_PER_BLOCK 4
 __device__ __forceinline__ void group_sums(const double *bsum, long long nb, double *gsum, long long ngrp, long long g0,

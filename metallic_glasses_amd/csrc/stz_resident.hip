@@ -3,20 +3,23 @@
 // or KMC with its loading and clock update, the stress solves) with no host round trip,
 // for a batch of replicas of one grid shape at a time.
 //
-// Per step and shape class the stream holds ten launches, every one batched over the
+// Per step and shape class the stream holds its launches, every one batched over the
 // replicas of the class (grid y/z = replica) and gated on the device:
 //   kb_rates      rate pass (shared code with k_rates)
 //   kb_group      group sums of the block rate sums
+//   kb_chunks     chunk totals of the group sums, athermal counts of the block chunks
 //   kb_decide     one workgroup per replica: R, the Q<0 count, the step kind, the
 //                 residence-time selection and the transform (thermal), or the
 //                 avalanche list offsets (athermal)
 //   kb_athermal   athermal compaction + transforms
-//   5 solve kernels (stz_solve.hip, batched)
+//   5 solve kernels (stz_solve.hip, batched), or kb_local (the event-local update)
 //   kb_finish     <sigma>, the |DWork| sum, clock, loading (STZ/evolution.c:262-358),
 //                 athermal correction (STZ/evolution.c:91-97), print-step and
 //                 termination tests (STZ/bmg.c:181-201, 382-410), the step record
-// No kernel waits on another workgroup: every cross-workgroup step is a kernel boundary.
-// The random numbers are the reference's own MT19937 stream, taken by index from a
+// In this kernel sequence no kernel waits on another workgroup: every cross-workgroup step
+// is a kernel boundary.  The persistent step (kp_step, opt-in) runs the same items of
+// every class in one launch per step instead, ordered by device counters (bit-identical
+// results; slower on this chip, see the ticket sizes below).  The random numbers are the reference's own MT19937 stream, taken by index from a
 // lookup window built on the host (StzMtWindow), so a resident run draws exactly what
 // the per-call path and the reference draw.  A replica halts (every later kernel of
 // it exits at once) on termination, on a print step (the host re-solves and writes
@@ -103,34 +106,42 @@ __device__ __forceinline__ ApplyArgs apply_args_of(const RepDev &d, const RepSta
     return a;
 }
 
+// Each kernel of the step is a thin wrapper around an item function (one workgroup's
+// share of one replica); the persistent step (kp_step) calls the same item functions.
 template <typename R, int DIM>
-__global__ __launch_bounds__(NT) void kb_rates(const RepDev *__restrict__ reps, long long N, long long nblk, int M)
+__device__ __forceinline__ void rates_item(const RepDev &d, long long N, long long nblk, int M, long long blk)
 {
-    const RepDev &d = reps[blockIdx.y];
     const RepState *st = d.st;
     if (st->halt) return;
     const RateArgs<R> a = rate_args_of<R>(d, st, N, nblk, M);
-    rates_block<R, DIM>(a, blockIdx.x);
+    rates_block<R, DIM>(a, blk);
+}
+
+template <typename R, int DIM>
+__global__ __launch_bounds__(NT) void kb_rates(const RepDev *__restrict__ reps, long long N, long long nblk, int M)
+{
+    rates_item<R, DIM>(reps[blockIdx.y], N, nblk, M, blockIdx.x);
+}
+
+__device__ __forceinline__ void group_item(const RepDev &d, long long nb, long long ngrp, long long gi, double *ws)
+{
+    if (d.st->halt) return;
+    group_sums(d.bsum, nb, d.gsum, ngrp, gi * STZ_GRP_PER_BLOCK, ws);
 }
 
 __global__ __launch_bounds__(NT) void kb_group(const RepDev *__restrict__ reps, long long nb, long long ngrp)
 {
     __shared__ double ws[NW];
-    const RepDev &d = reps[blockIdx.y];
-    if (d.st->halt) return;
-    group_sums(d.bsum, nb, d.gsum, ngrp, (long long)blockIdx.x * STZ_GRP_PER_BLOCK, ws);
+    group_item(reps[blockIdx.y], nb, ngrp, blockIdx.x, ws);
 }
 
 // Chunk totals per replica, one workgroup per chunk: the running scan's total of NT
 // group sums (chunk c < nchr), and the athermal count of NT voxel blocks (c < ncha).
 // Both levels let the single-workgroup decision read a few thousand values at most.
-__global__ __launch_bounds__(NT) void kb_chunks(const RepDev *__restrict__ reps, long long ngrp, long long nblk)
+__device__ __forceinline__ void chunks_item(const RepDev &d, long long ngrp, long long nblk, long long c, double *sh,
+                                            int *wsi)
 {
-    __shared__ double sh[NW];
-    __shared__ int wsi[NW];
-    const RepDev &d = reps[blockIdx.y];
     if (d.st->halt) return;
-    const long long c = blockIdx.x;
     if (c < (ngrp + NT - 1) / NT) {
         const double t = chunk_total(d.gsum, ngrp, c, sh);
         if (threadIdx.x == NT - 1) d.csum[c] = t;
@@ -148,13 +159,19 @@ __global__ __launch_bounds__(NT) void kb_chunks(const RepDev *__restrict__ reps,
     }
 }
 
+__global__ __launch_bounds__(NT) void kb_chunks(const RepDev *__restrict__ reps, long long ngrp, long long nblk)
+{
+    __shared__ double sh[NW];
+    __shared__ int wsi[NW];
+    chunks_item(reps[blockIdx.y], ngrp, nblk, blockIdx.x, sh, wsi);
+}
+
 // One workgroup per replica: R, the Q<0 count, the step kind, and the stream positions
 // of the step's random numbers; then the residence-time selection and transform of a
 // thermal event, or the list offsets of an athermal avalanche.  A step whose draws are
 // not all inside the lookup window is marked KIND_STALL (nothing changes).
 template <typename R, int DIM>
-__global__ __launch_bounds__(NT) void kb_decide(const RepDev *__restrict__ reps, long long N, long long nblk,
-                                                long long ngrp, int M)
+__device__ __forceinline__ void decide_item(const RepDev &d, long long N, long long nblk, long long ngrp, int M)
 {
     constexpr int S = Comp<DIM>::S;
     __shared__ int s_kind;
@@ -162,7 +179,6 @@ __global__ __launch_bounds__(NT) void kb_decide(const RepDev *__restrict__ reps,
     __shared__ long long s_k0;
     __shared__ int s_par;
     __shared__ double s_R;
-    const RepDev &d = reps[blockIdx.x];
     RepState *st = d.st;
     if (st->halt) return;
     double *res = d.res;
@@ -261,16 +277,22 @@ __global__ __launch_bounds__(NT) void kb_decide(const RepDev *__restrict__ reps,
     }
 }
 
+template <typename R, int DIM>
+__global__ __launch_bounds__(NT) void kb_decide(const RepDev *__restrict__ reps, long long N, long long nblk,
+                                                long long ngrp, int M)
+{
+    decide_item<R, DIM>(reps[blockIdx.x], N, nblk, ngrp, M);
+}
+
 // event-local stress update of the replicas whose step decided it (stz_local_dev.h): the
 // chosen site of a thermal event or the avalanche list, each with the strain it recorded
 template <typename R, int DIM>
-__global__ __launch_bounds__(NT_LOCAL) void kb_local(const RepDev *__restrict__ reps, int fp64)
+__device__ __forceinline__ void local_item(const RepDev &d, int fp64, int lb)
 {
     constexpr int C = Comp<DIM>::C;
     __shared__ LocalSrc<R> src[STZ_LOCAL_MAXSRC];
     __shared__ double wred[NW_LOCAL * 6];
     __shared__ double uni[6];
-    const RepDev &d = reps[blockIdx.y];
     const RepState *st = d.st;
     double *res = d.res;
     if (st->halt || reinterpret_cast<const int *>(res + RES_LOCAL)[0] != 1) return;
@@ -291,8 +313,14 @@ __global__ __launch_bounds__(NT_LOCAL) void kb_local(const RepDev *__restrict__ 
         for (int c = 0; c < 6; ++c) uni[c] = fp64 ? u[c] - st->usum[c] : u[c];
     }
     __syncthreads();
-    local_rows<R, DIM>(g, (const R *)d.kresp, src, ns, uni, (R *)d.sigma, blockIdx.x, d.spart,
+    local_rows<R, DIM>(g, (const R *)d.kresp, src, ns, uni, (R *)d.sigma, lb, d.spart,
                        reinterpret_cast<int *>(res + 15), wred);
+}
+
+template <typename R, int DIM>
+__global__ __launch_bounds__(NT_LOCAL) void kb_local(const RepDev *__restrict__ reps, int fp64)
+{
+    local_item<R, DIM>(reps[blockIdx.y], fp64, blockIdx.x);
 }
 
 // Athermal compaction and transforms: one workgroup per chunk of NT voxel blocks.  The
@@ -303,17 +331,14 @@ __global__ __launch_bounds__(NT_LOCAL) void kb_local(const RepDev *__restrict__ 
 // the min-Q field is read only where an avalanche is.  The partials socp[blk] are written
 // for those blocks only; kb_finish sums the others as the +0.0 they are.
 template <typename R, int DIM>
-__global__ __launch_bounds__(NT) void kb_athermal(const RepDev *__restrict__ reps, long long N, int M,
-                                                  long long nblk)
+__device__ __forceinline__ void athermal_item(const RepDev &d, long long N, int M, long long nblk, long long c)
 {
     __shared__ int sw[NW];
     __shared__ int s_nb;
     __shared__ long long s_blk[NT], s_off[NT];
-    const RepDev &d = reps[blockIdx.y];
     const RepState *st = d.st;
     const double *res = d.res;
     if (st->halt || reinterpret_cast<const int *>(res + RES_KIND)[0] != KIND_ATHERMAL) return;
-    const long long c = blockIdx.x;
     if (d.ccnt[c] == 0) return;                 // uniform over the workgroup
     const int tid = threadIdx.x, lane = tid % WAVE, w = tid / WAVE;
     const long long blk = c * NT + tid;
@@ -370,6 +395,13 @@ __global__ __launch_bounds__(NT) void kb_athermal(const RepDev *__restrict__ rep
         athermal_block_f<R, DIM>(a, b, v < N ? athermal_flag(a, v) : 0, par, s_off[i]);
         par ^= 1;
     }
+}
+
+template <typename R, int DIM>
+__global__ __launch_bounds__(NT) void kb_athermal(const RepDev *__restrict__ reps, long long N, int M,
+                                                  long long nblk)
+{
+    athermal_item<R, DIM>(reps[blockIdx.y], N, M, nblk, blockIdx.x);
 }
 
 // --- host-loop arithmetic of the reference, on the device, with the host's operation order
@@ -523,13 +555,13 @@ __device__ __forceinline__ void publish_halt(const RepDev &d, int h)
 // the host loop's bookkeeping for one replica, after the device decided and ran its step:
 // <sigma> from the solve partials and the |DWork| sum (all threads, the per-call path's
 // reductions), then the scalar updates on a private copy of the run state (one thread)
-template <int DIM>
-__global__ __launch_bounds__(STZ_NT_MEAN) void kb_finish(const RepDev *__restrict__ reps, int fp64, int nparts,
-                                                         int nparts_local, double invN, long long nblk)
+// NTH: the workgroup size; the reductions are the STZ_NT_MEAN-thread trees whatever it is
+template <int DIM, int NTH>
+__device__ __forceinline__ void finish_item(const RepDev &d, int fp64, int nparts, int nparts_local, double invN,
+                                            long long nblk)
 {
     constexpr int C = Comp<DIM>::C;
     __shared__ double mean[C];
-    const RepDev &d = reps[blockIdx.x];
     RepState *sp = d.st;
     if (sp->halt) return;
     double *res = d.res;
@@ -537,9 +569,14 @@ __global__ __launch_bounds__(STZ_NT_MEAN) void kb_finish(const RepDev *__restric
     const int kind = kindp[0];
     const bool solved = kind == KIND_THERMAL || kind == KIND_ATHERMAL;
     const int local = reinterpret_cast<const int *>(res + RES_LOCAL)[0];
-    if (solved) mean_partials<C>(d.spart, local ? nparts_local : nparts, invN, mean);
     double soc = 0.0;
-    if (kind == KIND_ATHERMAL) soc = ordered_sum_flagged<STZ_NT_MEAN>(d.socp, d.acnt, d.ccnt, nblk);
+    if constexpr (NTH == STZ_NT_MEAN) {
+        if (solved) mean_partials<C>(d.spart, local ? nparts_local : nparts, invN, mean);
+        if (kind == KIND_ATHERMAL) soc = ordered_sum_flagged<STZ_NT_MEAN>(d.socp, d.acnt, d.ccnt, nblk);
+    } else {
+        if (solved) mean_partials_n<C, NTH>(d.spart, local ? nparts_local : nparts, invN, mean);
+        if (kind == KIND_ATHERMAL) soc = ordered_sum_flagged_n<STZ_NT_MEAN, NTH>(d.socp, d.acnt, d.ccnt, nblk);
+    }
     if (threadIdx.x != 0) return;
     // the solve gates of this step are spent (every thread has read them: the reductions
     // above end in barriers, and without a reduction no other thread uses them)
@@ -644,7 +681,280 @@ __global__ __launch_bounds__(STZ_NT_MEAN) void kb_finish(const RepDev *__restric
     if (s.halt != HALT_NONE) publish_halt(d, s.halt);
     kindp[0] = KIND_NONE;
 }
+
+template <int DIM>
+__global__ __launch_bounds__(STZ_NT_MEAN) void kb_finish(const RepDev *__restrict__ reps, int fp64, int nparts,
+                                                         int nparts_local, double invN, long long nblk)
+{
+    finish_item<DIM, STZ_NT_MEAN>(reps[blockIdx.x], fp64, nparts, nparts_local, invN, nblk);
+}
 #pragma clang fp contract(on)
+
+// ---------------------------------------------------------------------------------
+// Persistent step: one launch runs the whole step of every replica of a plan (all shape
+// classes).  The step's work is a list of tickets in dependency order -- phase by phase
+// (rates, group sums, chunk totals, decision, athermal pass, local update, bookkeeping),
+// within a phase the classes largest first, replica by replica, a few items per ticket.
+// The workgroup of a ticket waits until its replica has completed every item of the
+// earlier phases (completion counts per replica), runs the items with the item functions
+// of the kernels above, and adds them to the count.  Every item a ticket waits for carries
+// a smaller ticket: the step progresses without any co-residency promise; the wait is
+// bounded in time all the same (abort flag, then every workgroup finishes at once).  The
+// replica's last item (its bookkeeping) clears its counts for the next launch.
+// Results equal the kernel sequence bit for bit: same item code, same reductions
+// (kb_finish's 1024-thread trees computed by 256 threads).
+// ---------------------------------------------------------------------------------
+constexpr int PH_RATES = 0, PH_GROUP = 1, PH_CHUNKS = 2, PH_DECIDE = 3, PH_ATH = 4, PH_LOCAL = 5, PH_FINISH = 6;
+constexpr int NPH = 7;
+constexpr int PMAXCLS = 8;
+// sync words: the ticket counter, the exit count, the abort flag, then one count per replica
+// sync words: the abort flag, then per replica STZ_PERSIST_SLOTS completion counts, each
+// on a 128-byte line of its own (a ticket adds its items to slot ticket % slots, a waiter
+// sums the slots), so the completions of one phase do not all update one address
+// (measured: the same step time with one slot; the write-backs dominate)
+#ifndef STZ_PERSIST_SLOTS
+#define STZ_PERSIST_SLOTS 16
+#endif
+constexpr int PSYNC_ABORT = 0, PSYNC_REP = 32, PSYNC_LINE = 32;
+constexpr int PSYNC_PER_REP = STZ_PERSIST_SLOTS * PSYNC_LINE;
+__host__ __device__ constexpr long long psync_words(int nrep) { return PSYNC_REP + (long long)nrep * PSYNC_PER_REP; }
+
+struct PClass {
+    long long N, nblk, ngrp;
+    long long items[NPH];         // items of each phase, per replica
+    long long before[NPH];        // items of the earlier phases, per replica
+    int per[NPH];                 // items per ticket
+    int M, rep0, nparts_local;    // rep0: the first replica's completion count
+    double invN;
+};
+struct PSeg {
+    unsigned t0, t1;              // tickets [t0, t1) of one (phase, class)
+    unsigned units;               // tickets per replica
+    int ph, cls;
+};
+// The launch block in device memory: this header, then the RepDev of every replica (class
+// by class).  All of it is read through the kernel's one __restrict__ argument, so the
+// compiler takes the descriptors with scalar loads into scalar registers, as the kernels
+// above do with their reps argument.
+struct PArgs {
+    PClass cls[PMAXCLS];
+    PSeg seg[NPH * PMAXCLS];
+    int nseg, fp64, nrep;
+    unsigned total;               // tickets of one step
+    unsigned *sync;               // [psync_words(nrep)] words, zero between launches
+    unsigned long long spin_ticks;   // wall-clock ticks a dependency wait may last
+};
+
+// Polling of a waiting workgroup backs off to STZ_PERSIST_BACKOFF sleeps of 512 cycles
+// between reads (measured: no change from a one-sleep poll; kept, it costs nothing).
+#ifndef STZ_PERSIST_BACKOFF
+#define STZ_PERSIST_BACKOFF 16
+#endif
+#ifndef STZ_PERSIST_ACQ_ALL
+#define STZ_PERSIST_ACQ_ALL 0      // every wave runs an agent-scope acquire after the wait
+#endif
+// Items per ticket.  Every ticket (workgroup) ends in an agent-scope release, i.e. a
+// write-back of its XCD's L2 (8 L2s that are not coherent with each other), where the
+// kernel sequence pays one per kernel boundary: fewer, larger tickets cost less, up to
+// where the last tickets of a phase leave compute units idle.  Measured per step at
+// 512x512x256 mixed (kernel sequence 7.9 ms): 4 rate blocks and 2 row groups per ticket
+// 26.4 ms, 16 and 8 13.0 ms, 32 and 16 11.1 ms; 64 replicas of config 4 (7.35 ms): 47.2,
+// 26.7, 21.4 ms.
+#ifndef STZ_PERSIST_RPER
+#define STZ_PERSIST_RPER 32        // rate blocks per ticket
+#endif
+#ifndef STZ_PERSIST_LPER
+#define STZ_PERSIST_LPER 16        // local-update row groups per ticket
+#endif
+
+// ticket t (< total) -> phase, class, replica within the class, items [i0, i1) of the phase
+template <typename PA>
+__host__ __device__ __forceinline__ void persist_decode(PA *P, unsigned t, int &ph, int &ci, int &r, long long &i0,
+                                                        long long &i1)
+{
+    int si = 0;
+    while (t >= P->seg[si].t1) ++si;
+    ph = P->seg[si].ph;
+    ci = P->seg[si].cls;
+    const unsigned units = P->seg[si].units;
+    const unsigned u = t - P->seg[si].t0;
+    r = (int)(u / units);
+    const long long per = P->cls[ci].per[ph], n = P->cls[ci].items[ph];
+    i0 = (long long)(u % units) * per;
+    i1 = i0 + per < n ? i0 + per : n;
+}
+
+// the shape of one class as the table needs it
+struct PShape {
+    long long N, nblk, ngrp, nlb;
+    int M, nrep;
+};
+
+// Builds the ticket table of classes shp[0..n) (already in launch order, largest first):
+// items per phase and replica, the segments of (phase, class), the replicas' completion
+// counts.  false when it does not fit (more than PMAXCLS classes, counts beyond 31 bits).
+inline bool persist_table(const PShape *shp, int n, PArgs &A)
+{
+    static const int per[NPH] = {STZ_PERSIST_RPER, 1, 1, 1, 8, STZ_PERSIST_LPER, 1};
+    if (n < 1 || n > PMAXCLS) return false;
+    std::memset(&A, 0, sizeof(A));
+    int nrep = 0;
+    for (int j = 0; j < n; ++j) {
+        const PShape &p = shp[j];
+        PClass &c = A.cls[j];
+        c.N = p.N;
+        c.nblk = p.nblk;
+        c.ngrp = p.ngrp;
+        c.M = p.M;
+        c.rep0 = nrep;
+        c.nparts_local = (int)p.nlb;
+        c.invN = 1.0 / (double)p.N;
+        c.items[PH_RATES] = p.nblk;
+        c.items[PH_GROUP] = (p.ngrp + STZ_GRP_PER_BLOCK - 1) / STZ_GRP_PER_BLOCK;
+        c.items[PH_CHUNKS] = std::max((p.ngrp + NT - 1) / NT, (p.nblk + NT - 1) / NT);
+        c.items[PH_DECIDE] = 1;
+        c.items[PH_ATH] = (p.nblk + NT - 1) / NT;
+        c.items[PH_LOCAL] = p.nlb;
+        c.items[PH_FINISH] = 1;
+        long long b = 0;
+        for (int ph = 0; ph < NPH; ++ph) {
+            c.per[ph] = per[ph];
+            c.before[ph] = b;
+            b += c.items[ph];
+        }
+        if (b >= (1ll << 31)) return false;      // beyond the 32-bit completion counts
+        nrep += p.nrep;
+    }
+    long long t = 0;
+    int ns = 0;
+    for (int ph = 0; ph < NPH; ++ph)
+        for (int j = 0; j < n; ++j) {
+            const PClass &c = A.cls[j];
+            const long long units = (c.items[ph] + c.per[ph] - 1) / c.per[ph];
+            if (units == 0) continue;
+            PSeg &g = A.seg[ns++];
+            g.t0 = (unsigned)t;
+            t += units * (long long)shp[j].nrep;
+            g.t1 = (unsigned)t;
+            g.units = (unsigned)units;
+            g.ph = ph;
+            g.cls = j;
+        }
+    if (t >= (1ll << 31)) return false;          // one workgroup per ticket: a grid dimension
+    A.nseg = ns;
+    A.total = (unsigned)t;
+    A.nrep = nrep;
+    return true;
+}
+
+// the replica's completed items: the sum of its slots
+__device__ __forceinline__ unsigned count_sum(const unsigned *cnt)
+{
+    unsigned s = 0;
+#pragma unroll
+    for (int i = 0; i < STZ_PERSIST_SLOTS; ++i)
+        s += __hip_atomic_load(cnt + i * PSYNC_LINE, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return s;
+}
+
+// thread 0: waits until the replica's completed items reach need, then acquires (the
+// writes of those items are visible to the workgroup); false when the step aborts
+__device__ __forceinline__ bool dep_wait(const unsigned *cnt, unsigned need, unsigned *abortf, unsigned long long limit)
+{
+    if (count_sum(cnt) < need) {
+        const unsigned long long t0 = wall_clock64();
+        unsigned nap = 1;
+        for (;;) {
+            for (unsigned j = 0; j < nap; ++j) __builtin_amdgcn_s_sleep(8);
+            if (nap < STZ_PERSIST_BACKOFF) nap *= 2;
+            if (count_sum(cnt) >= need) break;
+            if (__hip_atomic_load(abortf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return false;
+            if (wall_clock64() - t0 > limit) {
+                __hip_atomic_store(abortf, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                return false;
+            }
+        }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    return true;
+}
+
+// The launch block is read-only while a launch runs, so it is read through the constant
+// address space: uniform loads of it (and of the descriptors) are scalar loads into
+// scalar registers, which the compiler cannot otherwise prove safe next to the phases'
+// stores.
+typedef __attribute__((address_space(4))) const PArgs CPArgs;
+typedef __attribute__((address_space(4))) const RepDev CRepDev;
+
+// One workgroup per ticket, ticket = block index.  Workgroups are dispatched in block-index
+// order, so every ticket a workgroup waits for belongs to a workgroup dispatched before it,
+// which runs to its end: the waits cannot close a cycle, whatever the number of resident
+// workgroups.  (Tickets taken from an atomic counter instead, with an exit counter for
+// the reset, cost about 260 ns per ticket in same-address atomics: 31 ms per step at
+// 512x512x256 against 8 ms for the kernel sequence.  A loop over tickets in fewer
+// workgroups let the compiler hoist every phase's loop-invariant values out of the loop
+// and keep them all live: 250 vector registers, one wave per SIMD.)  Three waves per SIMD
+// at least: the FP64 3-D variant then fits in 165 registers without spilling, like the
+// FP64 rate pass and local update alone; the others reach four.
+template <typename R, int DIM>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void kp_step(const PArgs *__restrict__ Pg)
+{
+    __shared__ int s_ok;
+    __shared__ double ws[NW];
+    __shared__ int wsi[NW];
+    const CPArgs *P = (const CPArgs *)(unsigned long long)Pg;
+    unsigned *sync = P->sync;
+    const unsigned t = blockIdx.x;
+    int ph, ci, r;
+    long long i0, i1;
+    persist_decode(P, t, ph, ci, r, i0, i1);
+    const auto &k = P->cls[ci];
+    unsigned *cnt = sync + PSYNC_REP + (long long)(k.rep0 + r) * PSYNC_PER_REP;
+    if (threadIdx.x == 0) {
+        // after an abort no workgroup runs its items any more (they would read the
+        // unfinished results of the items that skipped theirs)
+        const bool ok = k.before[ph] == 0 || dep_wait(cnt, (unsigned)k.before[ph], sync + PSYNC_ABORT, P->spin_ticks);
+        s_ok = ok && __hip_atomic_load(sync + PSYNC_ABORT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u;
+    }
+    __syncthreads();
+    // thread 0's acquire invalidated the compute unit's caches, which every wave of the
+    // workgroup reads through
+    if (STZ_PERSIST_ACQ_ALL) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    if (__builtin_amdgcn_readfirstlane(s_ok)) {
+        const RepDev &d = *(const RepDev *)((const CRepDev *)(unsigned long long)(Pg + 1) + k.rep0 + r);
+        const long long N = k.N, nblk = k.nblk, ngrp = k.ngrp;
+        const int M = k.M, fp64 = P->fp64;
+        auto items = [&](auto f) {
+            for (long long i = i0; i < i1; ++i) {
+                f(i);
+                __syncthreads();
+            }
+        };
+        switch (ph) {
+        case PH_RATES: items([&](long long i) { rates_item<R, DIM>(d, N, nblk, M, i); }); break;
+        case PH_GROUP: items([&](long long i) { group_item(d, (long long)M * nblk, ngrp, i, ws); }); break;
+        case PH_CHUNKS: items([&](long long i) { chunks_item(d, ngrp, nblk, i, ws, wsi); }); break;
+        case PH_DECIDE: decide_item<R, DIM>(d, N, nblk, ngrp, M); break;
+        case PH_ATH: items([&](long long i) { athermal_item<R, DIM>(d, N, M, nblk, i); }); break;
+        case PH_LOCAL: items([&](long long i) { local_item<R, DIM>(d, fp64, (int)i); }); break;
+        default: finish_item<DIM, NT>(d, fp64, 0, k.nparts_local, k.invN, nblk); break;
+        }
+    }
+    // every wave's writes reach agent scope before the count moves
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        if (ph == PH_FINISH) {
+            // the replica's last item: every other item of it has completed and no workgroup
+            // of this launch reads its counts any more; they start from zero next launch
+            for (int i = 0; i < STZ_PERSIST_SLOTS; ++i)
+                __hip_atomic_store(cnt + i * PSYNC_LINE, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            __hip_atomic_fetch_add(cnt + (t % STZ_PERSIST_SLOTS) * PSYNC_LINE, (unsigned)(i1 - i0), __ATOMIC_RELEASE,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+}
 
 }  // namespace
 
@@ -711,6 +1021,15 @@ struct StzPlan {
     std::vector<hipEvent_t> eva, evb;
     std::vector<hipEvent_t> evc;      // progress markers of the chunked launches
     int timed = 0;
+    // persistent step (kp_step): requested, and in use for the current replicas
+    int persist = 0;
+    bool pactive = false;
+    unsigned *psync = nullptr;        // device sync words
+    long long psync_n = 0;
+    PArgs *pargs = nullptr;           // device launch block: PArgs + the replicas' RepDev
+    int pargs_rep = 0;                // replicas it has room for
+    PArgs hargs;                      // host copy of the header
+    std::vector<RepDev> hreps;        // host copy of the descriptors, class by class
 };
 
 namespace {
@@ -1005,6 +1324,18 @@ int launch_class(StzPlan *pl, StzClass &k, hipEvent_t ea, hipEvent_t eb)
 // (a captured graph holds them as parallel branches).
 int launch_step(StzPlan *pl, hipEvent_t ea = nullptr, hipEvent_t eb = nullptr)
 {
+    if (pl->pactive) {                  // the whole step of every class in one launch
+        const stz_ctx *p = pl->cls[0].reps[0];
+        const dim3 g(pl->hargs.total), b(NT);       // one workgroup per ticket
+        if (p->prec == STZ_FP64) {
+            if (p->dim == 2) hipLaunchKernelGGL((kp_step<double, 2>), g, b, 0, pl->stream, pl->pargs);
+            else hipLaunchKernelGGL((kp_step<double, 3>), g, b, 0, pl->stream, pl->pargs);
+        } else {
+            if (p->dim == 2) hipLaunchKernelGGL((kp_step<float, 2>), g, b, 0, pl->stream, pl->pargs);
+            else hipLaunchKernelGGL((kp_step<float, 3>), g, b, 0, pl->stream, pl->pargs);
+        }
+        return hipGetLastError() == hipSuccess ? STZ_OK : STZ_E_HIP;
+    }
     if (pl->fork) {
         if (hipEventRecord(pl->fork, pl->stream) != hipSuccess) return STZ_E_HIP;
         for (StzClass &k : pl->cls)
@@ -1037,6 +1368,8 @@ void plan_free(StzPlan *pl)
     if (pl->exec) (void)hipGraphExecDestroy(pl->exec);
     if (pl->graph) (void)hipGraphDestroy(pl->graph);
     if (pl->stream) (void)hipStreamSynchronize(pl->stream);
+    if (pl->psync) (void)hipFree(pl->psync);
+    if (pl->pargs) (void)hipFree(pl->pargs);
     for (StzClass &k : pl->cls) {
         if (k.stream) (void)hipStreamSynchronize(k.stream);
         if (k.d_reps) (void)hipFree(k.d_reps);
@@ -1094,6 +1427,73 @@ int plan_make(stz_ctx *const *ctxs, int n, StzPlan **out)
     return STZ_OK;
 }
 
+// why a plan cannot take the persistent step, or nullptr
+const char *persist_refusal(const std::vector<stz_ctx *> &all, size_t nclasses)
+{
+    if (nclasses > (size_t)PMAXCLS) return "the persistent step takes at most 8 shape classes";
+    for (const stz_ctx *c : all) {
+        if (c->prec != all[0]->prec || c->dim != all[0]->dim)
+            return "the persistent step takes replicas of one precision and dimension";
+        if (c->solver_mode != STZ_SOLVER_LOCAL)
+            return "the persistent step takes replicas on the event-local stress update";
+    }
+    return nullptr;
+}
+
+// The launch description of the persistent step (tickets, phases, classes); pactive
+// stays false when the plan's replicas do not allow it.  Items per ticket: 4 rate blocks
+// (a ticket and a count update per 4 x 256 voxels), 8 athermal chunks (most return at
+// once), 2 local-update row groups, one item of the other phases.
+int plan_persist_setup(StzPlan *pl)
+{
+    pl->pactive = false;
+    if (!pl->persist || persist_refusal(pl->all, pl->cls.size())) return STZ_OK;
+    for (const StzClass &k : pl->cls)
+        if (k.fft || !k.local) return STZ_OK;
+    stz_ctx *o = pl->owner;
+    std::vector<int> order(pl->cls.size());
+    for (size_t i = 0; i < order.size(); ++i) order[i] = (int)i;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int a, int b) { return pl->cls[(size_t)a].reps[0]->N > pl->cls[(size_t)b].reps[0]->N; });
+    std::vector<PShape> shp;
+    pl->hreps.clear();
+    for (int j : order) {
+        const StzClass &k = pl->cls[(size_t)j];
+        const stz_ctx *p = k.reps[0];
+        for (const stz_ctx *c : k.reps) pl->hreps.push_back(c->rs->hdev);
+        shp.push_back(PShape{p->N, p->nblk, p->ngrp, stz_local_blocks(p), p->M, (int)k.reps.size()});
+    }
+    PArgs &A = pl->hargs;
+    if (!persist_table(shp.data(), (int)shp.size(), A)) return STZ_OK;
+    const int nrep = A.nrep;
+    A.fp64 = o->prec == STZ_FP64;
+    if (pl->psync_n < psync_words(nrep)) {
+        if (pl->psync) (void)hipFree(pl->psync);
+        pl->psync = nullptr;
+        pl->psync_n = 0;
+        RS_HIP(o, hipMalloc((void **)&pl->psync, sizeof(unsigned) * (size_t)psync_words(nrep)));
+        RS_HIP(o, hipMemset(pl->psync, 0, sizeof(unsigned) * (size_t)psync_words(nrep)));
+        pl->psync_n = psync_words(nrep);
+    }
+    A.sync = pl->psync;
+    int dev = 0, khz = 0;
+    RS_HIP(o, hipGetDevice(&dev));
+    RS_HIP(o, hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+    A.spin_ticks = (unsigned long long)std::max(khz, 1000) * 1000ull * 5ull;   // 5 s
+    if (pl->pargs_rep < nrep) {
+        if (pl->pargs) (void)hipFree(pl->pargs);
+        pl->pargs = nullptr;
+        pl->pargs_rep = 0;
+        RS_HIP(o, hipMalloc((void **)&pl->pargs, sizeof(PArgs) + sizeof(RepDev) * (size_t)nrep));
+        pl->pargs_rep = nrep;
+    }
+    RS_HIP(o, hipMemcpyAsync(pl->pargs, &A, sizeof(PArgs), hipMemcpyHostToDevice, pl->stream));
+    RS_HIP(o, hipMemcpyAsync(pl->pargs + 1, pl->hreps.data(), sizeof(RepDev) * (size_t)nrep, hipMemcpyHostToDevice,
+                             pl->stream));
+    pl->pactive = true;
+    return STZ_OK;
+}
+
 // uploads the replica descriptors (they change when a lookup window is rebuilt)
 int plan_upload(StzPlan *pl)
 {
@@ -1119,6 +1519,10 @@ int plan_upload(StzPlan *pl)
         RS_HIP(o, hipMemcpyAsync(k.d_reps, d.data(), d.size() * sizeof(RepDev), hipMemcpyHostToDevice, pl->stream));
         RS_HIP(o, hipMemcpyAsync(k.d_solve, s.data(), s.size() * sizeof(SolveRep), hipMemcpyHostToDevice, pl->stream));
     }
+    const bool was = pl->pactive;
+    int rc = plan_persist_setup(pl);
+    if (rc) return rc;
+    if (was != pl->pactive) pl->graph_epoch = -1;   // a captured graph holds the other launch sequence
     RS_HIP(o, hipStreamSynchronize(pl->stream));
     return STZ_OK;
 }
@@ -1176,7 +1580,7 @@ int plan_launch_steps(StzPlan *pl, int k)
         if (rc) return rc;
     }
     stz_ctx *p = pl->all[0];
-    const bool timed = pl->use_graph == 0 && pl->all.size() == 1 && p->timing;
+    const bool timed = pl->use_graph == 0 && pl->all.size() == 1 && p->timing && !pl->pactive;
     if (timed && pl->eva.size() < (size_t)k) {
         while (pl->eva.size() < (size_t)k) {
             hipEvent_t a, b;
@@ -1281,7 +1685,20 @@ int stz_plan_run(StzPlan *pl, int nsteps, stz_step_result *const *results, int *
             stz_ctx *c = pl->all[i];
             RS_HIP(o, hipMemcpyAsync(&c->rs->hst, c->rs->st, sizeof(RepState), hipMemcpyDeviceToHost, pl->stream));
         }
+        unsigned aborted = 0;
+        if (pl->pactive)
+            RS_HIP(o, hipMemcpyAsync(&aborted, pl->psync + PSYNC_ABORT, sizeof(unsigned), hipMemcpyDeviceToHost, pl->stream));
         RS_HIP(o, hipStreamSynchronize(pl->stream));
+        if (aborted) {
+            // a dependency wait of the persistent step ran out of time: the replicas' state
+            // is not trustworthy any more
+            RS_HIP(o, hipMemset(pl->psync, 0, sizeof(unsigned) * (size_t)pl->psync_n));
+            for (size_t i = 0; i < n; ++i) {
+                pl->all[i]->err = "persistent step: a dependency wait exceeded its time limit";
+                rcs[i] = STZ_E_HIP;
+            }
+            return STZ_E_HIP;
+        }
         if (pl->timed > 0) {
             // only the steps that recorded ran the rate pass: the launches after a halt
             // (print, budget, window, error) return at once and are left out of the mean
@@ -1518,6 +1935,22 @@ int stz_run_steps(stz_ctx *ctx, int nsteps, stz_step_result *results, int *done)
     return rc;
 }
 
+int stz_run_persistent(stz_ctx *ctx, int on)
+{
+    if (!ctx) return STZ_E_ARG;
+    StzPlan *pl = stz_self_plan(ctx);
+    if (!pl) return STZ_E_HIP;
+    if (on) {
+        if (const char *why = persist_refusal(pl->all, pl->cls.size())) {
+            ctx->err = why;
+            return STZ_E_UNSUPPORTED;
+        }
+    }
+    pl->persist = on != 0;
+    pl->graph_epoch = -1;
+    return STZ_OK;
+}
+
 int stz_run_stop_at_print(stz_ctx *ctx, int on)
 {
     if (!ctx) return STZ_E_ARG;
@@ -1579,6 +2012,90 @@ int stz_ensemble_run(stz_ensemble *e, int nsteps, stz_step_result *const *result
             return STZ_E_UNSUPPORTED;
         }
     return stz_plan_run(e->pl, nsteps, results, done, rcs, 0);
+}
+
+// CPU check of the persistent step's ticket table (no device call): builds the table of
+// ncls classes (shape[c] = {dim, n0, n1, n2, M, precision, replicas}, in launch order),
+// decodes every ticket as the kernel does, and counts violations: a ticket outside its
+// segment's ranges, an item covered other than exactly once, or a ticket that a ticket of
+// an earlier phase of its replica follows.  out = {tickets, items, violations}.
+int stz_persist_probe(int ncls, const int *shape, long long *out)
+{
+    if (ncls < 1 || ncls > PMAXCLS || !shape || !out) return STZ_E_ARG;
+    std::vector<PShape> shp;
+    for (int c = 0; c < ncls; ++c) {
+        const int *q = shape + 7 * c;
+        const int dim = q[0], n[3] = {q[1], q[2], dim == 2 ? 1 : q[3]}, M = q[4], prec = q[5], nrep = q[6];
+        if ((dim != 2 && dim != 3) || n[0] < 1 || n[1] < 1 || n[2] < 1 || M < 1 || M > STZ_MAXM || nrep < 1)
+            return STZ_E_ARG;
+        PShape p;
+        p.N = (long long)n[0] * n[1] * n[2];
+        p.nblk = (p.N + STZ_BLK - 1) / STZ_BLK;
+        p.ngrp = ((long long)M * p.nblk + STZ_GRP - 1) / STZ_GRP;
+        const LocalGeom g = local_geom(dim, n, local_row_voxels(prec == STZ_MIXED));
+        p.nlb = (g.items + g.ipw - 1) / g.ipw;
+        p.M = M;
+        p.nrep = nrep;
+        shp.push_back(p);
+    }
+    PArgs A;
+    if (!persist_table(shp.data(), ncls, A)) return STZ_E_TOO_LARGE;
+    long long bad = 0, items = 0;
+    // coverage per (replica, phase, item), first/last ticket per (replica, phase)
+    std::vector<std::vector<std::vector<unsigned char>>> cov((size_t)A.nrep);
+    std::vector<long long> first((size_t)A.nrep * NPH, -1), last((size_t)A.nrep * NPH, -1);
+    for (int c = 0; c < ncls; ++c)
+        for (int r = 0; r < shp[(size_t)c].nrep; ++r) {
+            auto &v = cov[(size_t)(A.cls[c].rep0 + r)];
+            v.resize(NPH);
+            for (int ph = 0; ph < NPH; ++ph) {
+                v[(size_t)ph].assign((size_t)A.cls[c].items[ph], 0);
+                items += A.cls[c].items[ph];
+            }
+        }
+    for (unsigned t = 0; t < A.total; ++t) {
+        int ph, ci, r;
+        long long i0, i1;
+        persist_decode(&A, t, ph, ci, r, i0, i1);
+        if (ph < 0 || ph >= NPH || ci < 0 || ci >= ncls || r < 0 || r >= shp[(size_t)ci].nrep || i0 < 0 || i0 >= i1 ||
+            i1 > A.cls[ci].items[ph]) {
+            ++bad;
+            continue;
+        }
+        const int gr = A.cls[ci].rep0 + r;
+        for (long long i = i0; i < i1; ++i) cov[(size_t)gr][(size_t)ph][(size_t)i]++;
+        long long &f = first[(size_t)gr * NPH + ph], &l = last[(size_t)gr * NPH + ph];
+        if (f < 0) f = t;
+        l = t;
+    }
+    for (int gr = 0; gr < A.nrep; ++gr)
+        for (int ph = 0; ph < NPH; ++ph) {
+            for (unsigned char x : cov[(size_t)gr][(size_t)ph]) bad += x != 1;
+            // every ticket of an earlier phase comes first (the waits only look back)
+            for (int q = 0; q < ph; ++q)
+                if (last[(size_t)gr * NPH + q] >= 0 && first[(size_t)gr * NPH + ph] >= 0 &&
+                    last[(size_t)gr * NPH + q] > first[(size_t)gr * NPH + ph])
+                    ++bad;
+        }
+    out[0] = A.total;
+    out[1] = items;
+    out[2] = bad;
+    return STZ_OK;
+}
+
+int stz_ensemble_persistent(stz_ensemble *e, int on)
+{
+    if (!e) return STZ_E_ARG;
+    StzPlan *pl = e->pl;
+    if (on) {
+        if (const char *why = persist_refusal(pl->all, pl->cls.size())) {
+            pl->owner->err = why;
+            return STZ_E_UNSUPPORTED;
+        }
+    }
+    pl->persist = on != 0;
+    pl->graph_epoch = -1;
+    return STZ_OK;
 }
 
 void stz_ensemble_destroy(stz_ensemble *e)

@@ -133,6 +133,8 @@ def lib():
         L.stz_ensemble_run.argtypes = [vp, C.c_int, P(P(StepResult)), P(C.c_int), P(C.c_int)]
         L.stz_ensemble_destroy.argtypes = [vp]
         L.stz_ensemble_destroy.restype = None
+        L.stz_ensemble_persistent.argtypes = [vp, C.c_int]
+        L.stz_run_persistent.argtypes = [vp, C.c_int]
         L.stz_mt_window_probe.argtypes = [C.c_ulong, C.c_longlong, C.c_longlong, C.c_longlong, C.c_int, C.c_int,
                                           dp, dp, P(C.c_longlong), P(C.c_longlong)]
         L.stz_copy_probe.argtypes = [C.c_longlong, C.c_int, dp, dp]
@@ -366,6 +368,10 @@ class Stz:
     def run_config(self, batch=0, use_graph=-1, window_words=0):
         self._check(lib().stz_run_config(self._h, int(batch), int(use_graph), int(window_words)))
 
+    def persistent(self, on=True):
+        """Resident steps as one persistent launch per step (stz_run_persistent)."""
+        self._check(lib().stz_run_persistent(self._h, int(bool(on))))
+
     def autotune(self, reps=10, force=False):
         """Times the solve kernels' tile variants and keeps the fastest; returns
         (choices, mean ms of the chosen variants)."""
@@ -449,7 +455,7 @@ class Stz:
 class Ensemble:
     """Replicas advanced together by the batched resident step (stz_ensemble_*)."""
 
-    def __init__(self, members, batch=0, use_graph=-1):
+    def __init__(self, members, batch=0, use_graph=-1, persistent=False):
         self.members = list(members)
         hs = (C.c_void_p * len(self.members))(*[m._h.value for m in self.members])
         self._h = C.c_void_p()
@@ -460,6 +466,14 @@ class Ensemble:
             rc = lib().stz_ensemble_config(self._h, int(batch), int(use_graph))
             if rc:
                 raise StzError(rc, "stz_ensemble_config")
+        if persistent:
+            self.persistent(True)
+
+    def persistent(self, on=True):
+        """One persistent launch per step for all replicas (stz_ensemble_persistent)."""
+        rc = lib().stz_ensemble_persistent(self._h, int(bool(on)))
+        if rc:
+            raise StzError(rc, lib().stz_last_error(self.members[0]._h).decode())
 
     def run(self, n, keep=True):
         """Advances every replica by up to n steps; returns (rcs, [results per replica])."""
