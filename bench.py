@@ -45,7 +45,10 @@ def parse():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--legs", default="c5,cpu,c2,c3,c4",
-                    help="sections to run: c5 (headline), cpu (with c1_64), c2, c3, c4")
+                    help="sections to run: c5 (headline), cpu (with c1_64), c2, c3, c4; c3long (not in the "
+                         "default: config 3 at its length, 10^5 events, about 10 minutes)")
+    ap.add_argument("--c3long-events", type=int, default=100000)
+    ap.add_argument("--c3long-fp64-events", type=int, default=20000)
     ap.add_argument("--c2-steps", type=int, default=1000)
     ap.add_argument("--grid", default="128,128,128")
     ap.add_argument("--no-mixed", action="store_true")
@@ -344,6 +347,9 @@ def main():
         la = argparse.Namespace(**dict(vars(a), large="256", steps=500, warmup=20, timing_steps=200))
         out["c3_256"] = large_leg(la, mg)
         print(f"# c3_256: {out['c3_256']['value']} events/s mixed", file=sys.stderr, flush=True)
+    if "c3long" in sections:
+        out["c3_long"] = c3long_leg(a, mg)
+        print(f"# c3_long: {out['c3_long']}", file=sys.stderr, flush=True)
     if "c4" in sections:
         out["c4_ensemble"] = ensemble_leg(a, mg)
         print(f"# c4_ensemble: {out['c4_ensemble']}", file=sys.stderr, flush=True)
@@ -807,6 +813,70 @@ def large_leg(a, mg):
                      "launches_timed": mx["kb_rates"]["launches_timed"]},
     }
     return out
+
+
+def c3long_leg(a, mg):
+    """Config 3 at its length: the 256^3 mixed run (Q0 ~ N(2.0, 0.25), event-local update
+    with a full re-solve every 1024 updates) to 10^5 thermal events, beside the FP64 run of
+    the same device-generated initial state (seed 1234) for the first 2x10^4 events.
+    Reported: events/s of each (run time only), the steps and events over which the two
+    event sequences agree and the stress-strain-curve error over them, the mixed run's
+    same-state field error (its stress field against the FP64 solve of its own eigenstrain
+    field and mean strain) every 10^4 events, and past the point where the sequences part,
+    the largest difference of the two stress-strain curves compared at equal strain."""
+    la = argparse.Namespace(**dict(vars(a), large="256"))
+    kw = large_kw(la, mg)
+
+    def run(name, prec, events, check_every):
+        g = large_context(la, mg, name, prec, "local")
+        curve, checks = [], []
+        nev = nst = 0
+        t_run = 0.0
+        nxt = check_every
+        while nev < events:
+            dt, kinds, _, _, recs = run_steps(g, 1000, curve)
+            t_run += dt
+            nst += len(recs)
+            nev += int(kinds[1])
+            if check_every and nev >= nxt:
+                checks.append({"events": nev, "steps": nst, "stress_rel_err_same_state": same_state_err(g, kw, mg)})
+                nxt += check_every
+            print(f"# c3long {name}: {nst} steps, {nev} events, {t_run:.1f}s" +
+                  (f", same-state err {checks[-1]['stress_rel_err_same_state']:.2e}" if checks and checks[-1]["steps"] == nst
+                   else ""), file=sys.stderr, flush=True)
+            if len(recs) < 1000:
+                break
+        out = {"events": nev, "steps": nst, "seconds": round(t_run, 2), "value": round(nev / t_run, 2),
+               "unit": "events/s", "strain_end": curve[-1][4] if curve else None}
+        g.close()
+        return out, curve, checks
+
+    fp, cd, _ = run("fp64", mg.FP64, a.c3long_fp64_events, 0)
+    mx, cm, checks = run("mixed", mg.MIXED, a.c3long_events, 10000)
+    n_same = same_prefix(cd, cm)
+    # past the common prefix: both curves on the strain range they share
+    ed = np.array([x[4] for x in cd])
+    sd = np.array([x[5] for x in cd])
+    em = np.array([x[4] for x in cm])
+    sm = np.array([x[5] for x in cm])
+    hi = min(ed.max(), em.max())
+    lo = ed[n_same] if n_same < len(ed) else hi
+    curve_diff = None
+    if hi > lo:
+        od, om = np.argsort(ed, kind="stable"), np.argsort(em, kind="stable")
+        grid = np.linspace(lo, hi, 2000)
+        a1 = np.interp(grid, ed[od], sd[od])
+        a2 = np.interp(grid, em[om], sm[om])
+        curve_diff = float(np.max(np.abs(a2 - a1)) / max(np.max(np.abs(a1)), 1e-30))
+    return {"grid": [256, 256, 256], "precision": "mixed vs fp64", "solver": "local (full re-solve every 1024)",
+            "mixed": mx, "fp64": fp,
+            "steps_identical": n_same, "events_identical": int(sum(1 for x in cm[:n_same] if x[0] == 1)),
+            "ss_curve_rel_err_identical": curve_err(cd, cm, n_same),
+            "same_state_checks": checks,
+            "ss_curve_rel_diff_after_parting": curve_diff,
+            "sample": "device-generated initial state (seed 1234), Q0 ~ N(2.0, 0.25); the FP64 run to "
+                      f"{a.c3long_fp64_events} events, the mixed run to {a.c3long_events}; a same-state check "
+                      "folds the pending uniform increments into the mixed field first (a zero-site local update)"}
 
 
 def ensemble_plan(nrep):
