@@ -21,6 +21,28 @@ __global__ void k_keep(const double *__restrict__ sig, R *__restrict__ K, int C,
     K[i] = (R)sig[(long long)c * N + j];
 }
 
+// the compact table (stz_local_dev.h, Resp) from the S full responses F[k][c][i] (FP64):
+// A_ik = t_i : sigma_k with the symmetric pair averaged, and the responses of the
+// reference normal component
+template <typename R, int DIM>
+__global__ void k_compact(const double *__restrict__ F, R *__restrict__ K, int H)
+{
+    using Q = Resp<DIM>;
+    const long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= H) return;
+    auto f = [&](int k, int c) { return F[((long long)k * Q::C + c) * H + j]; };
+    // row i of A applied to the response of k: normal rows e_i - e_Z, shear rows 2 e_i
+    auto a = [&](int i, int k) {
+        return i < Q::NN ? f(k, Q::comp(i)) - f(k, Q::Z) : 2.0 * f(k, Q::comp(i));
+    };
+#pragma unroll
+    for (int i = 0; i < Q::S; ++i)
+#pragma unroll
+        for (int k = i; k < Q::S; ++k) K[(long long)Q::sym(i, k) * H + j] = (R)(0.5 * (a(i, k) + a(k, i)));
+#pragma unroll
+    for (int k = 0; k < Q::S; ++k) K[(long long)(Q::NSYM + k) * H + j] = (R)f(k, Q::Z);
+}
+
 struct Uni {
     double u[6];
 };
@@ -83,14 +105,19 @@ LocalGeom geom_of(const stz_ctx *ctx) { return local_geom(ctx->dim, ctx->n, loca
         }                                                                          \
     } while (0)
 
+// planes of the compact response table of a context
+int resp_planes(const stz_ctx *ctx) { return ctx->dim == 2 ? Resp<2>::P : Resp<3>::P; }
+
 // S FP64 impulse solves on a solver-only shadow context of the same grid and constants
-// (the stress-solve kernels read only the arrays allocated here), kept in K
+// (the stress-solve kernels read only the arrays allocated here), their half-space
+// responses kept in FP64 and compacted into K (resp_planes x H values)
 int build_response(stz_ctx *ctx, void *K)
 {
     const LocalGeom g = geom_of(ctx);
     const int C = ctx->C;
     const size_t N = (size_t)ctx->N, rows = (size_t)ctx->n[1] * ctx->n[2];
     stz_ctx *sh = new stz_ctx();
+    double *full = nullptr;       // [S][C][H] FP64 responses
     int rc = STZ_OK;
     sh->p = ctx->p;
     sh->p.precision = STZ_FP64;
@@ -112,6 +139,7 @@ int build_response(stz_ctx *ctx, void *K)
     LHIP(hipMalloc(&sh->spec, (size_t)C * rows * sh->hp * 16));
     LHIP(hipMalloc((void **)&sh->spart, rows * C * 8));
     LHIP(hipMalloc((void **)&sh->res, 64 * 8));
+    LHIP(hipMalloc((void **)&full, (size_t)ctx->S * C * g.H * 8));
     LHIP(hipMemsetAsync(sh->res, 0, 64 * 8, ctx->stream));
     rc = stz_fft_setup(sh);
     if (rc) {
@@ -132,12 +160,20 @@ int build_response(stz_ctx *ctx, void *K)
         }
         const long long ch = (long long)C * g.H;
         const unsigned grid = (unsigned)((ch + 255) / 256);
-        if (ctx->prec == STZ_FP64)
-            hipLaunchKernelGGL(k_keep<double>, dim3(grid), dim3(256), 0, ctx->stream, (const double *)sh->sigma,
-                               (double *)K + (size_t)k * ch, C, g.N, g.H);
+        hipLaunchKernelGGL(k_keep<double>, dim3(grid), dim3(256), 0, ctx->stream, (const double *)sh->sigma,
+                           full + (size_t)k * ch, C, g.N, g.H);
+        LHIP(hipGetLastError());
+    }
+    {
+        const unsigned grid = (unsigned)((g.H + 255) / 256);
+        if (ctx->prec == STZ_FP64 && ctx->dim == 2)
+            hipLaunchKernelGGL((k_compact<double, 2>), dim3(grid), dim3(256), 0, ctx->stream, full, (double *)K, g.H);
+        else if (ctx->prec == STZ_FP64)
+            hipLaunchKernelGGL((k_compact<double, 3>), dim3(grid), dim3(256), 0, ctx->stream, full, (double *)K, g.H);
+        else if (ctx->dim == 2)
+            hipLaunchKernelGGL((k_compact<float, 2>), dim3(grid), dim3(256), 0, ctx->stream, full, (float *)K, g.H);
         else
-            hipLaunchKernelGGL(k_keep<float>, dim3(grid), dim3(256), 0, ctx->stream, (const double *)sh->sigma,
-                               (float *)K + (size_t)k * ch, C, g.N, g.H);
+            hipLaunchKernelGGL((k_compact<float, 3>), dim3(grid), dim3(256), 0, ctx->stream, full, (float *)K, g.H);
         LHIP(hipGetLastError());
     }
     LHIP(hipStreamSynchronize(ctx->stream));
@@ -151,7 +187,7 @@ int build_response(stz_ctx *ctx, void *K)
     }
 done:
     (void)hipStreamSynchronize(ctx->stream);
-    void *ptrs[] = {sh->eps0, sh->sigma, sh->spec, sh->spart, sh->res};
+    void *ptrs[] = {sh->eps0, sh->sigma, sh->spec, sh->spart, sh->res, full};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
     stz_fft_free(sh);
@@ -178,7 +214,7 @@ int stz_local_prepare(stz_ctx *ctx)
         ctx->kresp = it->second.k;
     } else {
         void *K = nullptr;
-        if (hipMalloc(&K, (size_t)ctx->S * ctx->C * g.H * ctx->rsz) != hipSuccess) {
+        if (hipMalloc(&K, (size_t)resp_planes(ctx) * g.H * ctx->rsz) != hipSuccess) {
             ctx->err = "hipMalloc of the impulse response failed";
             return STZ_E_HIP;
         }

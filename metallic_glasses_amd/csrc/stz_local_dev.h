@@ -89,6 +89,51 @@ __device__ __forceinline__ void load_src(const LocalGeom &g, const R *deps_last,
     for (int k = 0; k < S; ++k) s.d[k] = deps_last[(long long)k * g.N + v];
 }
 
+// The response table, compacted by the elastic operator's major symmetry.  K_kc(d), the
+// stress component c at offset d of a unit stored strain component k (a traceless basis
+// tensor t_k), is G(d) : t_k with G the operator from eigenstrain to stress, which has
+// G_ijkl = G_klij.  So the S x S matrix A_ik = t_i : G : t_k (the response's work-like
+// contraction, shear entries weighted 2) is symmetric, and K is fixed by A and by one
+// normal stress row: the table stores, per offset, the S(S+1)/2 entries of A (upper
+// triangle, row by row) and the S responses of the reference normal component Z (zz in
+// 3-D, yy in 2-D): 20 values instead of 30 in 3-D, 5 instead of 6 in 2-D.  With u = A d and
+// z = sum_k d_k K_kZ for a strain increment d: sigma_Z = z, the other normal components
+// u_i + z (A's normal rows are t_i = e_i - e_Z), the shear components u_i / 2.
+template <int DIM>
+struct Resp {
+    static constexpr int S = DIM == 2 ? 2 : 5, C = DIM == 2 ? 3 : 6;
+    static constexpr int NSYM = S * (S + 1) / 2, P = NSYM + S;   // planes of the table
+    static constexpr int Z = DIM == 2 ? 1 : 2;                   // reference normal component
+    static constexpr int NN = DIM == 2 ? 1 : 2;                  // normal rows of A
+    // the stress component of row i of A
+    __host__ __device__ static constexpr int comp(int i) { return DIM == 2 ? (i == 0 ? 0 : 2) : (i < 2 ? i : i + 1); }
+    // plane of A_ik
+    __host__ __device__ static constexpr int sym(int i, int k)
+    {
+        return i <= k ? i * S - i * (i - 1) / 2 + (k - i) : k * S - k * (k - 1) / 2 + (i - k);
+    }
+};
+
+// da[c] = sum_k d_k K_kc from the P table values kv of one offset
+template <typename R, int DIM>
+__device__ __forceinline__ void resp_apply(const R *kv, const R *d, R *da)
+{
+    using Q = Resp<DIM>;
+    R u[Q::S];
+#pragma unroll
+    for (int i = 0; i < Q::S; ++i) {
+        u[i] = (R)0;
+#pragma unroll
+        for (int k = 0; k < Q::S; ++k) u[i] = u[i] + d[k] * kv[Q::sym(i, k)];
+    }
+    R z = (R)0;
+#pragma unroll
+    for (int k = 0; k < Q::S; ++k) z = z + d[k] * kv[Q::NSYM + k];
+    da[Q::Z] = z;
+#pragma unroll
+    for (int i = 0; i < Q::S; ++i) da[Q::comp(i)] = i < Q::NN ? u[i] + z : u[i] * (R)0.5;
+}
+
 // index of offset d (each coordinate in [0, n)) in the half-space table: d itself when
 // its slowest coordinate is at most n2/2, else -d
 __device__ __forceinline__ int half_index(const LocalGeom &g, int d0, int d1, int d2)
@@ -155,7 +200,7 @@ template <typename R, int DIM>
 __device__ __forceinline__ void local_sum(const LocalGeom &g, const R *__restrict__ K, const LocalSrc<R> *src, int ns,
                                           int x0, int x1, int x2, R *out)
 {
-    constexpr int C = DIM == 2 ? 3 : 6, S = DIM == 2 ? 2 : 5;
+    constexpr int C = DIM == 2 ? 3 : 6;
 #pragma unroll
     for (int c = 0; c < C; ++c) out[c] = (R)0;
     for (int j = 0; j < ns; ++j) {
@@ -164,15 +209,12 @@ __device__ __forceinline__ void local_sum(const LocalGeom &g, const R *__restric
         if (d1 < 0) d1 += g.n[1];
         if (d2 < 0) d2 += g.n[2];
         const int hj = half_index(g, d0, d1, d2);
+        R kv[Resp<DIM>::P], da[C];   // the table values of the offset, loaded before their use
 #pragma unroll
-        for (int k = 0; k < S; ++k) {
-            R kv[C];   // the C response values of strain component k, loaded before their use
+        for (int i = 0; i < Resp<DIM>::P; ++i) kv[i] = gload(K + (long long)i * g.H + hj);
+        resp_apply<R, DIM>(kv, src[j].d, da);
 #pragma unroll
-            for (int c = 0; c < C; ++c) kv[c] = gload(K + ((long long)k * C + c) * g.H + hj);
-            const R dk = src[j].d[k];
-#pragma unroll
-            for (int c = 0; c < C; ++c) out[c] = out[c] + dk * kv[c];
-        }
+        for (int c = 0; c < C; ++c) out[c] = out[c] + da[c];
     }
 }
 
@@ -185,7 +227,7 @@ template <typename R, int DIM>
 __device__ void local_rows(const LocalGeom &g, const R *__restrict__ K, const LocalSrc<R> *src, int ns,
                            const double *uni, R *sigma, int w, double *spart, int *nanflag, double *wred)
 {
-    constexpr int C = DIM == 2 ? 3 : 6, S = DIM == 2 ? 2 : 5;
+    constexpr int C = DIM == 2 ? 3 : 6;
     const int tid = threadIdx.x, lane = tid % 64, wv = tid / 64;
     const int n0 = g.n[0], n1 = g.n[1], n2 = g.n[2];
     double acc[C];
@@ -222,19 +264,12 @@ __device__ void local_rows(const LocalGeom &g, const R *__restrict__ K, const Lo
             // the response row first, then the stress: all S*C + 2*C loads are issued
             // before the first use
             const int hj = (d2 * n1 + d1) * n0 + d0;
-            R kv[S * C];
+            R kv[Resp<DIM>::P];
 #pragma unroll
-            for (int i = 0; i < S * C; ++i) kv[i] = gld_stream<4>(K + (long long)i * g.H + hj);
+            for (int i = 0; i < Resp<DIM>::P; ++i) kv[i] = gld_stream<4>(K + (long long)i * g.H + hj);
             local_load<R, C>(g, sigma, va, sa);
             if (pair) local_load<R, C>(g, sigma, vb, sb);
-#pragma unroll
-            for (int c = 0; c < C; ++c) da[c] = (R)0;
-#pragma unroll
-            for (int k = 0; k < S; ++k) {
-                const R dk = s0.d[k];
-#pragma unroll
-                for (int c = 0; c < C; ++c) da[c] = da[c] + dk * kv[k * C + c];
-            }
+            resp_apply<R, DIM>(kv, s0.d, da);
 #pragma unroll
             for (int c = 0; c < C; ++c) db[c] = da[c];
         } else {
