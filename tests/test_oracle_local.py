@@ -163,3 +163,43 @@ def test_local_update_detects_transposed_axes():
     assert np.max(np.abs(good - exact)) <= 1e-12 * np.max(np.abs(exact))
     bad = 0.07 * np.roll(K[4], shift=(3, 1, 11), axis=(1, 2, 3))   # (x, y, z) on (z, y, x)
     assert np.max(np.abs(bad - exact)) > 1e-3 * np.max(np.abs(exact))
+
+
+@pytest.mark.parametrize("dims", GRIDS + [(32, 16), (16, 16, 16)])
+def test_response_table_compaction(dims):
+    """The compacted table of the local update (20 planes instead of 30 in 3-D, 5 instead
+    of 6 in 2-D) rests on A = T^T M K being symmetric at every offset (T: the stored
+    strain components' basis tensors, M: weight 2 on shear components) -- the elastic
+    operator's major symmetry -- and on K being recovered from A and the responses of the
+    reference normal stress (zz in 3-D, yy in 2-D): sigma_Z = z, the other normal
+    components u_i + z, the shear components u_i / 2 with u = A d.  A planted slip (the
+    wrong reference row) is caught."""
+    o = po.Oracle(**_kw(dims))
+    C, S, dim = o.C, o.S, o.params.dim
+    K = np.stack([r.reshape(C, -1) for r in _responses(o)], axis=1)      # [C][S][N]
+    T = np.zeros((C, S))
+    for k in range(S):
+        for lab, val in BASIS[dim][k].items():
+            T[COMP[dim].index(lab), k] = val
+    M = np.diag([1.0, 1.0, 2.0] if C == 3 else [1.0, 1.0, 1.0, 2.0, 2.0, 2.0])
+    A = np.einsum("ck,cd,dsn->ksn", T, M, K)
+    scale = np.abs(K).max()
+    assert np.abs(A - A.transpose(1, 0, 2)).max() <= 1e-15 * scale
+    A = 0.5 * (A + A.transpose(1, 0, 2))
+
+    zc = 1 if dim == 2 else 2                                   # the reference normal row
+    rows = [0, 2] if dim == 2 else [0, 1, 3, 4, 5]              # stress component of each row of A
+    nn = 1 if dim == 2 else 2                                   # normal rows of A
+
+    def rebuild(zsrc, half=0.5):
+        z = K[zsrc]                                             # [S][N]
+        Kr = np.zeros_like(K)
+        for i, c in enumerate(rows):
+            Kr[c] = A[i] + z if i < nn else half * A[i]
+        Kr[zc] = z
+        return Kr
+
+    assert np.abs(rebuild(zc) - K).max() <= 1e-14 * scale
+    # planted slips: the reference row taken from xx, the shear weight not undone
+    assert np.abs(rebuild(0) - K).max() > 1e-3 * scale
+    assert np.abs(rebuild(zc, half=1.0) - K).max() > 1e-3 * scale
