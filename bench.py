@@ -45,8 +45,12 @@ def parse():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--legs", default="c5,cpu,c2,c3,c4",
-                    help="sections to run: c5 (headline), cpu (with c1_64), c2, c3, c4; c3long (not in the "
-                         "default: config 3 at its length, 10^5 events, about 10 minutes)")
+                    help="sections to run: c5 (headline), cpu (with c1_64), c2, c3, c4; not in the default, "
+                         "the configurations at their length: c2long (10^5 events, plain FP64 and FP64 "
+                         "local, about 4 minutes), c3long (10^5 events, about 10 minutes), c4long (2x10^4 "
+                         "events per replica, about 5 minutes)")
+    ap.add_argument("--c2long-events", type=int, default=100000)
+    ap.add_argument("--c4long-events", type=int, default=20000)
     ap.add_argument("--c3long-events", type=int, default=100000)
     ap.add_argument("--c3long-fp64-events", type=int, default=20000)
     ap.add_argument("--c2-steps", type=int, default=1000)
@@ -347,6 +351,12 @@ def main():
         la = argparse.Namespace(**dict(vars(a), large="256", steps=500, warmup=20, timing_steps=200))
         out["c3_256"] = large_leg(la, mg)
         print(f"# c3_256: {out['c3_256']['value']} events/s mixed", file=sys.stderr, flush=True)
+    if "c2long" in sections:
+        out["c2_long"] = c2long_leg(a, mg)
+        print(f"# c2_long: {out['c2_long']}", file=sys.stderr, flush=True)
+    if "c4long" in sections:
+        out["c4_long"] = c4long_leg(a, mg)
+        print(f"# c4_long: {out['c4_long']}", file=sys.stderr, flush=True)
     if "c3long" in sections:
         out["c3_long"] = c3long_leg(a, mg)
         print(f"# c3_long: {out['c3_long']}", file=sys.stderr, flush=True)
@@ -813,6 +823,103 @@ def large_leg(a, mg):
                      "launches_timed": mx["kb_rates"]["launches_timed"]},
     }
     return out
+
+
+GOLDEN_128 = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests", "golden", "trace_shear3d_128.npz")
+
+
+def c2long_leg(a, mg):
+    """Config 2 at its length: the 128^3 FP64 run (seed 1234, the reference's MT19937 initial
+    state and stream) to 10^5 thermal events, once with the full FFT solve after every
+    transform (the plain GPU path) and once with the event-local update.  Each run's steps
+    are checked against the oracle's committed trace of the same run (tests/golden): the
+    steps and thermal events over which kind, site, mode and avalanche size agree, and the
+    largest relative difference of the loaded-component mean stress over them."""
+    trace = None
+    kw = mg.config_3d(n=(128, 128, 128), seed=a.seed, q0_std=0.2)
+    pp, es = 5, 1.0
+    if os.path.exists(GOLDEN_128):
+        import ast
+        trace = np.load(GOLDEN_128)
+        kw = dict(ast.literal_eval(str(trace["params"])))
+        pp, es = int(trace["printperiod"]), float(trace["endstrain"])
+    legs = {}
+    for name, solver in (("fp64_fft", "fft"), ("fp64_local", "local")):
+        g = mg.Stz(**kw)
+        g.set_print(pp, es)
+        g.autotune(reps=5)
+        g.set_solver(SOLVERS[solver], a.resync, a.local_max)
+        curve = []
+        nev = nst = 0
+        t_run = 0.0
+        while nev < a.c2long_events:
+            dt, kinds, _, _, recs = run_steps(g, 2000, curve)
+            t_run += dt
+            nst += len(recs)
+            nev += int(kinds[1])
+            print(f"# c2long {name}: {nst} steps, {nev} events, {t_run:.1f}s", file=sys.stderr, flush=True)
+            if len(recs) < 2000:
+                break
+        rec = {"events": nev, "steps": nst, "seconds": round(t_run, 2), "value": round(nev / t_run, 2),
+               "unit": "events/s"}
+        if trace is not None:
+            n = min(len(trace["kind"]), len(curve))
+            same = 0
+            for i in range(n):
+                x = curve[i]
+                k = int(trace["kind"][i])
+                if x[0] != k or x[3] != int(trace["ntrans"][i]) or \
+                        (k == 1 and (x[1] != int(trace["site"][i]) or x[2] != int(trace["mode"][i]))):
+                    break
+                same += 1
+            st = np.array([x[5] for x in curve[:same]])
+            ref = trace["stress"][:same]
+            rec["oracle_trace"] = {
+                "steps": int(len(trace["kind"])), "events": int(np.sum(trace["kind"] == 1)),
+                "steps_identical": same, "events_identical": int(np.sum(trace["kind"][:same] == 1)),
+                "stress_rel_err_max": float(np.max(np.abs(st - ref) / np.maximum(np.abs(ref), 1e-3))) if same else None}
+        legs[name] = rec
+        g.close()
+    return {"grid": [128, 128, 128], "precision": "fp64", "legs": legs,
+            "sample": "seed 1234 initial state and stream of the reference; the resident step, 2000 steps per "
+                      "call, run time only; kinds, sites, modes and avalanche sizes against the oracle's trace"}
+
+
+def c4long_leg(a, mg):
+    """Config 4 at its length: the 64-replica mixed ensemble (event-local update, 4 steps per
+    captured graph) until every replica has 2x10^4 thermal events; replica-events/s over the
+    whole run, the spread of events per replica, and every replica's same-state field error
+    at the end (its stress field against the FP64 solve of its own eigenstrain field)."""
+    plan = ensemble_plan(a.ensemble_replicas)
+    kws = [mg.config_3d(n=g, seed=seed, q0_std=sd, temperature=T, precision=mg.MIXED) for g, seed, T, sd in plan]
+    members = make_contexts(mg, kws)
+    for m in members:
+        m.set_print(5, 1.0)
+        m.autotune(reps=5)
+        m.set_solver(SOLVERS["local"], a.resync, a.local_max)
+    ens = mg.Ensemble(members, batch=64, use_graph=ENSEMBLE_GRAPH_STEPS)
+    ev = np.zeros(len(members), dtype=np.int64)
+    nst = 0
+    t_run = 0.0
+    while ev.min() < a.c4long_events and nst < 200000:
+        t0 = time.perf_counter()
+        rcs, outs = ens.run(1000)
+        t_run += time.perf_counter() - t0
+        nst += 1000
+        ev += np.array([count_events(o) for o in outs])
+        print(f"# c4long: {nst} steps per replica, events per replica {ev.min()}..{ev.max()}, {t_run:.1f}s",
+              file=sys.stderr, flush=True)
+        if any(rc != 0 for rc in rcs):
+            break
+    errs = [same_state_err(m, kw, mg) for m, kw in zip(members, kws)]
+    ens.close()
+    for m in members:
+        m.close()
+    return {"replicas": len(plan), "grids": [list(g) for g in ENSEMBLE_GRIDS], "precision": "mixed",
+            "solver": "local", "steps_per_replica": nst, "events_per_replica_min": int(ev.min()),
+            "events_per_replica_max": int(ev.max()), "replica_events": int(ev.sum()), "seconds": round(t_run, 2),
+            "value": round(float(ev.sum()) / t_run, 2), "unit": "replica-events/s",
+            "stress_rel_err_same_state_max": max(errs), "stress_rel_err_same_state_median": float(np.median(errs))}
 
 
 def c3long_leg(a, mg):
