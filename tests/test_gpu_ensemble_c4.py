@@ -95,3 +95,27 @@ def test_c4_ensemble_equals_single_runs_and_tracks_fp64():
     assert worst_state < 1e-4
     tens.close()
     ens.close()
+
+
+def test_c4_persistent_step_equals_kernel_sequence():
+    """The same 64 mixed replicas as one persistent launch per step (replayed 4 steps per
+    graph) against the kernel sequence: every record and every final stress and
+    eigenstrain field equal bit for bit."""
+    kws = [dict(k, precision=mg.MIXED) for k in _plan()]
+    runs = []
+    for persistent in (False, True):
+        print("c4 persistent" if persistent else "c4 kernel sequence", flush=True)
+        members = _make(kws)
+        ens = mg.Ensemble(members, batch=64, use_graph=4, persistent=persistent)
+        rcs, outs = ens.run(60)
+        runs.append((members, rcs, outs, ens))
+    (ma, rca, oa, ea), (mb, rcb, ob, eb) = runs
+    assert rca == rcb
+    for i in range(64):
+        assert len(oa[i]) == len(ob[i]) == 60, i
+        for s, (x, y) in enumerate(zip(oa[i], ob[i])):
+            assert _key(x) == _key(y) and list(x.stress) == list(y.stress) and x.totime == y.totime, (i, s)
+        for f in (mg.F_SIGMA, mg.F_EPS0):
+            assert np.array_equal(ma[i].download(f), mb[i].download(f)), (i, f)
+    ea.close()
+    eb.close()
