@@ -51,6 +51,9 @@ def parse():
                          "events per replica, about 5 minutes)")
     ap.add_argument("--c2long-events", type=int, default=100000)
     ap.add_argument("--c4long-events", type=int, default=20000)
+    ap.add_argument("--c4long-ensemble", choices=["mixed_local", "fp64_plain", "fp64_local"], default="mixed_local",
+                    help="c4long: the ensemble run at its length (fp64_plain: FP64 with the FFT solve after every "
+                         "transform, about 11 minutes)")
     ap.add_argument("--c3long-events", type=int, default=100000)
     ap.add_argument("--c3long-fp64-events", type=int, default=20000)
     ap.add_argument("--c2-steps", type=int, default=1000)
@@ -886,17 +889,22 @@ def c2long_leg(a, mg):
 
 
 def c4long_leg(a, mg):
-    """Config 4 at its length: the 64-replica mixed ensemble (event-local update, 4 steps per
-    captured graph) until every replica has 2x10^4 thermal events; replica-events/s over the
-    whole run, the spread of events per replica, and every replica's same-state field error
-    at the end (its stress field against the FP64 solve of its own eigenstrain field)."""
+    """Config 4 at its length: the 64-replica ensemble (by default mixed with the event-local
+    update; --c4long-ensemble fp64_plain: FP64 with the FFT solve after every transform), 4
+    steps per captured graph, until every replica has 2x10^4 thermal events; replica-events/s
+    over the whole run, the spread of events per replica, and (mixed) every replica's
+    same-state field error at the end (its stress field against the FP64 solve of its own
+    eigenstrain field)."""
     plan = ensemble_plan(a.ensemble_replicas)
-    kws = [mg.config_3d(n=g, seed=seed, q0_std=sd, temperature=T, precision=mg.MIXED) for g, seed, T, sd in plan]
+    kind = a.c4long_ensemble
+    prec = mg.MIXED if kind == "mixed_local" else mg.FP64
+    solver = "fft" if kind == "fp64_plain" else "local"
+    kws = [mg.config_3d(n=g, seed=seed, q0_std=sd, temperature=T, precision=prec) for g, seed, T, sd in plan]
     members = make_contexts(mg, kws)
     for m in members:
         m.set_print(5, 1.0)
         m.autotune(reps=5)
-        m.set_solver(SOLVERS["local"], a.resync, a.local_max)
+        m.set_solver(SOLVERS[solver], a.resync, a.local_max)
     ens = mg.Ensemble(members, batch=64, use_graph=ENSEMBLE_GRAPH_STEPS)
     ev = np.zeros(len(members), dtype=np.int64)
     nst = 0
@@ -911,15 +919,18 @@ def c4long_leg(a, mg):
               file=sys.stderr, flush=True)
         if any(rc != 0 for rc in rcs):
             break
-    errs = [same_state_err(m, kw, mg) for m, kw in zip(members, kws)]
+    errs = [same_state_err(m, kw, mg) for m, kw in zip(members, kws)] if prec == mg.MIXED else None
     ens.close()
     for m in members:
         m.close()
-    return {"replicas": len(plan), "grids": [list(g) for g in ENSEMBLE_GRIDS], "precision": "mixed",
-            "solver": "local", "steps_per_replica": nst, "events_per_replica_min": int(ev.min()),
-            "events_per_replica_max": int(ev.max()), "replica_events": int(ev.sum()), "seconds": round(t_run, 2),
-            "value": round(float(ev.sum()) / t_run, 2), "unit": "replica-events/s",
-            "stress_rel_err_same_state_max": max(errs), "stress_rel_err_same_state_median": float(np.median(errs))}
+    out = {"ensemble": kind, "replicas": len(plan), "grids": [list(g) for g in ENSEMBLE_GRIDS],
+           "precision": "mixed" if prec == mg.MIXED else "fp64", "solver": solver, "steps_per_replica": nst,
+           "events_per_replica_min": int(ev.min()), "events_per_replica_max": int(ev.max()),
+           "replica_events": int(ev.sum()), "seconds": round(t_run, 2),
+           "value": round(float(ev.sum()) / t_run, 2), "unit": "replica-events/s"}
+    if errs:
+        out.update(stress_rel_err_same_state_max=max(errs), stress_rel_err_same_state_median=float(np.median(errs)))
+    return out
 
 
 def c3long_leg(a, mg):
