@@ -707,7 +707,6 @@ __global__ __launch_bounds__(STZ_NT_MEAN) void kb_finish(const RepDev *__restric
 constexpr int PH_RATES = 0, PH_GROUP = 1, PH_CHUNKS = 2, PH_DECIDE = 3, PH_ATH = 4, PH_LOCAL = 5, PH_FINISH = 6;
 constexpr int NPH = 7;
 constexpr int PMAXCLS = 8;
-// sync words: the ticket counter, the exit count, the abort flag, then one count per replica
 // sync words: the abort flag, then per replica STZ_PERSIST_SLOTS completion counts, each
 // on a 128-byte line of its own (a ticket adds its items to slot ticket % slots, a waiter
 // sums the slots), so the completions of one phase do not all update one address
@@ -749,9 +748,6 @@ struct PArgs {
 // between reads (measured: no change from a one-sleep poll; kept, it costs nothing).
 #ifndef STZ_PERSIST_BACKOFF
 #define STZ_PERSIST_BACKOFF 16
-#endif
-#ifndef STZ_PERSIST_ACQ_ALL
-#define STZ_PERSIST_ACQ_ALL 0      // every wave runs an agent-scope acquire after the wait
 #endif
 // Items per ticket.  Every ticket (workgroup) ends in an agent-scope release, i.e. a
 // write-back of its XCD's L2 (8 L2s that are not coherent with each other), where the
@@ -916,10 +912,9 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void kp
         const bool ok = k.before[ph] == 0 || dep_wait(cnt, (unsigned)k.before[ph], sync + PSYNC_ABORT, P->spin_ticks);
         s_ok = ok && __hip_atomic_load(sync + PSYNC_ABORT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u;
     }
+    // thread 0's acquire invalidated the compute unit's vector caches, which every wave of
+    // the workgroup reads through after the barrier
     __syncthreads();
-    // thread 0's acquire invalidated the compute unit's caches, which every wave of the
-    // workgroup reads through
-    if (STZ_PERSIST_ACQ_ALL) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     if (__builtin_amdgcn_readfirstlane(s_ok)) {
         const RepDev &d = *(const RepDev *)((const CRepDev *)(unsigned long long)(Pg + 1) + k.rep0 + r);
         const long long N = k.N, nblk = k.nblk, ngrp = k.ngrp;
