@@ -370,10 +370,9 @@ def main():
 
 
 def c5_record(a, mg):
-    """The headline: 512x512x256 with the soft inclusion (config 5), mixed against FP64."""
+    """The headline: 512x512x256 with the soft inclusion, mixed against FP64."""
     la = argparse.Namespace(**dict(vars(a), large="512"))
     out = large_leg(la, mg)
-    out["config"]["workload"] = "config 5: " + out["config"]["workload"]
     return out
 
 
@@ -397,7 +396,7 @@ def physical_cores():
 
 
 def c1_leg(a, mg, warm_events=20, gpu_steps=500):
-    """Config 1 (64^3 FP64, seed a.seed, simple shear): the CPU FP64 oracle on the host's
+    """The 64^3 FP64 run (seed a.seed, simple shear): the CPU FP64 oracle on the host's
     threads for about --cpu-seconds from a snapshot taken after `warm_events` thermal
     events, and the GPU FP64 path (event-local and FFT solver) for `gpu_steps` resident
     steps from the same snapshot."""
@@ -457,18 +456,18 @@ def c1_leg(a, mg, warm_events=20, gpu_steps=500):
     phys = physical_cores()
     cpu = {"value": round(cev / cdt, 5), "unit": "events/s", "cores": cores, "kind": "port",
            "host_physical_cores": phys,
-           "sample": f"{csteps} steps ({cev} thermal events) of config 1 (64^3 FP64, M=20, simple shear, "
+           "sample": f"{csteps} steps ({cev} thermal events) of the 64^3 FP64 run (M=20, simple shear, "
                      f"seed {a.seed}) from a snapshot after {nev} thermal events, {cdt:.1f}s on {cores} OpenMP "
                      f"threads (the CPU share of one GPU on this host; {phys} physical cores)",
            "events_matching_gpu": cmatch}
     gpu["gpu_over_cpu_fp64_local"] = round(gpu["fp64_local"]["value"] / cpu["value"], 1)
-    gpu["config"] = "config 1: 64^3 FP64 from the same snapshot as the CPU sample"
+    gpu["config"] = "64^3 FP64 from the same snapshot as the CPU sample"
     print(f"# cpu: {cpu}\n# c1_64 gpu: {gpu}", file=sys.stderr, flush=True)
     return cpu, gpu
 
 
 def c2_leg(a, mg):
-    """128^3 FP64 (config 2) from a snapshot after W' = 100 host-driven thermal events:
+    """128^3 FP64 from a snapshot after W' = 100 host-driven thermal events:
     the event-local and the FFT solver, the same two in mixed precision, kb_rates and the
     five FFT solve kernels against their algorithmic bytes, per-voxel moduli."""
     steps = a.c2_steps
@@ -832,7 +831,7 @@ GOLDEN_128 = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests", "
 
 
 def c2long_leg(a, mg):
-    """Config 2 at its length: the 128^3 FP64 run (seed 1234, the reference's MT19937 initial
+    """The 128^3 FP64 run at its length (seed 1234, the reference's MT19937 initial
     state and stream) to 10^5 thermal events, once with the full FFT solve after every
     transform (the plain GPU path) and once with the event-local update.  Each run's steps
     are checked against the oracle's committed trace of the same run (tests/golden): the
@@ -889,7 +888,7 @@ def c2long_leg(a, mg):
 
 
 def c4long_leg(a, mg):
-    """Config 4 at its length: the 64-replica ensemble (by default mixed with the event-local
+    """The 64-replica ensemble at its length (by default mixed with the event-local
     update; --c4long-ensemble fp64_plain: FP64 with the FFT solve after every transform), 4
     steps per captured graph, until every replica has 2x10^4 thermal events; replica-events/s
     over the whole run, the spread of events per replica, and (mixed) every replica's
@@ -934,7 +933,7 @@ def c4long_leg(a, mg):
 
 
 def c3long_leg(a, mg):
-    """Config 3 at its length: the 256^3 mixed run (Q0 ~ N(2.0, 0.25), event-local update
+    """The 256^3 mixed run at its length (Q0 ~ N(2.0, 0.25), event-local update
     with a full re-solve every 1024 updates) to 10^5 thermal events, beside the FP64 run of
     the same device-generated initial state (seed 1234) for the first 2x10^4 events.
     Reported: events/s of each (run time only), the steps and events over which the two
@@ -1022,7 +1021,7 @@ def count_events(recs):
 
 
 def ensemble_leg(a, mg):
-    """Config 4: 64 replicas (grid r mod 4, seed r, T and barrier std from MT19937(4242)),
+    """The 64-replica ensemble: 64 replicas (grid r mod 4, seed r, T and barrier std from MT19937(4242)),
     batched by the resident ensemble step with the event-local update, in mixed precision
     (the measured value) and in FP64 with the same solver; then the FP64 ensemble with the
     full FFT solve after every transform (the plain FP64 path).  Each is initialised from

@@ -754,8 +754,8 @@ struct PArgs {
 // kernel sequence pays one per kernel boundary: fewer, larger tickets cost less, up to
 // where the last tickets of a phase leave compute units idle.  Measured per step at
 // 512x512x256 mixed (kernel sequence 7.9 ms): 4 rate blocks and 2 row groups per ticket
-// 26.4 ms, 16 and 8 13.0 ms, 32 and 16 11.1 ms; 64 replicas of config 4 (7.35 ms): 47.2,
-// 26.7, 21.4 ms.
+// 26.4 ms, 16 and 8 13.0 ms, 32 and 16 11.1 ms; the 64-replica ensemble of 32^3, 64x64x128,
+// 96^3 and 128^3 grids (7.35 ms): 47.2, 26.7, 21.4 ms.
 #ifndef STZ_PERSIST_RPER
 #define STZ_PERSIST_RPER 32        // rate blocks per ticket
 #endif
