@@ -48,7 +48,8 @@ def parse():
                     help="sections to run: c5 (headline), cpu (with c1_64), c2, c3, c4; not in the default, "
                          "the configurations at their length: c2long (10^5 events, plain FP64 and FP64 "
                          "local, about 4 minutes), c3long (10^5 events, about 10 minutes), c4long (2x10^4 "
-                         "events per replica, about 5 minutes)")
+                         "events per replica, about 5 minutes), c5long (512x512x256 mixed to "
+                         "--c5long-events, about 12 minutes at the default)")
     ap.add_argument("--c2long-events", type=int, default=100000)
     ap.add_argument("--c4long-events", type=int, default=20000)
     ap.add_argument("--c4long-ensemble", choices=["mixed_local", "fp64_plain", "fp64_local"], default="mixed_local",
@@ -56,6 +57,8 @@ def parse():
                          "transform, about 11 minutes)")
     ap.add_argument("--c3long-events", type=int, default=100000)
     ap.add_argument("--c3long-fp64-events", type=int, default=20000)
+    ap.add_argument("--c5long-events", type=int, default=30000)
+    ap.add_argument("--c5long-fp64-events", type=int, default=3000)
     ap.add_argument("--c2-steps", type=int, default=1000)
     ap.add_argument("--grid", default="128,128,128")
     ap.add_argument("--no-mixed", action="store_true")
@@ -363,6 +366,9 @@ def main():
     if "c3long" in sections:
         out["c3_long"] = c3long_leg(a, mg)
         print(f"# c3_long: {out['c3_long']}", file=sys.stderr, flush=True)
+    if "c5long" in sections:
+        out["c5_long"] = c5long_leg(a, mg)
+        print(f"# c5_long: {out['c5_long']}", file=sys.stderr, flush=True)
     if "c4" in sections:
         out["c4_ensemble"] = ensemble_leg(a, mg)
         print(f"# c4_ensemble: {out['c4_ensemble']}", file=sys.stderr, flush=True)
@@ -676,8 +682,9 @@ ENSEMBLE_GRAPH_STEPS = 4          # ensemble steps per captured hipGraph (shape 
 
 
 LARGE = {
-    "256": dict(n=(256, 256, 256), q0_std=0.25, spheres=None),
-    "512": dict(n=(512, 512, 256), q0_std=0.2, spheres=(0.05, 16.0, 1.6, 0.2)),
+    "256": dict(n=(256, 256, 256), q0_std=0.25, spheres=None, desc="Q0 ~ N(2.0, 0.25)"),
+    "512": dict(n=(512, 512, 256), q0_std=0.2, spheres=(0.05, 16.0, 1.6, 0.2),
+                desc="Q0 ~ N(2.0, 0.2) with a 5 % soft inclusion, Q0 ~ N(1.6, 0.2)"),
 }
 
 
@@ -941,7 +948,22 @@ def c3long_leg(a, mg):
     same-state field error (its stress field against the FP64 solve of its own eigenstrain
     field and mean strain) every 10^4 events, and past the point where the sequences part,
     the largest difference of the two stress-strain curves compared at equal strain."""
-    la = argparse.Namespace(**dict(vars(a), large="256"))
+    return long_run_leg(a, mg, "256", a.c3long_events, a.c3long_fp64_events, 10000, "c3long")
+
+
+def c5long_leg(a, mg):
+    """The 512x512x256 soft-inclusion run (the headline workload) over a long window: the
+    mixed run with the event-local update to --c5long-events thermal events (same-state
+    field error every 10^4), beside the FP64 run of the same device-generated initial
+    state for --c5long-fp64-events; reported as for the 256^3 run.  (The full length, a run
+    to 10 % shear strain, is about 10^6 events: 4.5 hours at the mixed rate.)"""
+    return long_run_leg(a, mg, "512", a.c5long_events, a.c5long_fp64_events, 10000, "c5long")
+
+
+def long_run_leg(a, mg, large, events, fp64_events, check_every, tag):
+    """A large grid's mixed run to `events` thermal events beside its FP64 run to
+    `fp64_events` (see c3long_leg)."""
+    la = argparse.Namespace(**dict(vars(a), large=large))
     kw = large_kw(la, mg)
 
     def run(name, prec, events, check_every):
@@ -958,7 +980,7 @@ def c3long_leg(a, mg):
             if check_every and nev >= nxt:
                 checks.append({"events": nev, "steps": nst, "stress_rel_err_same_state": same_state_err(g, kw, mg)})
                 nxt += check_every
-            print(f"# c3long {name}: {nst} steps, {nev} events, {t_run:.1f}s" +
+            print(f"# {tag} {name}: {nst} steps, {nev} events, {t_run:.1f}s" +
                   (f", same-state err {checks[-1]['stress_rel_err_same_state']:.2e}" if checks and checks[-1]["steps"] == nst
                    else ""), file=sys.stderr, flush=True)
             if len(recs) < 1000:
@@ -968,8 +990,8 @@ def c3long_leg(a, mg):
         g.close()
         return out, curve, checks
 
-    fp, cd, _ = run("fp64", mg.FP64, a.c3long_fp64_events, 0)
-    mx, cm, checks = run("mixed", mg.MIXED, a.c3long_events, 10000)
+    fp, cd, _ = run("fp64", mg.FP64, fp64_events, 0)
+    mx, cm, checks = run("mixed", mg.MIXED, events, check_every)
     n_same = same_prefix(cd, cm)
     # past the common prefix: both curves on the strain range they share
     ed = np.array([x[4] for x in cd])
@@ -985,14 +1007,14 @@ def c3long_leg(a, mg):
         a1 = np.interp(grid, ed[od], sd[od])
         a2 = np.interp(grid, em[om], sm[om])
         curve_diff = float(np.max(np.abs(a2 - a1)) / max(np.max(np.abs(a1)), 1e-30))
-    return {"grid": [256, 256, 256], "precision": "mixed vs fp64", "solver": "local (full re-solve every 1024)",
+    return {"grid": list(kw["n"]), "precision": "mixed vs fp64", "solver": "local (full re-solve every 1024)",
             "mixed": mx, "fp64": fp,
             "steps_identical": n_same, "events_identical": int(sum(1 for x in cm[:n_same] if x[0] == 1)),
             "ss_curve_rel_err_identical": curve_err(cd, cm, n_same),
             "same_state_checks": checks,
             "ss_curve_rel_diff_after_parting": curve_diff,
-            "sample": "device-generated initial state (seed 1234), Q0 ~ N(2.0, 0.25); the FP64 run to "
-                      f"{a.c3long_fp64_events} events, the mixed run to {a.c3long_events}; a same-state check "
+            "sample": f"device-generated initial state (seed {a.seed}), {LARGE[large]['desc']}; the FP64 run to "
+                      f"{fp64_events} events, the mixed run to {events}; a same-state check "
                       "folds the pending uniform increments into the mixed field first (a zero-site local update)"}
 
 
