@@ -45,7 +45,7 @@ def _key(r):
     return (r.kind, r.n_transformed) + ((r.site, r.mode) if r.kind == 1 else ())
 
 
-def test_c4_ensemble_equals_single_runs_and_tracks_fp64():
+def test_64_replicas_equal_single_runs_and_track_fp64():
     kws = _plan()
     mixed_kws = [dict(k, precision=mg.MIXED) for k in kws]
     print("c4: 64 mixed replicas", flush=True)
@@ -97,7 +97,7 @@ def test_c4_ensemble_equals_single_runs_and_tracks_fp64():
     ens.close()
 
 
-def test_c4_persistent_step_equals_kernel_sequence():
+def test_64_replicas_persistent_step_equals_kernel_sequence():
     """The same 64 mixed replicas as one persistent launch per step (replayed 4 steps per
     graph) against the kernel sequence: every record and every final stress and
     eigenstrain field equal bit for bit."""
