@@ -18,7 +18,7 @@ import metallic_glasses_amd as mg
 L = C.CDLL(mg.stz.LIB_PATH)
 L.stz_persist_probe.argtypes = [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_longlong)]
 
-C4 = [((128, 128, 128), 16), ((96, 96, 96), 16), ((64, 64, 128), 16), ((32, 32, 32), 16)]
+ENS64 = [((128, 128, 128), 16), ((96, 96, 96), 16), ((64, 64, 128), 16), ((32, 32, 32), 16)]
 
 
 def _probe(classes, M=20, prec=mg.MIXED):
@@ -49,13 +49,13 @@ def _expected_items(n, nrep, M, prec):
 
 
 @pytest.mark.parametrize("classes,prec", [
-    (C4, mg.MIXED), (C4, mg.FP64),
+    (ENS64, mg.MIXED), (ENS64, mg.FP64),
     ([((512, 512, 256), 1)], mg.MIXED),
     ([((256, 256, 256), 1)], mg.FP64),
     ([((512, 512), 1)], mg.FP64),
     ([((16, 16, 16), 3), ((8, 16, 32), 2), ((24, 24, 24), 1)], mg.MIXED),
     ([((7, 5, 3), 1)], mg.FP64),
-], ids=["c4_mixed", "c4_fp64", "c5", "c3", "ref2d", "small", "tiny"])
+], ids=["ens64_mixed", "ens64_fp64", "512x512x256_mixed", "256cube_fp64", "2d_512sq", "small", "tiny"])
 def test_ticket_table_covers_every_item_once_in_dependency_order(classes, prec):
     rc, (tickets, items, bad) = _probe(classes, prec=prec)
     assert rc == 0
