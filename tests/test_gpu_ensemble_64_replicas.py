@@ -48,12 +48,12 @@ def _key(r):
 def test_64_replicas_equal_single_runs_and_track_fp64():
     kws = _plan()
     mixed_kws = [dict(k, precision=mg.MIXED) for k in kws]
-    print("c4: 64 mixed replicas", flush=True)
+    print("64 replicas: 64 mixed replicas", flush=True)
     members = _make(mixed_kws)
     ens = mg.Ensemble(members, batch=64, use_graph=4)
     rcs, outs = ens.run(STEPS)
     assert all(rc == 0 for rc in rcs) and all(len(o) == STEPS for o in outs)
-    print("c4: each replica alone", flush=True)
+    print("64 replicas: each replica alone", flush=True)
     for i in range(0, 64, 16):
         singles = _make(mixed_kws[i:i + 16])
         for j, g in enumerate(singles):
@@ -63,7 +63,7 @@ def test_64_replicas_equal_single_runs_and_track_fp64():
                 assert _key(a) == _key(b) and list(a.stress) == list(b.stress), (i + j, s)
             assert np.array_equal(members[i + j].download(mg.F_SIGMA), g.download(mg.F_SIGMA)), i + j
             g.close()
-    print("c4: FP64 twin ensemble", flush=True)
+    print("64 replicas: FP64 twin ensemble", flush=True)
     twins = _make(kws)
     tens = mg.Ensemble(twins, batch=64, use_graph=4)
     _, outs64 = tens.run(STEPS)
@@ -89,7 +89,7 @@ def test_64_replicas_equal_single_runs_and_track_fp64():
         f.stress_solve()
         a, b = f.download(mg.F_SIGMA), m.download(mg.F_SIGMA)
         worst_state = max(worst_state, float(np.linalg.norm(b - a) / np.linalg.norm(a)))
-    print(f"c4: {parted} of 64 replicas parted from FP64 within {STEPS} steps; worst curve error "
+    print(f"64 replicas: {parted} of 64 replicas parted from FP64 within {STEPS} steps; worst curve error "
           f"{worst_curve:.2e}, worst same-state error {worst_state:.2e}", flush=True)
     assert worst_curve < 1e-4
     assert worst_state < 1e-4
@@ -104,7 +104,7 @@ def test_64_replicas_persistent_step_equals_kernel_sequence():
     kws = [dict(k, precision=mg.MIXED) for k in _plan()]
     runs = []
     for persistent in (False, True):
-        print("c4 persistent" if persistent else "c4 kernel sequence", flush=True)
+        print("64 replicas, persistent" if persistent else "64 replicas, kernel sequence", flush=True)
         members = _make(kws)
         ens = mg.Ensemble(members, batch=64, use_graph=4, persistent=persistent)
         rcs, outs = ens.run(60)
